@@ -1,0 +1,209 @@
+"""Classic tablebase builder: the reference's Python build surface
+(engine_core/BookBuilder.py) over the HIP engine.
+
+``start_build(pattern, target, pathname)`` keeps the reference's signature,
+file set and error behaviour: it writes ``pathname + "config.txt"``, runs the
+whole forward/backward build on the GPU, writes ``<pathname><i>.book`` for
+every layer plus ``<pathname>stats.txt``, returns True, and raises on
+failure.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+
+from . import native
+from .config import SingletonConfig
+from .patterns import build_params, pattern_catalog, steps_and_docheck
+
+_RATE_DTYPES = {"float32": np.float32, "float64": np.float64, "uint32": np.uint32, "uint64": np.uint64}
+
+
+def save_config_to_txt(output_path: str) -> None:
+    """engine_core/BookBuilder.py:130-146: the option snapshot of a build."""
+    keys = [
+        "compress",
+        "optimal_branch_only",
+        "compress_temp_files",
+        "advanced_algo",
+        "direct_io",
+        "direct_io_queue_depth",
+        "direct_io_chunk_mib",
+        "deletion_threshold",
+        "4_spawn_rate",
+        "success_rate_dtype",
+    ]
+    cfg = SingletonConfig().config
+    with open(output_path, "w", encoding="utf-8") as fh:
+        for key in keys:
+            fh.write(f"{key}: {str(cfg.get(key, '?'))}\n")
+
+
+class _Args:
+    """ctypes argument block kept alive for the duration of a native call."""
+
+    def __init__(self, seeds, masks, shifts, symm, target, steps, docheck, pathname, is_free,
+                 spawn_rate4, dtype, write_files, keep_raw, device, radix_bits):
+        self.seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+        self.masks = np.ascontiguousarray(np.asarray(list(masks) or [0], dtype=np.uint64))
+        self.shifts = np.ascontiguousarray(np.asarray(list(shifts) or [0], dtype=np.uint8))
+        self.path = (pathname or "").encode()
+        self.dtype = str(dtype).encode()
+        self.pattern = native.Pattern(
+            self.masks.ctypes.data_as(native.u64p), len(masks),
+            self.shifts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(shifts), int(symm))
+        self.options = native.Options()
+        o = self.options
+        o.target = int(target)
+        o.steps = int(steps)
+        o.docheck_step = int(docheck)
+        o.pathname = self.path
+        o.is_free = int(bool(is_free))
+        o.spawn_rate4 = float(spawn_rate4)
+        o.success_rate_dtype = self.dtype
+        o.write_files = int(bool(write_files))
+        o.keep_raw = int(bool(keep_raw))
+        o.device = int(device)
+        o.radix_bits = int(radix_bits)
+
+
+class BuildResult:
+    """Solved layers of a build kept resident in HBM (egtb_result)."""
+
+    def __init__(self, handle: int, dtype: str):
+        self._h = ctypes.c_void_p(handle)
+        self.dtype = dtype
+        self.steps = native.lib().egtb_result_steps(self._h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def layer_info(self, i: int):
+        L = native.lib()
+        v = [ctypes.c_uint64() for _ in range(4)]
+        native.check(L.egtb_result_layer_info(self._h, i, *[ctypes.byref(x) for x in v]))
+        return tuple(int(x.value) for x in v)  # (n_raw, n_live, c2, c4)
+
+    def raw(self, i: int) -> np.ndarray:
+        L = native.lib()
+        n = ctypes.c_size_t()
+        native.check(L.egtb_result_raw(self._h, i, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value, dtype=np.uint64)
+        native.check(L.egtb_result_raw(self._h, i, out.ctypes.data_as(native.u64p), out.size,
+                                       ctypes.byref(n)))
+        return out
+
+    def book(self, i: int):
+        L = native.lib()
+        n = ctypes.c_size_t()
+        native.check(L.egtb_result_book(self._h, i, None, None, 0, ctypes.byref(n)))
+        keys = np.empty(n.value, dtype=np.uint64)
+        rates = np.empty(n.value, dtype=_RATE_DTYPES[self.dtype])
+        native.check(L.egtb_result_book(self._h, i, keys.ctypes.data_as(native.u64p),
+                                        rates.ctypes.data_as(ctypes.c_void_p), keys.size, ctypes.byref(n)))
+        return keys, rates
+
+    def timing(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        native.check(native.lib().egtb_result_timing(self._h, ctypes.byref(a), ctypes.byref(b),
+                                                     ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            native.lib().egtb_result_free(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], symm_mode: int, target: int,
+                      steps: int, docheck_step: int, pathname: str = "", *, is_free: bool = False,
+                      spawn_rate4: float = 0.1, success_rate_dtype: str = "float32",
+                      write_files: bool = True, keep: bool = False, keep_raw: bool = False,
+                      device: int = 0, radix_bits: int = 0) -> Optional[BuildResult]:
+    """The native build call (reference formation_core.run_pattern_build,
+    native_core/src/bindings_formation.cpp:220-229) with the PatternSpec and
+    RunOptions fields spelled out.  ``keep=True`` returns the solved layers
+    resident in HBM as a BuildResult."""
+    a = _Args(arr_init, masks, shifts, symm_mode, target, steps, docheck_step, pathname, is_free,
+              spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits)
+    L = native.lib()
+    seeds_p = a.seeds.ctypes.data_as(native.u64p)
+    if keep or keep_raw:
+        h = ctypes.c_void_p()
+        native.check(L.egtb_build_ex(seeds_p, a.seeds.size, ctypes.byref(a.pattern), ctypes.byref(a.options),
+                                     ctypes.byref(h)))
+        return BuildResult(h.value, success_rate_dtype)
+    native.check(L.egtb_build(seeds_p, a.seeds.size, ctypes.byref(a.pattern), ctypes.byref(a.options)))
+    return None
+
+
+def get_build_progress():
+    """(current, total) of the running build (bindings_formation.cpp:147-153)."""
+    cur, tot = ctypes.c_uint32(), ctypes.c_uint32()
+    native.lib().egtb_get_build_progress(ctypes.byref(cur), ctypes.byref(tot))
+    return int(cur.value), int(tot.value)
+
+
+def _should_retry_build_resume(exc: Exception) -> bool:
+    message = str(exc).strip()
+    return message == "The length multiplier is not big enough. Please restart." or \
+        message.startswith("length multiplier ")
+
+
+def _run_with_single_resume_retry(build_fn) -> None:
+    """engine_core/BookBuilder.py:246-261.  The GPU build sizes every buffer
+    exactly, so the capacity error it retries never occurs here; the retry
+    is kept for the same error contract."""
+    retried = False
+    while True:
+        try:
+            build_fn()
+            return
+        except Exception as exc:
+            if retried or not _should_retry_build_resume(exc):
+                raise
+            retried = True
+
+
+def start_build(pattern: str, target: int, pathname: str) -> bool:
+    """Build ``pattern`` at tile 2**target into files prefixed ``pathname``
+    (engine_core/BookBuilder.py:264-317, classic path)."""
+    cfg = SingletonConfig().config
+    if cfg.get("advanced_algo", False):
+        raise NotImplementedError("the advanced (masked) algorithm is not part of this engine")
+    if pattern not in pattern_catalog:
+        raise KeyError(f"Unknown pattern: {pattern}")
+    p = build_params(pattern, target)
+    save_config_to_txt(pathname + "config.txt")
+    dtype = str(cfg.get("success_rate_dtype", "uint32"))
+    _run_with_single_resume_retry(lambda: run_pattern_build(
+        p["seeds"], p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"], p["docheck_step"],
+        pathname, is_free=p["is_free"], spawn_rate4=float(cfg["4_spawn_rate"]), success_rate_dtype=dtype,
+        write_files=True, device=int(cfg.get("device", 0))))
+    return True
+
+
+def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "float32",
+                    spawn_rate4: float = 0.1, keep_raw: bool = False, device: int = 0,
+                    radix_bits: int = 0) -> BuildResult:
+    """Build a catalogue pattern and keep every solved layer in HBM."""
+    p = build_params(pattern, target)
+    return run_pattern_build(p["seeds"], p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"],
+                             p["docheck_step"], "", is_free=p["is_free"], spawn_rate4=spawn_rate4,
+                             success_rate_dtype=success_rate_dtype, write_files=False, keep=True,
+                             keep_raw=keep_raw, device=device, radix_bits=radix_bits)

@@ -1,0 +1,67 @@
+// Host-visible declarations of the kernel launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "egtb_device.h"
+
+namespace egtb {
+
+// A sorted, zero-compacted layer as the solve kernel sees it: SoA keys and
+// rates plus a bucket index idx[0..2^bits] over the top `bits` key bits.
+template <typename T>
+struct TableDev {
+    const uint64_t *keys;
+    const T *rates;
+    const uint32_t *idx;
+    uint64_t n;
+    uint32_t bits;
+    uint32_t pad;
+};
+
+struct LaunchShape {
+    uint32_t blocks;   // persistent grid size (upper bound)
+    uint32_t threads;  // threads per block
+};
+
+void configure_kernels();
+void launch_build_lut(hipStream_t s, uint16_t *lut);
+void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n,
+                   const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *counts,
+                   const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
+                   uint64_t *or_and4);
+
+uint64_t scan_partials_needed(uint64_t n);
+void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *partial);
+void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out, uint32_t *partial);
+
+uint32_t radix_tile(int bits);
+void launch_radix_pass(hipStream_t s, int bits, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
+                       uint32_t *counts, uint32_t *offsets, uint32_t *partial);
+
+uint64_t unique_tiles(uint64_t n);
+void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts, uint64_t *tile_offs,
+                   uint64_t *partial, uint64_t *out);
+
+void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n, uint32_t bits, uint32_t *idx);
+
+template <typename T>
+void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
+                  const PatternDev &p, int do_check, const TableDev<T> &t1, const TableDev<T> &t2, double w2,
+                  double w4, T max_scale, T *rates);
+
+template <typename T>
+void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,
+                        T *rates);
+
+uint64_t compact_tiles(uint64_t n);
+template <typename T>
+void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
+                    uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
+                    unsigned long long *max_bits);
+
+template <typename T>
+void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);
+
+}  // namespace egtb

@@ -1,0 +1,785 @@
+// HIP kernels for the classic tablebase build on gfx950 (wave64).
+//
+//   k_build_lut        65536-row left-slide table (global; staged to LDS)
+//   k_expand<WRITE>    forward expansion: count pass / write pass
+//   k_scan_*           device-wide exclusive scan (reduce / partials / down)
+//   k_radix_hist/scatter  LSD radix sort pass with wave-ballot multisplit
+//   k_unique_*         adjacent dedup of a sorted array
+//   k_bucket_index     top-bit range index over a sorted layer
+//   k_solve<T>         backward success-rate recurrence
+//   k_final_rates<T>   final layers: success -> max_scale else zero
+//   k_compact_*<T>     keep rate > zero (stable)
+//   k_pack_records<T>  SoA -> packed {u64 board; T rate} records
+//
+// Launch wrappers at the bottom are the only entry points used by the host.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "egtb_device.h"
+#include "egtb_kernels.h"
+
+namespace egtb {
+
+// ---------------------------------------------------------------------------
+// Row table
+// ---------------------------------------------------------------------------
+
+// Slide a 4-tile row toward its high nibble: compact the non-empty tiles,
+// merge equal neighbours once from the move side, 0xF (32768) never merges
+// (reference StandardMergePolicy::_internal_merge, BoardMover.h:7-55).
+__global__ void k_build_lut(uint16_t *lut) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= kLutEntries) return;
+    uint32_t tiles[4], n = 0;
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t v = (row >> (12 - 4 * c)) & 0xFu;
+        if (v) tiles[n++] = v;
+    }
+    uint32_t out[4] = {0, 0, 0, 0}, m = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (i + 1 < n && tiles[i] == tiles[i + 1] && tiles[i] != 0xFu) {
+            out[m++] = tiles[i] + 1;
+            ++i;
+        } else {
+            out[m++] = tiles[i];
+        }
+    }
+    lut[row] = (uint16_t)((out[0] << 12) | (out[1] << 8) | (out[2] << 4) | out[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Wave / block helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint64_t lanes_below(uint64_t m) { return m & __lanemask_lt(); }
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T y = __shfl_up(v, off, 64);
+        if (lane >= (uint32_t)off) v += y;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_reduce_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Exclusive scan of one value per thread over the block; `total` receives
+// the block sum.  `scratch` needs blockDim/64 entries.  Ends synchronised.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_sum(T v, T *scratch, T &total) {
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const T inc = wave_inclusive_sum(v);
+    if (lane == 63) scratch[wave] = inc;
+    __syncthreads();
+    T before = 0, sum = 0;
+    for (uint32_t w = 0; w < nwaves; ++w) {
+        const T t = scratch[w];
+        if (w < wave) before += t;
+        sum += t;
+    }
+    __syncthreads();
+    total = sum;
+    return before + inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// Forward expansion (reference gen_boards_* , BookGenerator.cpp:473-760)
+// ---------------------------------------------------------------------------
+
+// One thread per parent.  For each empty nibble k (ascending) and spawn
+// value 1 (a 2) / 2 (a 4): the four moves of the spawned board; a move is
+// kept when it changed the board and the result is in the pattern; the
+// canonical form is emitted into the spawn-2 (next layer) or spawn-4
+// (layer after) stream.  Parents are skipped when do_check and they are
+// success boards (BookGenerator.cpp:634,991).
+//
+// WRITE = false: store packed counts (c2 | c4 << 32) per parent and fold the
+//   OR / AND of every child into or_and2 / or_and4 (varying-bit span used to
+//   size the radix passes).
+// WRITE = true: write children at the exclusive prefix `offs` (packed the
+//   same way) into out2 / out4.
+template <bool WRITE>
+__global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ par, uint64_t n,
+                                                 const uint16_t *__restrict__ glut, PatternDev p,
+                                                 int do_check, uint64_t *__restrict__ counts,
+                                                 const uint64_t *__restrict__ offs,
+                                                 uint64_t *__restrict__ out2,
+                                                 uint64_t *__restrict__ out4,
+                                                 uint64_t *__restrict__ or_and2,
+                                                 uint64_t *__restrict__ or_and4) {
+    extern __shared__ uint16_t lut[];
+    stage_lut(lut, glut);
+    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t b = par[i];
+        uint32_t c2 = 0, c4 = 0;
+        uint64_t w2 = 0, w4 = 0;
+        if (WRITE) {
+            const uint64_t o = offs[i];
+            w2 = o & 0xFFFFFFFFull;
+            w4 = o >> 32;
+        }
+        if (!(do_check && is_success(b, p))) {
+            uint32_t em = empty_cells(b);
+            while (em) {
+                const int k = __builtin_ctz(em);
+                em &= em - 1;
+#pragma unroll
+                for (int s = 1; s <= 2; ++s) {
+                    const uint64_t sp = b | ((uint64_t)s << (4 * k));
+                    uint64_t mv[4];
+                    move4(sp, lut, mv);
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        if (mv[d] != sp && is_pattern(mv[d], p)) {
+                            const uint64_t c = canonical(mv[d], p.symm);
+                            if (s == 1) {
+                                if (WRITE) out2[w2 + c2] = c;
+                                else { or2 |= c; and2 &= c; }
+                                ++c2;
+                            } else {
+                                if (WRITE) out4[w4 + c4] = c;
+                                else { or4 |= c; and4 &= c; }
+                                ++c4;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (!WRITE) counts[i] = (uint64_t)c2 | ((uint64_t)c4 << 32);
+    }
+    if (!WRITE) {
+        // fold the span accumulators: wave, then one atomic per wave
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            or2 |= __shfl_xor(or2, off, 64);
+            and2 &= __shfl_xor(and2, off, 64);
+            or4 |= __shfl_xor(or4, off, 64);
+            and4 &= __shfl_xor(and4, off, 64);
+        }
+        if (lane_id() == 0) {
+            atomicOr((unsigned long long *)&or_and2[0], (unsigned long long)or2);
+            atomicAnd((unsigned long long *)&or_and2[1], (unsigned long long)and2);
+            atomicOr((unsigned long long *)&or_and4[0], (unsigned long long)or4);
+            atomicAnd((unsigned long long *)&or_and4[1], (unsigned long long)and4);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (out[0..n) exclusive prefix, out[n] = total)
+// ---------------------------------------------------------------------------
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI *__restrict__ in, uint64_t n,
+                                                              TO *__restrict__ partial) {
+    __shared__ TO scratch[kScanThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    TO s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
+        if (idx < n) s += (TO)in[idx];
+    }
+    s = wave_reduce_sum(s);
+    if (lane_id() == 0) scratch[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        TO t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += scratch[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// Single block: exclusive scan of the per-tile partials in place.
+template <typename TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(TO *__restrict__ partial, uint64_t m) {
+    __shared__ TO scratch[kScanThreads / 64];
+    TO carry = 0;
+    for (uint64_t base = 0; base < m; base += kScanThreads) {
+        const uint64_t idx = base + threadIdx.x;
+        const TO v = idx < m ? partial[idx] : (TO)0;
+        TO total;
+        const TO ex = block_exclusive_sum(v, scratch, total);
+        if (idx < m) partial[idx] = carry + ex;
+        carry += total;
+    }
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI *__restrict__ in, uint64_t n,
+                                                            TO *__restrict__ out,
+                                                            const TO *__restrict__ partial,
+                                                            int write_total) {
+    __shared__ TO tile[kScanTile];
+    __shared__ TO scratch[kScanThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
+        tile[j * kScanThreads + threadIdx.x] = idx < n ? (TO)in[idx] : (TO)0;
+    }
+    __syncthreads();
+    TO local[kScanItems];
+    TO s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        local[j] = s;
+        s += tile[threadIdx.x * kScanItems + j];
+    }
+    TO total;
+    const TO ex = block_exclusive_sum(s, scratch, total) + partial[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) tile[threadIdx.x * kScanItems + j] = ex + local[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
+        if (idx < n) out[idx] = tile[j * kScanThreads + threadIdx.x];
+    }
+    if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        out[n] = partial[blockIdx.x] + total;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort pass (stable), wave64 ballot multisplit
+// ---------------------------------------------------------------------------
+
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;
+
+// Lanes of this wave whose digit equals mine, among the lanes in `valid`.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t digit, uint64_t valid) {
+    uint64_t m = valid;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        const bool bit = (digit >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+// Per-tile digit histogram -> counts[bin * ntiles + tile].
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ src, uint64_t n,
+                                                             int shift, uint32_t *__restrict__ counts,
+                                                             uint32_t ntiles) {
+    constexpr int BINS = 1 << BITS;
+    constexpr int TILE = kSortThreads * ITEMS;
+    __shared__ uint32_t hist[BINS];
+    for (int i = threadIdx.x; i < BINS; i += kSortThreads) hist[i] = 0;
+    __syncthreads();
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)wave * ITEMS * 64;
+#pragma unroll 4
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)j * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (uint32_t)((src[idx] >> shift) & (BINS - 1)) : 0u;
+        const uint64_t m = match_digit<BITS>(d, __ballot(valid));
+        if (valid && lanes_below(m) == 0) atomicAdd(&hist[d], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < BINS; i += kSortThreads) counts[(uint64_t)i * ntiles + blockIdx.x] = hist[i];
+}
+
+// Scatter one tile: stable local ranks by wave multisplit, reorder through
+// LDS by digit, then write each digit run contiguously at its global offset
+// (offsets = exclusive scan of counts in bin-major order).
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *__restrict__ src,
+                                                                uint64_t *__restrict__ dst, uint64_t n,
+                                                                int shift,
+                                                                const uint32_t *__restrict__ offsets,
+                                                                uint32_t ntiles) {
+    constexpr int BINS = 1 << BITS;
+    constexpr int TILE = kSortThreads * ITEMS;
+    __shared__ uint64_t stage[TILE];
+    __shared__ uint32_t whist[kSortWaves][BINS];
+    __shared__ uint32_t binbase[BINS];
+    __shared__ uint32_t scratch[kSortWaves];
+
+    for (int i = threadIdx.x; i < kSortWaves * BINS; i += kSortThreads) (&whist[0][0])[i] = 0;
+    __syncthreads();
+
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t tile0 = (uint64_t)blockIdx.x * TILE;
+    const uint64_t base = tile0 + (uint64_t)wave * ITEMS * 64;
+    uint64_t key[ITEMS];
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)j * 64 + lane;
+        key[j] = idx < n ? src[idx] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)j * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (uint32_t)((key[j] >> shift) & (BINS - 1));
+        const uint64_t m = match_digit<BITS>(d, __ballot(valid));
+        const uint32_t below = (uint32_t)__popcll(lanes_below(m));
+        uint32_t prev = 0;
+        if (valid) prev = whist[wave][d];
+        rank[j] = prev + below;
+        if (valid && below == 0) whist[wave][d] = prev + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    // wave-exclusive prefix per bin, block total per bin
+    for (int bin = threadIdx.x; bin < BINS; bin += kSortThreads) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            const uint32_t t = whist[w][bin];
+            whist[w][bin] = s;
+            s += t;
+        }
+        binbase[bin] = s;
+    }
+    __syncthreads();
+    // exclusive scan of the per-bin totals (each thread owns a run of bins)
+    {
+        constexpr int PER = BINS >= kSortThreads ? BINS / kSortThreads : 1;
+        const int first = threadIdx.x * PER;
+        uint32_t local[PER];
+        uint32_t s = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            local[q] = s;
+            s += (first + q < BINS) ? binbase[first + q] : 0u;
+        }
+        uint32_t total;
+        const uint32_t ex = block_exclusive_sum(s, scratch, total);
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (first + q < BINS) binbase[first + q] = ex + local[q];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)j * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)((key[j] >> shift) & (BINS - 1));
+            stage[binbase[d] + whist[wave][d] + rank[j]] = key[j];
+        }
+    }
+    __syncthreads();
+    const uint64_t m = (n - tile0) < (uint64_t)TILE ? (n - tile0) : (uint64_t)TILE;
+    for (uint32_t i = threadIdx.x; i < m; i += kSortThreads) {
+        const uint64_t k = stage[i];
+        const uint32_t d = (uint32_t)((k >> shift) & (BINS - 1));
+        dst[(uint64_t)offsets[(uint64_t)d * ntiles + blockIdx.x] + (i - binbase[d])] = k;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Unique of a sorted array (reference unique_sorted_u64_inplace,
+// UniqueUtils.h:218-256): keep a[i] when i == 0 or a[i] != a[i-1].
+// ---------------------------------------------------------------------------
+
+constexpr int kUniqThreads = 256;
+constexpr int kUniqItems = 8;
+constexpr int kUniqTile = kUniqThreads * kUniqItems;
+
+__global__ __launch_bounds__(kUniqThreads) void k_unique_count(const uint64_t *__restrict__ a, uint64_t n,
+                                                               uint64_t *__restrict__ counts) {
+    __shared__ uint64_t scratch[kUniqThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kUniqTile;
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kUniqItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kUniqThreads + threadIdx.x;
+        if (idx < n) c += (idx == 0 || a[idx] != a[idx - 1]) ? 1u : 0u;
+    }
+    c = wave_reduce_sum(c);
+    if (lane_id() == 0) scratch[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kUniqThreads / 64; ++w) t += scratch[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+// offsets: exclusive scan of the tile counts; out_n (optional) gets the total.
+__global__ __launch_bounds__(kUniqThreads) void k_unique_write(const uint64_t *__restrict__ a, uint64_t n,
+                                                               const uint64_t *__restrict__ offsets,
+                                                               uint64_t *__restrict__ out) {
+    __shared__ uint64_t scratch[kUniqThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kUniqTile + (uint64_t)threadIdx.x * kUniqItems;
+    uint64_t v[kUniqItems];
+    uint32_t f = 0;
+    uint64_t cnt = 0;
+#pragma unroll
+    for (int j = 0; j < kUniqItems; ++j) {
+        const uint64_t idx = base + j;
+        const bool keep = idx < n && (idx == 0 || a[idx] != a[idx - 1]);
+        v[j] = idx < n ? a[idx] : 0;
+        f |= (keep ? 1u : 0u) << j;
+        cnt += keep;
+    }
+    uint64_t total;
+    uint64_t pos = block_exclusive_sum(cnt, scratch, total) + offsets[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kUniqItems; ++j)
+        if ((f >> j) & 1u) out[pos++] = v[j];
+}
+
+// ---------------------------------------------------------------------------
+// Bucket index over a sorted layer: idx[v] = lower_bound(keys, v << shift),
+// idx[2^bits] = n.  Narrows every lookup to one bucket (the reference's
+// AdaptiveIndex level 1 over the top key bits, AdaptiveIndex.cpp:89-91).
+// ---------------------------------------------------------------------------
+
+__global__ void k_bucket_index(const uint64_t *__restrict__ keys, uint64_t n, uint32_t bits,
+                               uint32_t *__restrict__ idx) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nb = 1ull << bits;
+    if (v > nb) return;
+    if (v == nb) {
+        idx[v] = (uint32_t)n;
+        return;
+    }
+    const uint64_t target = v << (64 - bits);
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    idx[v] = (uint32_t)lo;
+}
+
+// ---------------------------------------------------------------------------
+// Backward solve (reference recalculate_layer, BookSolver.cpp:413-470)
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T table_lookup(const TableDev<T> &t, uint64_t key) {
+    if (t.n == 0) return (T)0;
+    const uint64_t bk = key >> (64 - t.bits);
+    uint64_t lo = t.idx[bk], hi = t.idx[bk + 1];
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (t.keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < t.n && t.keys[lo] == key) ? t.rates[lo] : (T)0;
+}
+
+// rate(b) = max_scale when do_check and b is a success board; otherwise
+// with E empty nibbles visited in ascending order,
+//   sp = fma(best2_k, 1 - p4, sp); sp = fma(best4_k, p4, sp)   (double)
+//   rate = (T)(sp / E)   (E = 0 -> 0)
+// where best_x_k is the T-typed max over the valid moves of the spawned
+// board of the next (spawn 2) / after-next (spawn 4) layer's rate of the
+// canonical moved board (absent -> 0).  The fused update is what the
+// reference's -Ofast -march=x86-64-v3 build computes (CMakeLists.txt:82-83).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ keys, uint64_t n,
+                                                const uint16_t *__restrict__ glut, PatternDev p,
+                                                int do_check, TableDev<T> t1, TableDev<T> t2,
+                                                double w2, double w4, T max_scale,
+                                                T *__restrict__ rates) {
+    extern __shared__ uint16_t lut[];
+    stage_lut(lut, glut);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t b = keys[i];
+        if (do_check && is_success(b, p)) {
+            rates[i] = max_scale;
+            continue;
+        }
+        double sp = 0.0;
+        uint32_t em = empty_cells(b);
+        const int e = __popc(em);
+        while (em) {
+            const int k = __builtin_ctz(em);
+            em &= em - 1;
+            T best[2] = {(T)0, (T)0};
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const uint64_t sp_board = b | ((uint64_t)(s + 1) << (4 * k));
+                uint64_t mv[4];
+                move4(sp_board, lut, mv);
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    if (mv[d] != sp_board && is_pattern(mv[d], p)) {
+                        const uint64_t c = canonical(mv[d], p.symm);
+                        const T r = s == 0 ? table_lookup(t1, c) : table_lookup(t2, c);
+                        best[s] = best[s] < r ? r : best[s];
+                    }
+                }
+            }
+            sp = __fma_rn((double)best[0], w2, sp);
+            sp = __fma_rn((double)best[1], w4, sp);
+        }
+        rates[i] = e > 0 ? (T)(sp / (double)e) : (T)0;
+    }
+}
+
+// Final layers (reference final_situation_process, BookSolver.cpp:237-261).
+template <typename T>
+__global__ void k_final_rates(const uint64_t *__restrict__ keys, uint64_t n, PatternDev p, T max_scale,
+                              T *__restrict__ rates) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rates[i] = is_success(keys[i], p) ? max_scale : (T)0;
+}
+
+// ---------------------------------------------------------------------------
+// Stable compaction of rate > 0 (reference compact_live_entries,
+// BookSolver.cpp:380-394), SoA in -> SoA out; also max rate for stats.
+// ---------------------------------------------------------------------------
+
+constexpr int kCompThreads = 256;
+constexpr int kCompItems = 8;
+constexpr int kCompTile = kCompThreads * kCompItems;
+
+template <typename T>
+__global__ __launch_bounds__(kCompThreads) void k_compact_count(const T *__restrict__ rates, uint64_t n,
+                                                                uint64_t *__restrict__ counts) {
+    __shared__ uint64_t scratch[kCompThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kCompTile;
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kCompItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kCompThreads + threadIdx.x;
+        if (idx < n) c += rates[idx] > (T)0 ? 1u : 0u;
+    }
+    c = wave_reduce_sum(c);
+    if (lane_id() == 0) scratch[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kCompThreads / 64; ++w) t += scratch[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ uint64_t rate_bits(T v) {
+    if constexpr (sizeof(T) == 4) return (uint64_t)__builtin_bit_cast(uint32_t, v);
+    else return __builtin_bit_cast(uint64_t, v);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *__restrict__ keys,
+                                                                const T *__restrict__ rates, uint64_t n,
+                                                                const uint64_t *__restrict__ offsets,
+                                                                uint64_t *__restrict__ out_keys,
+                                                                T *__restrict__ out_rates,
+                                                                unsigned long long *__restrict__ max_bits) {
+    __shared__ uint64_t scratch[kCompThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kCompTile + (uint64_t)threadIdx.x * kCompItems;
+    T r[kCompItems];
+    uint64_t cnt = 0;
+    T mx = (T)0;
+#pragma unroll
+    for (int j = 0; j < kCompItems; ++j) {
+        const uint64_t idx = base + j;
+        r[j] = idx < n ? rates[idx] : (T)0;
+        cnt += r[j] > (T)0;
+        mx = mx < r[j] ? r[j] : mx;
+    }
+    uint64_t total;
+    uint64_t pos = block_exclusive_sum(cnt, scratch, total) + offsets[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kCompItems; ++j) {
+        if (r[j] > (T)0) {
+            out_keys[pos] = keys[base + j];
+            out_rates[pos] = r[j];
+            ++pos;
+        }
+    }
+    // non-negative IEEE floats and unsigned ints both order as their bits
+    if (mx > (T)0) atomicMax(max_bits, (unsigned long long)rate_bits(mx));
+}
+
+// SoA -> packed records {u64 board; T rate} (SuccessEntry<T>,
+// FormationRuntime.h:190-200), as 32-bit words.
+template <typename T>
+__global__ void k_pack_records(const uint64_t *__restrict__ keys, const T *__restrict__ rates, uint64_t n,
+                               uint32_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    constexpr int W = (8 + sizeof(T)) / 4;
+    uint32_t *rec = out + i * W;
+    const uint64_t k = keys[i];
+    rec[0] = (uint32_t)k;
+    rec[1] = (uint32_t)(k >> 32);
+    const uint64_t rb = rate_bits(rates[i]);
+    rec[2] = (uint32_t)rb;
+    if constexpr (W == 4) rec[3] = (uint32_t)(rb >> 32);
+}
+
+// ===========================================================================
+// Launch wrappers
+// ===========================================================================
+
+static inline uint32_t blocks_for(uint64_t n, uint64_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void configure_kernels() {
+    const int lds = kLutEntries * sizeof(uint16_t);
+    (void)hipFuncSetAttribute((const void *)k_expand<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_solve<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_solve<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+void launch_build_lut(hipStream_t s, uint16_t *lut) {
+    hipLaunchKernelGGL(k_build_lut, dim3(kLutEntries / 256), dim3(256), 0, s, lut);
+}
+
+void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n,
+                   const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *counts,
+                   const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
+                   uint64_t *or_and4) {
+    if (n == 0) return;
+    const uint32_t want = blocks_for(n, ls.threads);
+    const uint32_t grid = want < ls.blocks ? want : ls.blocks;
+    const size_t lds = kLutEntries * sizeof(uint16_t);
+    if (write)
+        hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(ls.threads), lds, s, par, n, lut, p, do_check,
+                           counts, offs, out2, out4, or_and2, or_and4);
+    else
+        hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(ls.threads), lds, s, par, n, lut, p, do_check,
+                           counts, offs, out2, out4, or_and2, or_and4);
+}
+
+template <typename TI, typename TO>
+static void scan_impl(hipStream_t s, const TI *in, uint64_t n, TO *out, TO *partial, bool write_total) {
+    const uint32_t tiles = blocks_for(n ? n : 1, kScanTile);
+    hipLaunchKernelGGL((k_scan_reduce<TI, TO>), dim3(tiles), dim3(kScanThreads), 0, s, in, n, partial);
+    hipLaunchKernelGGL((k_scan_partials<TO>), dim3(1), dim3(kScanThreads), 0, s, partial, (uint64_t)tiles);
+    hipLaunchKernelGGL((k_scan_down<TI, TO>), dim3(tiles), dim3(kScanThreads), 0, s, in, n, out, partial,
+                       write_total ? 1 : 0);
+}
+
+uint64_t scan_partials_needed(uint64_t n) { return blocks_for(n ? n : 1, kScanTile); }
+
+void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *partial) {
+    scan_impl<uint64_t, uint64_t>(s, in, n, out, partial, true);
+}
+
+void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out, uint32_t *partial) {
+    scan_impl<uint32_t, uint32_t>(s, in, n, out, partial, false);
+}
+
+uint32_t radix_tile(int bits) {
+    return kSortThreads * (bits >= 11 ? 16 : 16);
+}
+
+template <int BITS>
+static void radix_pass_impl(hipStream_t s, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
+                            uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
+    constexpr int ITEMS = 16;
+    constexpr int TILE = kSortThreads * ITEMS;
+    const uint32_t ntiles = blocks_for(n, TILE);
+    hipLaunchKernelGGL((k_radix_hist<BITS, ITEMS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, n, shift,
+                       counts, ntiles);
+    launch_scan_u32(s, counts, (uint64_t)ntiles << BITS, offsets, partial);
+    hipLaunchKernelGGL((k_radix_scatter<BITS, ITEMS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst, n,
+                       shift, offsets, ntiles);
+}
+
+void launch_radix_pass(hipStream_t s, int bits, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
+                       uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
+    if (n == 0) return;
+    switch (bits) {
+        case 4: radix_pass_impl<4>(s, src, dst, n, shift, counts, offsets, partial); break;
+        case 6: radix_pass_impl<6>(s, src, dst, n, shift, counts, offsets, partial); break;
+        case 11: radix_pass_impl<11>(s, src, dst, n, shift, counts, offsets, partial); break;
+        default: radix_pass_impl<8>(s, src, dst, n, shift, counts, offsets, partial); break;
+    }
+}
+
+uint64_t unique_tiles(uint64_t n) { return blocks_for(n ? n : 1, kUniqTile); }
+
+void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts, uint64_t *tile_offs,
+                   uint64_t *partial, uint64_t *out) {
+    if (n == 0) return;
+    const uint32_t tiles = (uint32_t)unique_tiles(n);
+    hipLaunchKernelGGL(k_unique_count, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_counts);
+    launch_scan_u64(s, tile_counts, tiles, tile_offs, partial);
+    hipLaunchKernelGGL(k_unique_write, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_offs, out);
+}
+
+void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n, uint32_t bits, uint32_t *idx) {
+    const uint64_t nb = (1ull << bits) + 1;
+    hipLaunchKernelGGL(k_bucket_index, dim3(blocks_for(nb, 256)), dim3(256), 0, s, keys, n, bits, idx);
+}
+
+template <typename T>
+void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
+                  const PatternDev &p, int do_check, const TableDev<T> &t1, const TableDev<T> &t2, double w2,
+                  double w4, T max_scale, T *rates) {
+    if (n == 0) return;
+    const uint32_t want = blocks_for(n, ls.threads);
+    const uint32_t grid = want < ls.blocks ? want : ls.blocks;
+    hipLaunchKernelGGL((k_solve<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n,
+                       lut, p, do_check, t1, t2, w2, w4, max_scale, rates);
+}
+
+template <typename T>
+void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,
+                        T *rates) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_final_rates<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, p, max_scale,
+                       rates);
+}
+
+uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
+
+template <typename T>
+void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
+                    uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
+                    unsigned long long *max_bits) {
+    if (n == 0) return;
+    const uint32_t tiles = (uint32_t)compact_tiles(n);
+    hipLaunchKernelGGL((k_compact_count<T>), dim3(tiles), dim3(kCompThreads), 0, s, rates, n, tile_counts);
+    launch_scan_u64(s, tile_counts, tiles, tile_offs, partial);
+    hipLaunchKernelGGL((k_compact_write<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, tile_offs,
+                       out_keys, out_rates, max_bits);
+}
+
+template <typename T>
+void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_pack_records<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, rates, n, out);
+}
+
+#define EGTB_INSTANTIATE(T)                                                                                   \
+    template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
+                                  const uint16_t *, const PatternDev &, int, const TableDev<T> &,             \
+                                  const TableDev<T> &, double, double, T, T *);                               \
+    template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
+    template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *,           \
+                                    uint64_t *, uint64_t *, uint64_t *, T *, unsigned long long *);           \
+    template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
+
+EGTB_INSTANTIATE(float)
+EGTB_INSTANTIATE(double)
+EGTB_INSTANTIATE(uint32_t)
+EGTB_INSTANTIATE(uint64_t)
+
+}  // namespace egtb

@@ -1,0 +1,822 @@
+// Host runtime of libegtb_hip.so: device context, stream-ordered buffers,
+// the dedup pipeline (LSD radix sort + unique), the plain per-layer build
+// schedule (forward, final layers, backward), reference-layout file output
+// and the C ABI declared in include/egtb.h.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+#include "egtb.h"
+#include "egtb_kernels.h"
+#include "egtb_runtime.h"
+
+namespace egtb {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+
+thread_local std::string g_last_error;
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw Error(EGTB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// Device context
+// ---------------------------------------------------------------------------
+
+static std::mutex g_ctx_mu;
+static std::map<int, std::unique_ptr<DeviceCtx>> g_ctx;
+
+DeviceCtx &device_ctx(int dev) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    auto it = g_ctx.find(dev);
+    if (it != g_ctx.end()) {
+        HIPCHK(hipSetDevice(dev));
+        return *it->second;
+    }
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (dev < 0 || dev >= ndev) throw Error(EGTB_ERR_ARG, "no HIP device " + std::to_string(dev));
+    HIPCHK(hipSetDevice(dev));
+    auto c = std::make_unique<DeviceCtx>();
+    c->dev = dev;
+    HIPCHK(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (lds < kLutEntries * 2)
+        throw Error(EGTB_ERR_HIP, "device LDS per block (" + std::to_string(lds) +
+                                      " B) cannot hold the 128 KiB row table");
+    configure_kernels();
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&c->lut, kLutEntries * sizeof(uint16_t)));
+    launch_build_lut(c->stream, c->lut);
+    HIPCHK(hipGetLastError());
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // one 1024-thread block per CU: the 128 KiB LDS table allows no more
+    c->shape.blocks = (uint32_t)c->cus;
+    c->shape.threads = 1024;
+    DeviceCtx &ref = *c;
+    g_ctx[dev] = std::move(c);
+    return ref;
+}
+
+// ---------------------------------------------------------------------------
+// Stream-ordered device buffers
+// ---------------------------------------------------------------------------
+
+void DBuf::alloc(size_t nbytes, hipStream_t s) {
+    release();
+    stream = s;
+    bytes = nbytes;
+    if (nbytes == 0) return;
+    hipError_t e = hipMallocAsync(&p, nbytes, s);
+    if (e != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        throw Error(EGTB_ERR_MEMORY, "device allocation of " + std::to_string(nbytes) + " bytes failed: " +
+                                         hipGetErrorString(e));
+    }
+}
+
+void DBuf::release() {
+    if (p) (void)hipFreeAsync(p, stream);
+    p = nullptr;
+    bytes = 0;
+}
+
+void DBuf::ensure(size_t nbytes, hipStream_t s) {
+    if (nbytes <= bytes && p) return;
+    alloc(nbytes + nbytes / 4 + 256, s);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel statistics (HIP events around launch groups on the build stream)
+// ---------------------------------------------------------------------------
+
+static std::atomic<int> g_stats_on{0};
+static std::mutex g_stats_mu;
+struct StatAcc {
+    uint64_t launches = 0;
+    double ms = 0, bytes = 0;
+};
+struct PendingStat {
+    const char *name;
+    hipEvent_t a, b;
+    double bytes;
+};
+static std::map<std::string, StatAcc> g_stats;
+static std::vector<PendingStat> g_pending;
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t pooled_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+}
+
+// Resolve recorded event pairs into the accumulators (waits for them).
+static void resolve_pending_locked() {
+    for (auto &pe : g_pending) {
+        (void)hipEventSynchronize(pe.b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, pe.a, pe.b);
+        auto &acc = g_stats[pe.name];
+        acc.launches++;
+        acc.ms += ms;
+        acc.bytes += pe.bytes;
+        g_event_pool.push_back(pe.a);
+        g_event_pool.push_back(pe.b);
+    }
+    g_pending.clear();
+}
+
+Timer::Timer(hipStream_t s, const char *name, double bytes) : s_(s), name_(name), bytes_(bytes) {
+    if (!g_stats_on.load(std::memory_order_relaxed)) return;
+    on_ = true;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    a_ = pooled_event();
+    b_ = pooled_event();
+    HIPCHK(hipEventRecord(a_, s_));
+}
+
+Timer::~Timer() {
+    if (!on_) return;
+    (void)hipEventRecord(b_, s_);
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    g_pending.push_back(PendingStat{name_, a_, b_, bytes_});
+    if (g_pending.size() > 4096) resolve_pending_locked();
+}
+
+// ---------------------------------------------------------------------------
+// Dedup pipeline: LSD radix sort over the varying-bit span + unique
+// ---------------------------------------------------------------------------
+
+int choose_radix_bits(int span, int requested) {
+    if (requested == 4 || requested == 6 || requested == 8 || requested == 11) return requested;
+    if (span <= 0) return 8;
+    // fewest passes; on a tie the narrower digit (cheaper histograms)
+    const int p8 = (span + 7) / 8, p11 = (span + 10) / 11;
+    return p11 < p8 ? 11 : 8;
+}
+
+uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all,
+                     uint64_t and_all, int bits_req, SortWork &w) {
+    const uint64_t diff = or_all ^ and_all;
+    if (n < 2 || diff == 0) return keys;
+    const int lo = __builtin_ctzll(diff);
+    const int hi = 64 - __builtin_clzll(diff);
+    const int bits = choose_radix_bits(hi - lo, bits_req);
+    const uint64_t ntiles = (n + radix_tile(bits) - 1) / radix_tile(bits);
+    const uint64_t nc = ntiles << bits;
+    w.counts.ensure(nc * 4, s);
+    w.offsets.ensure(nc * 4, s);
+    w.partial.ensure(scan_partials_needed(nc) * 8 + 64, s);
+    uint64_t *src = keys, *dst = tmp;
+    for (int shift = lo; shift < hi; shift += bits) {
+        launch_radix_pass(s, bits, src, dst, n, shift, (uint32_t *)w.counts.p, (uint32_t *)w.offsets.p,
+                          (uint32_t *)w.partial.p);
+        std::swap(src, dst);
+    }
+    HIPCHK(hipGetLastError());
+    return src;
+}
+
+// Sort keys[0..n) (keys and tmp both n long), unique into `out` (resized),
+// return the unique count.  Synchronises the stream to read the count.
+uint64_t sort_unique(hipStream_t s, DBuf &keys, DBuf &tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
+                     int bits_req, SortWork &w, DBuf &out) {
+    if (n == 0) {
+        out.alloc(0, s);
+        return 0;
+    }
+    uint64_t *sorted;
+    {
+        Timer t(s, "sort", 16.0 * (double)n);
+        sorted = radix_sort(s, (uint64_t *)keys.p, (uint64_t *)tmp.p, n, or_all, and_all, bits_req, w);
+    }
+    const uint64_t tiles = unique_tiles(n);
+    w.tile_counts.ensure(tiles * 8, s);
+    w.tile_offs.ensure((tiles + 1) * 8, s);
+    w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+    DBuf tmp_out;
+    tmp_out.alloc(n * 8, s);
+    {
+        Timer t(s, "unique", 16.0 * (double)n);
+        launch_unique(s, sorted, n, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
+                      (uint64_t *)w.partial.p, (uint64_t *)tmp_out.p);
+    }
+    uint64_t m = 0;
+    HIPCHK(hipMemcpyAsync(&m, (uint64_t *)w.tile_offs.p + tiles, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (m == n) {
+        out = std::move(tmp_out);
+        out.bytes = n * 8;
+    } else {
+        out.alloc(m * 8, s);
+        if (m) HIPCHK(hipMemcpyAsync(out.p, tmp_out.p, m * 8, hipMemcpyDeviceToDevice, s));
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Files (reference FileIOUtils write path: <path>.tmp then rename,
+// FileIOUtils.cpp:730-830)
+// ---------------------------------------------------------------------------
+
+void write_file_atomic(const std::string &path, const void *data, size_t bytes) {
+    const std::string tmp = path + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) throw Error(EGTB_ERR_IO, "cannot open " + tmp + " for writing");
+    size_t done = bytes ? std::fwrite(data, 1, bytes, f) : 0;
+    const bool ok = done == bytes && std::fflush(f) == 0;
+    std::fclose(f);
+    if (!ok) {
+        std::remove(tmp.c_str());
+        throw Error(EGTB_ERR_IO, "short write to " + tmp);
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) throw Error(EGTB_ERR_IO, "cannot rename " + tmp);
+}
+
+static bool file_exists(const std::string &path) {
+    struct stat st;
+    return ::stat(path.c_str(), &st) == 0;
+}
+
+static std::string now_string() {
+    std::time_t t = std::time(nullptr);
+    std::tm tmv{};
+    localtime_r(&t, &tmv);
+    char buf[64];
+    std::strftime(buf, sizeof buf, "%Y-%m-%d %H:%M:%S", &tmv);
+    return buf;
+}
+
+static double round_to_2(double v) { return std::round(v * 100.0) / 100.0; }
+
+// ---------------------------------------------------------------------------
+// Build progress (reference FormationProgress, FormationRuntime.h:61-86)
+// ---------------------------------------------------------------------------
+
+static std::atomic<uint32_t> g_prog_cur{0}, g_prog_total{0};
+
+void progress_set(uint32_t cur, uint32_t total) {
+    g_prog_total.store(total, std::memory_order_relaxed);
+    g_prog_cur.store(cur, std::memory_order_relaxed);
+}
+
+// ---------------------------------------------------------------------------
+// Rate type traits
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct RateTraits;
+template <>
+struct RateTraits<float> {
+    static float max_scale() { return 1.0f; }
+    static double to_unit(float v) { return (double)v; }
+};
+template <>
+struct RateTraits<double> {
+    static double max_scale() { return 1.0; }
+    static double to_unit(double v) { return v; }
+};
+template <>
+struct RateTraits<uint32_t> {
+    static uint32_t max_scale() { return 4000000000u; }
+    static double to_unit(uint32_t v) { return (double)v / 4000000000.0; }
+};
+template <>
+struct RateTraits<uint64_t> {
+    static uint64_t max_scale() { return 1600000000000000000ull; }
+    static double to_unit(uint64_t v) { return (double)v / 1.6e18; }
+};
+
+// ---------------------------------------------------------------------------
+// The plain per-layer build
+// ---------------------------------------------------------------------------
+
+PatternDev make_pattern(const egtb_pattern *pat, int target) {
+    if (!pat) throw Error(EGTB_ERR_ARG, "pattern is NULL");
+    if (pat->n_masks > (uint32_t)kMaxMasks) throw Error(EGTB_ERR_ARG, "more than 16 pattern masks");
+    if (pat->n_shifts > 16) throw Error(EGTB_ERR_ARG, "more than 16 success shifts");
+    if (pat->symm_mode < 0 || pat->symm_mode > 3)
+        throw Error(EGTB_ERR_ARG, "symm_mode " + std::to_string(pat->symm_mode) +
+                                      " is not supported (identity, full, diagonal, horizontal)");
+    if (target < 1 || target > 15) throw Error(EGTB_ERR_ARG, "target must be in 1..15 (log2 of the tile)");
+    PatternDev p;
+    std::memset(&p, 0, sizeof p);
+    p.n_masks = pat->n_masks;
+    for (uint32_t i = 0; i < pat->n_masks; ++i) p.masks[i] = pat->masks[i];
+    p.symm = (uint32_t)pat->symm_mode;
+    p.any_success = pat->n_shifts == 0 ? 1u : 0u;
+    for (uint32_t i = 0; i < pat->n_shifts; ++i) {
+        const uint32_t sh = pat->shifts[i];
+        if (sh > 60 || sh % 4) throw Error(EGTB_ERR_ARG, "success shift must be a nibble offset 0..60");
+        p.success_mask |= 0xFull << sh;
+    }
+    p.target_stacked = (uint64_t)target * 0x1111111111111111ull;
+    return p;
+}
+
+static uint32_t index_bits_for(uint64_t n) {
+    uint32_t b = 1;
+    while (b < 24 && (1ull << (b + 3)) < n) ++b;
+    return b;
+}
+
+template <typename T>
+void make_table(hipStream_t s, Book &bk) {
+    bk.bits = index_bits_for(bk.n);
+    bk.idx.alloc(((1ull << bk.bits) + 1) * 4, s);
+    launch_bucket_index(s, (const uint64_t *)bk.keys.p, bk.n, bk.bits, (uint32_t *)bk.idx.p);
+}
+
+template <typename T>
+TableDev<T> table_of(const Book &bk) {
+    TableDev<T> t;
+    t.keys = (const uint64_t *)bk.keys.p;
+    t.rates = (const T *)bk.rates.p;
+    t.idx = (const uint32_t *)bk.idx.p;
+    t.n = bk.n;
+    t.bits = bk.bits;
+    t.pad = 0;
+    return t;
+}
+
+// keys[0..n) with rates -> compacted book (rate > 0); returns max rate bits.
+template <typename T>
+uint64_t compact_into(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, SortWork &w, Book &out) {
+    out.rates_size = sizeof(T);
+    if (n == 0) {
+        out.n = 0;
+        out.keys.alloc(0, s);
+        out.rates.alloc(0, s);
+        return 0;
+    }
+    const uint64_t tiles = compact_tiles(n);
+    w.tile_counts.ensure(tiles * 8, s);
+    w.tile_offs.ensure((tiles + 1) * 8, s);
+    w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+    w.maxbits.ensure(8, s);
+    HIPCHK(hipMemsetAsync(w.maxbits.p, 0, 8, s));
+    DBuf k, r;
+    k.alloc(n * 8, s);
+    r.alloc(n * sizeof(T), s);
+    {
+        Timer t(s, "compact", (double)n * (sizeof(T) + 8) * 2);
+        launch_compact<T>(s, keys, rates, n, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
+                          (uint64_t *)w.partial.p, (uint64_t *)k.p, (T *)r.p,
+                          (unsigned long long *)w.maxbits.p);
+    }
+    uint64_t host[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&host[0], (uint64_t *)w.tile_offs.p + tiles, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&host[1], w.maxbits.p, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    out.n = host[0];
+    out.keys = std::move(k);
+    out.rates = std::move(r);
+    return host[1];
+}
+
+template <typename T>
+void write_book_file(hipStream_t s, const std::string &path, const Book &bk, HostStage &stage) {
+    constexpr size_t rec = 8 + sizeof(T);
+    const size_t bytes = bk.n * rec;
+    DBuf packed;
+    if (bk.n) {
+        packed.alloc(bytes, s);
+        launch_pack_records<T>(s, (const uint64_t *)bk.keys.p, (const T *)bk.rates.p, bk.n, (uint32_t *)packed.p);
+        stage.ensure(bytes);
+        HIPCHK(hipMemcpyAsync(stage.p, packed.p, bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    write_file_atomic(path, stage.p, bytes);
+}
+
+void HostStage::ensure(size_t n) {
+    if (n <= bytes) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    const size_t want = n + n / 4 + 4096;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        throw Error(EGTB_ERR_MEMORY, "pinned host allocation failed");
+    }
+    bytes = want;
+}
+
+HostStage::~HostStage() {
+    if (p) (void)hipHostFree(p);
+}
+
+template <typename T>
+void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const egtb_options &o, Result &res) {
+    DeviceCtx &C = device_ctx(o.device);
+    hipStream_t s = C.stream;
+    const int steps = o.steps;
+    const bool files = o.write_files != 0;
+    const std::string prefix = o.pathname ? o.pathname : "";
+    const T max_scale = RateTraits<T>::max_scale();
+    const double w4 = o.spawn_rate4, w2 = 1.0 - o.spawn_rate4;
+    const uint32_t prog_total = (uint32_t)(2 * steps);
+    progress_set(0, prog_total);
+
+    res.steps = steps;
+    res.rate_size = sizeof(T);
+    res.raw.clear();
+    res.raw.resize(steps);
+    res.books.clear();
+    res.books.resize(steps);
+    res.info.assign(steps, LayerInfo{});
+    SortWork w;
+    HostStage stage;
+
+    const auto t_begin = std::chrono::steady_clock::now();
+
+    // ---- forward (reference generate_process, BookGenerator.cpp:963-1121)
+    std::vector<DBuf> &raw = res.raw;
+    raw[0].alloc(n_seeds * 8, s);
+    if (n_seeds) HIPCHK(hipMemcpyAsync(raw[0].p, seeds, n_seeds * 8, hipMemcpyHostToDevice, s));
+    res.info[0].n_raw = n_seeds;
+
+    DBuf partial;  // unsorted spawn-4 children of the previous step (next layer)
+    uint64_t partial_n = 0, partial_or = 0, partial_and = ~0ull;
+    DBuf counts, offs, spans;
+    spans.alloc(4 * 8, s);
+    for (int i = 1; i <= steps - 2; ++i) {
+        progress_set((uint32_t)i, prog_total);
+        const uint64_t n = res.info[i - 1].n_raw;
+        const int do_check = i > o.docheck_step ? 1 : 0;
+        uint64_t c2 = 0, c4 = 0;
+        uint64_t span[4] = {0, ~0ull, 0, ~0ull};
+        if (n) {
+            counts.ensure(n * 8, s);
+            offs.ensure((n + 1) * 8, s);
+            w.partial.ensure(scan_partials_needed(n) * 8 + 64, s);
+            HIPCHK(hipMemcpyAsync(spans.p, span, sizeof span, hipMemcpyHostToDevice, s));
+            {
+                Timer t(s, "expand_count", 8.0 * (double)n);
+                launch_expand(s, false, C.shape, (const uint64_t *)raw[i - 1].p, n, C.lut, p, do_check,
+                              (uint64_t *)counts.p, nullptr, nullptr, nullptr, (uint64_t *)spans.p,
+                              (uint64_t *)spans.p + 2);
+            }
+            launch_scan_u64(s, (const uint64_t *)counts.p, n, (uint64_t *)offs.p, (uint64_t *)w.partial.p);
+            uint64_t tot = 0;
+            HIPCHK(hipMemcpyAsync(&tot, (uint64_t *)offs.p + n, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(span, spans.p, sizeof span, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            c2 = tot & 0xFFFFFFFFull;
+            c4 = tot >> 32;
+        }
+        DBuf cand, next_partial;
+        cand.alloc((partial_n + c2) * 8, s);
+        next_partial.alloc(c4 * 8, s);
+        if (partial_n) HIPCHK(hipMemcpyAsync(cand.p, partial.p, partial_n * 8, hipMemcpyDeviceToDevice, s));
+        if (n && (c2 || c4)) {
+            Timer t(s, "expand", 8.0 * (double)n + 8.0 * (double)(c2 + c4));
+            launch_expand(s, true, C.shape, (const uint64_t *)raw[i - 1].p, n, C.lut, p, do_check, nullptr,
+                          (const uint64_t *)offs.p, (uint64_t *)cand.p + partial_n, (uint64_t *)next_partial.p,
+                          nullptr, nullptr);
+        }
+        HIPCHK(hipGetLastError());
+        partial.release();
+        DBuf tmp;
+        tmp.alloc((partial_n + c2) * 8, s);
+        const uint64_t m = sort_unique(s, cand, tmp, partial_n + c2, span[0] | partial_or, span[1] & partial_and,
+                                       o.radix_bits, w, raw[i]);
+        res.info[i].n_raw = m;
+        res.info[i].c2 = c2;
+        res.info[i].c4 = c4;
+        partial = std::move(next_partial);
+        partial_n = c4;
+        partial_or = span[2];
+        partial_and = span[3];
+    }
+    {
+        DBuf tmp;
+        tmp.alloc(partial_n * 8, s);
+        res.info[steps - 1].n_raw =
+            sort_unique(s, partial, tmp, partial_n, partial_or, partial_and, o.radix_bits, w, raw[steps - 1]);
+        partial.release();
+    }
+    counts.release();
+    offs.release();
+    HIPCHK(hipStreamSynchronize(s));
+    const auto t_fwd = std::chrono::steady_clock::now();
+
+    // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
+    for (int i = steps - 2; i <= steps - 1; ++i) {
+        const uint64_t n = res.info[i].n_raw;
+        DBuf r;
+        r.alloc(n * sizeof(T), s);
+        launch_final_rates<T>(s, (const uint64_t *)raw[i].p, n, p, max_scale, (T *)r.p);
+        compact_into<T>(s, (const uint64_t *)raw[i].p, (const T *)r.p, n, w, res.books[i]);
+        res.info[i].n_live = res.books[i].n;
+        make_table<T>(s, res.books[i]);
+        if (files) write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], stage);
+        if (!o.keep_raw) raw[i].release();
+    }
+
+    // ---- backward (reference recalculate_process_impl, BookSolver.cpp:513-605)
+    if (files && !file_exists(prefix + "stats.txt")) {
+        const char *hdr = "layer,length,max success rate,speed,deletion_threshold,time\n";
+        FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
+        if (!f) throw Error(EGTB_ERR_IO, "cannot open " + prefix + "stats.txt");
+        std::fputs(hdr, f);
+        std::fclose(f);
+    }
+    for (int i = steps - 3; i >= 0; --i) {
+        progress_set(prog_total - (uint32_t)i - 2u, prog_total);
+        const auto l0 = std::chrono::steady_clock::now();
+        const uint64_t n = res.info[i].n_raw;
+        DBuf rates;
+        rates.alloc(n * sizeof(T), s);
+        const Book &b1 = res.books[i + 1], &b2 = res.books[i + 2];
+        {
+            Timer t(s, "solve", (double)n * (8 + sizeof(T)) + (double)(b1.n + b2.n) * (8 + sizeof(T)));
+            launch_solve<T>(s, C.shape, (const uint64_t *)raw[i].p, n, C.lut, p, i > o.docheck_step ? 1 : 0,
+                            table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p);
+        }
+        HIPCHK(hipGetLastError());
+        const uint64_t maxbits = compact_into<T>(s, (const uint64_t *)raw[i].p, (const T *)rates.p, n, w,
+                                                 res.books[i]);
+        res.info[i].n_live = res.books[i].n;
+        if (i > 0) make_table<T>(s, res.books[i]);
+        rates.release();
+        if (!o.keep_raw) raw[i].release();
+        const auto l1 = std::chrono::steady_clock::now();
+        if (files) {
+            if (res.books[i].n) {
+                T mx;
+                if (sizeof(T) == 4) {
+                    uint32_t b32 = (uint32_t)maxbits;
+                    std::memcpy(&mx, &b32, 4);
+                } else {
+                    std::memcpy(&mx, &maxbits, 8);
+                }
+                const double dt = std::chrono::duration<double>(l1 - l0).count();
+                char row[256];
+                std::snprintf(row, sizeof row, "%d,%llu,%g,%g mbps,%g,%s\n", i, (unsigned long long)n,
+                              RateTraits<T>::to_unit(mx), round_to_2((double)n / std::max(dt, 0.01) / 1e6), 0.0,
+                              now_string().c_str());
+                FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
+                if (f) {
+                    std::fputs(row, f);
+                    std::fclose(f);
+                }
+            }
+            write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], stage);
+        }
+        if (!res.keep_tables) res.books[i + 2] = Book{};
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    const auto t_end = std::chrono::steady_clock::now();
+    res.fwd_ms = std::chrono::duration<double, std::milli>(t_fwd - t_begin).count();
+    res.bwd_ms = std::chrono::duration<double, std::milli>(t_end - t_fwd).count();
+    res.total_ms = std::chrono::duration<double, std::milli>(t_end - t_begin).count();
+    progress_set(prog_total, prog_total);
+}
+
+void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *o,
+                    Result &res) {
+    if (!o) throw Error(EGTB_ERR_ARG, "options is NULL");
+    if (!seeds && n_seeds) throw Error(EGTB_ERR_ARG, "seeds is NULL");
+    if (o->steps < 3) throw Error(EGTB_ERR_ARG, "steps must be at least 3");
+    if (o->write_files && !o->pathname) throw Error(EGTB_ERR_ARG, "pathname is NULL");
+    if (!(o->spawn_rate4 >= 0.0 && o->spawn_rate4 <= 1.0)) throw Error(EGTB_ERR_ARG, "spawn_rate4 out of [0, 1]");
+    const PatternDev p = make_pattern(pattern, o->target);
+    const std::string dt = o->success_rate_dtype ? o->success_rate_dtype : "float32";
+    if (dt == "float32") run_build<float>(seeds, n_seeds, p, *o, res);
+    else if (dt == "float64") run_build<double>(seeds, n_seeds, p, *o, res);
+    else if (dt == "uint32") run_build<uint32_t>(seeds, n_seeds, p, *o, res);
+    else if (dt == "uint64") run_build<uint64_t>(seeds, n_seeds, p, *o, res);
+    else throw Error(EGTB_ERR_ARG, "success_rate_dtype '" + dt + "' is not supported");
+}
+
+}  // namespace egtb
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+using namespace egtb;
+
+struct egtb_result : Result {};
+
+template <typename F>
+static int guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        f();
+        return EGTB_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host out of memory";
+        return EGTB_ERR_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return EGTB_ERR_INTERNAL;
+    }
+}
+
+extern "C" {
+
+const char *egtb_last_error(void) { return g_last_error.c_str(); }
+
+void egtb_get_build_progress(uint32_t *current, uint32_t *total) {
+    uint32_t t = g_prog_total.load(std::memory_order_relaxed);
+    uint32_t c = g_prog_cur.load(std::memory_order_relaxed);
+    if (c > t) t = c;
+    if (current) *current = c;
+    if (total) *total = t;
+}
+
+int egtb_build(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *options) {
+    return guarded([&] {
+        Result r;
+        r.keep_tables = false;
+        build_dispatch(seeds, n_seeds, pattern, options, r);
+    });
+}
+
+int egtb_build_ex(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *options,
+                  egtb_result **out) {
+    if (!out) {
+        g_last_error = "out is NULL";
+        return EGTB_ERR_ARG;
+    }
+    *out = nullptr;
+    auto r = std::make_unique<egtb_result>();
+    r->keep_tables = true;
+    const int rc = guarded([&] { build_dispatch(seeds, n_seeds, pattern, options, *r); });
+    if (rc == EGTB_OK) *out = r.release();
+    return rc;
+}
+
+int egtb_result_steps(const egtb_result *r) { return r ? r->steps : 0; }
+
+int egtb_result_layer_info(const egtb_result *r, int layer, uint64_t *n_raw, uint64_t *n_live, uint64_t *c2,
+                           uint64_t *c4) {
+    if (!r || layer < 0 || layer >= r->steps) {
+        g_last_error = "layer out of range";
+        return EGTB_ERR_ARG;
+    }
+    const LayerInfo &li = r->info[layer];
+    if (n_raw) *n_raw = li.n_raw;
+    if (n_live) *n_live = li.n_live;
+    if (c2) *c2 = li.c2;
+    if (c4) *c4 = li.c4;
+    return EGTB_OK;
+}
+
+int egtb_result_raw(const egtb_result *r, int layer, uint64_t *keys, size_t cap, size_t *n) {
+    return guarded([&] {
+        if (!r || layer < 0 || layer >= r->steps) throw Error(EGTB_ERR_ARG, "layer out of range");
+        const uint64_t m = r->info[layer].n_raw;
+        if (n) *n = m;
+        if (m && !r->raw[layer].p) throw Error(EGTB_ERR_ARG, "raw layers were not kept (options.keep_raw)");
+        if (!keys) return;
+        if (cap < m) throw Error(EGTB_ERR_ARG, "buffer too small");
+        if (m) HIPCHK(hipMemcpy(keys, r->raw[layer].p, m * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int egtb_result_book(const egtb_result *r, int layer, uint64_t *keys, void *rates, size_t cap, size_t *n) {
+    return guarded([&] {
+        if (!r || layer < 0 || layer >= r->steps) throw Error(EGTB_ERR_ARG, "layer out of range");
+        const Book &b = r->books[layer];
+        if (n) *n = b.n;
+        if (!keys && !rates) return;
+        if (cap < b.n) throw Error(EGTB_ERR_ARG, "buffer too small");
+        if (b.n && keys) HIPCHK(hipMemcpy(keys, b.keys.p, b.n * 8, hipMemcpyDeviceToHost));
+        if (b.n && rates) HIPCHK(hipMemcpy(rates, b.rates.p, b.n * r->rate_size, hipMemcpyDeviceToHost));
+    });
+}
+
+int egtb_result_timing(const egtb_result *r, double *fwd_ms, double *bwd_ms, double *total_ms) {
+    if (!r) return EGTB_ERR_ARG;
+    if (fwd_ms) *fwd_ms = r->fwd_ms;
+    if (bwd_ms) *bwd_ms = r->bwd_ms;
+    if (total_ms) *total_ms = r->total_ms;
+    return EGTB_OK;
+}
+
+void egtb_result_free(egtb_result *r) {
+    if (!r) return;
+    try {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        delete r;
+        (void)hipDeviceSynchronize();
+    } catch (...) {
+    }
+}
+
+void sort_uint64(uint64_t *arr, size_t arrsize, bool descending) {
+    if (!arr || arrsize < 2) return;
+    const int rc = guarded([&] {
+        DeviceCtx &C = device_ctx(0);
+        hipStream_t s = C.stream;
+        DBuf a, t;
+        a.alloc(arrsize * 8, s);
+        t.alloc(arrsize * 8, s);
+        HIPCHK(hipMemcpyAsync(a.p, arr, arrsize * 8, hipMemcpyHostToDevice, s));
+        SortWork w;
+        // full 64-bit span: no key-range assumption for arbitrary input
+        uint64_t *out = radix_sort(s, (uint64_t *)a.p, (uint64_t *)t.p, arrsize, ~0ull, 0ull, 0, w);
+        HIPCHK(hipMemcpyAsync(arr, out, arrsize * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+    if (rc != EGTB_OK) {
+        std::fprintf(stderr, "sort_uint64: %s\n", g_last_error.c_str());
+        std::abort();
+    }
+    if (descending)
+        for (size_t i = 0, j = arrsize - 1; i < j; ++i, --j) std::swap(arr[i], arr[j]);
+}
+
+size_t unique_sorted_u64_inplace(uint64_t *data, size_t size) {
+    if (!data || size < 2) return size;
+    size_t m = 0;
+    const int rc = guarded([&] {
+        DeviceCtx &C = device_ctx(0);
+        hipStream_t s = C.stream;
+        DBuf a, out;
+        a.alloc(size * 8, s);
+        HIPCHK(hipMemcpyAsync(a.p, data, size * 8, hipMemcpyHostToDevice, s));
+        SortWork w;
+        const uint64_t tiles = unique_tiles(size);
+        w.tile_counts.ensure(tiles * 8, s);
+        w.tile_offs.ensure((tiles + 1) * 8, s);
+        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+        out.alloc(size * 8, s);
+        launch_unique(s, (const uint64_t *)a.p, size, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
+                      (uint64_t *)w.partial.p, (uint64_t *)out.p);
+        uint64_t mm = 0;
+        HIPCHK(hipMemcpyAsync(&mm, (uint64_t *)w.tile_offs.p + tiles, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (mm) HIPCHK(hipMemcpyAsync(data, out.p, mm * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        m = mm;
+    });
+    if (rc != EGTB_OK) {
+        std::fprintf(stderr, "unique_sorted_u64_inplace: %s\n", g_last_error.c_str());
+        std::abort();
+    }
+    return m;
+}
+
+void egtb_stats_enable(int on) { g_stats_on.store(on ? 1 : 0); }
+
+void egtb_stats_reset(void) {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    resolve_pending_locked();
+    g_stats.clear();
+}
+
+int egtb_stats_get(const char *kernel, uint64_t *launches, double *total_ms, double *alg_bytes) {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    resolve_pending_locked();
+    auto it = g_stats.find(kernel ? kernel : "");
+    if (it == g_stats.end()) {
+        if (launches) *launches = 0;
+        if (total_ms) *total_ms = 0;
+        if (alg_bytes) *alg_bytes = 0;
+        return EGTB_ERR_ARG;
+    }
+    if (launches) *launches = it->second.launches;
+    if (total_ms) *total_ms = it->second.ms;
+    if (alg_bytes) *alg_bytes = it->second.bytes;
+    return EGTB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+"""ctypes binding of libegtb_hip.so (the C ABI in include/egtb.h).
+
+The library is the product: every build and lookup runs through it on the
+GPU.  There is no CPU fallback -- if the library is missing or fails to
+load, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libegtb_hip.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class EgtbError(RuntimeError):
+    """A failure reported by libegtb_hip.so (message from egtb_last_error)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class Pattern(ctypes.Structure):
+    _fields_ = [
+        ("masks", u64p),
+        ("n_masks", ctypes.c_uint32),
+        ("shifts", ctypes.POINTER(ctypes.c_uint8)),
+        ("n_shifts", ctypes.c_uint32),
+        ("symm_mode", ctypes.c_int32),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("target", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("docheck_step", ctypes.c_int32),
+        ("pathname", ctypes.c_char_p),
+        ("is_free", ctypes.c_int32),
+        ("spawn_rate4", ctypes.c_double),
+        ("success_rate_dtype", ctypes.c_char_p),
+        ("write_files", ctypes.c_int32),
+        ("keep_raw", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("radix_bits", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+# Every symbol the header declares, with its ctypes signature.
+SIGNATURES = {
+    "egtb_build": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(Pattern), ctypes.POINTER(Options)]),
+    "egtb_get_build_progress": (None, [u32p, u32p]),
+    "egtb_last_error": (ctypes.c_char_p, []),
+    "egtb_build_ex": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(Pattern), ctypes.POINTER(Options),
+                                     ctypes.POINTER(ctypes.c_void_p)]),
+    "egtb_result_layer_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u64p, u64p, u64p, u64p]),
+    "egtb_result_steps": (ctypes.c_int, [ctypes.c_void_p]),
+    "egtb_result_raw": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u64p, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+    "egtb_result_book": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u64p, ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_size_t)]),
+    "egtb_result_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "egtb_result_free": (None, [ctypes.c_void_p]),
+    "sort_uint64": (None, [u64p, ctypes.c_size_t, ctypes.c_bool]),
+    "unique_sorted_u64_inplace": (ctypes.c_size_t, [u64p, ctypes.c_size_t]),
+    "egtb_stats_enable": (None, [ctypes.c_int]),
+    "egtb_stats_reset": (None, []),
+    "egtb_stats_get": (ctypes.c_int, [ctypes.c_char_p, u64p, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def build_library(jobs: int = 8) -> str:
+    """Compile libegtb_hip.so for gfx950 in-tree (hipcc cross-compiles
+    without a GPU)."""
+    subprocess.run(["make", "-s", "-C", CSRC, f"-j{jobs}"], check=True)
+    if not os.path.exists(LIB_PATH):
+        raise EgtbError(-5, f"build did not produce {LIB_PATH}")
+    return LIB_PATH
+
+
+def lib():
+    """Load the HIP library (once).  Raises when it is missing."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise EgtbError(-5, f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                                    "or make -C 2048endgametablebase_amd/csrc")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+        return _lib
+
+
+def last_error() -> str:
+    msg = lib().egtb_last_error()
+    return msg.decode(errors="replace") if msg else ""
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise EgtbError(rc, last_error() or f"libegtb_hip error {rc}")
+
+
+def gpu_available() -> bool:
+    """True when a HIP device is visible (does not load the library)."""
+    try:
+        import torch
+
+        return bool(torch.cuda.is_available())
+    except Exception:
+        return False

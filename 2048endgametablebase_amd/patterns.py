@@ -1,0 +1,165 @@
+"""Pattern catalogue and build-parameter formulas for the classic builder.
+
+The entries restate, as plain data, the patterns of the reference's
+``docs_and_configs/patterns_config.json`` that the classic path is run on,
+parsed the way ``Config.load_patterns_from_file`` does (Config.py:179-282):
+
+* ``pattern_masks``  -- the "valid pattern" masks; a board belongs to the
+  pattern when some mask's nibbles are all 0xF on it (Formation.h:11-17);
+* ``success_shifts`` -- bit offsets of the 0xF nibbles of "target pos"
+  (``find_f_nibble_positions``, Config.py:154-160);
+* ``canonical_mode`` -- symmetry reduction applied before storage;
+* ``seed_boards``    -- layer 0, in catalogue order;
+* ``nums_adjust``    -- minus the tile sum of the first seed;
+* ``extra_steps``    -- layers past target/2 (default 36).
+
+``steps_and_docheck`` is ``_steps_and_docheck`` (engine_core/BookBuilder.py:
+124-128).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Tuple
+
+# Symmetry mode numbering of the reference's SymmMode enum
+# (native_core/include/FormationRuntime.h:13).
+SYMM_MODES = {
+    "identity": 0,
+    "full": 1,
+    "diagonal": 2,
+    "horizontal": 3,
+    "min33": 4,
+    "min24": 5,
+    "min34": 6,
+}
+
+
+def _f_nibble_shifts(mask: int) -> Tuple[int, ...]:
+    return tuple(4 * i for i in range(16) if (mask >> (4 * i)) & 0xF == 0xF)
+
+
+def tile_sum_of(board: int) -> int:
+    """Sum of tile values of an encoded board (BoardCodec.h:20-74)."""
+    total = 0
+    for k in range(16):
+        v = (board >> (4 * k)) & 0xF
+        if v:
+            total += 1 << v
+    return total
+
+
+@dataclass(frozen=True)
+class PatternMeta:
+    name: str
+    category: str
+    pattern_masks: Tuple[int, ...]
+    success_shifts: Tuple[int, ...]
+    canonical_mode: str
+    seed_boards: Tuple[int, ...]
+    extra_steps: int = 36
+
+    @property
+    def symm_mode(self) -> int:
+        return SYMM_MODES[self.canonical_mode]
+
+    @property
+    def tile_sum(self) -> int:
+        return tile_sum_of(self.seed_boards[0])
+
+    @property
+    def nums_adjust(self) -> int:
+        return -self.tile_sum
+
+    @property
+    def free_cells(self) -> int:
+        pinned = 0
+        inter = 0xFFFFFFFFFFFFFFFF
+        for m in self.pattern_masks:
+            inter &= m
+        for k in range(16):
+            if (inter >> (4 * k)) & 0xF == 0xF:
+                pinned += 1
+        return 16 - pinned
+
+    @property
+    def fixed_positions(self) -> Tuple[int, ...]:
+        inter = 0xFFFFFFFFFFFFFFFF
+        for m in self.pattern_masks:
+            inter &= m
+        return _f_nibble_shifts(inter)
+
+
+def _entry(name, category, masks, target_pos, mode, seeds, extra):
+    mask_ints = tuple(int(m, 16) for m in masks)
+    seed_ints = tuple(int(s, 16) for s in seeds)
+    # Config.py:243-248 keeps only seeds with the first seed's tile sum.
+    first = tile_sum_of(seed_ints[0])
+    seed_ints = tuple(s for s in seed_ints if tile_sum_of(s) == first)
+    return PatternMeta(
+        name=name,
+        category=category,
+        pattern_masks=mask_ints,
+        success_shifts=_f_nibble_shifts(int(target_pos, 16)),
+        canonical_mode=mode,
+        seed_boards=seed_ints,
+        extra_steps=extra,
+    )
+
+
+# Values from docs_and_configs/patterns_config.json (the runtime file,
+# Config.py:182), for the classic patterns that the tablebase path builds.
+pattern_catalog: Dict[str, PatternMeta] = {
+    m.name: m
+    for m in (
+        _entry("L3", "10 space", ["0xfff0fff"], "0xffffffffffffffff", "identity",
+               ["0x100000001fff2fff", "0x000000012fff1fff"], 48),
+        _entry("442", "10 space", ["0xffffff"], "0xf000000", "identity",
+               ["0x1000000021ffffff", "0x0000000112ffffff"], 48),
+        _entry("t", "10 space", ["0xf0fff0ff", "0xf000f0ff00ff", "0xf000f00000ff00ff"],
+               "0xff0f000f00", "identity",
+               ["0x10000000f1fff2ff", "0x00000001f2fff1ff"], 48),
+        _entry("LL", "12 space", ["0xff00ff"], "0xff0f000f00", "diagonal",
+               ["0x1000000021ff12ff", "0x0000000112ff21ff"], 48),
+        _entry("4431", "12 space", ["0xf0fff"], "0xf00000", "identity",
+               ["0x10000000121f2fff", "0x00000001121f2fff"], 48),
+        _entry("444", "12 space", ["0xffff"], "0xff00000", "horizontal",
+               ["0x100000000000ffff", "0x000000010000ffff"], 60),
+        _entry("4442", "others", ["0xff"], "0xff0f00", "identity",
+               ["0x00001000000012ff", "0x00010000000021ff"], 48),
+        _entry("4432", "others", ["0xf00ff"], "0xf00000", "diagonal",
+               ["0x00001000121f21ff", "0x00010000121f21ff"], 48),
+        _entry("4441", "others", ["0xfff"], "0xfff000", "identity",
+               ["0x0000100000001fff", "0x0001000000001fff"], 48),
+        _entry("44", "others", ["0xffffffff"], "0xffff00000000", "identity",
+               ["0x00001000ffffffff"], 48),
+    )
+}
+
+
+def steps_and_docheck(tile_sum: int, target: int, extra_steps: int) -> Tuple[int, int]:
+    """Number of layers and the success-check threshold for a build
+    (engine_core/BookBuilder.py:124-128).  ``target`` is log2 of the tile."""
+    target_tile = 2 ** int(target)
+    steps = int(target_tile / 2 + extra_steps)
+    docheck_step = int(target_tile / 2) - tile_sum % target_tile // 2
+    return steps, docheck_step
+
+
+def build_params(pattern: str, target: int) -> Dict[str, object]:
+    """Everything a classic build of ``pattern`` at 2**target needs."""
+    meta = pattern_catalog[pattern]
+    steps, docheck = steps_and_docheck(meta.tile_sum, target, meta.extra_steps)
+    # BookBuilder.py:283-285: is_free = fewer than 4 fixed cells and small sum.
+    is_free = len(meta.fixed_positions) < 4 and meta.tile_sum < 180000
+    return {
+        "pattern": pattern,
+        "target": int(target),
+        "steps": steps,
+        "docheck_step": docheck,
+        "masks": list(meta.pattern_masks),
+        "shifts": list(meta.success_shifts),
+        "symm_mode": meta.symm_mode,
+        "seeds": list(meta.seed_boards),
+        "tile_sum": meta.tile_sum,
+        "is_free": is_free,
+    }

@@ -1,0 +1,175 @@
+"""Benchmark: solved states/s of the 442@512 classic tablebase build on one
+MI355X (plain per-layer GPU schedule), with the dominant kernel's HBM
+roofline and the reference's own CPU builder timed on the host beside it.
+
+One step = one complete build of pattern 442 at target 512 (304 layers:
+forward enumeration + dedup, final layers, backward solve) with the seeds
+resident and every solved layer kept in HBM (no file I/O in the timed
+region).  value = sum of raw layer sizes / build wall time.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import importlib
+import json
+import os
+import shutil
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_stats(native, names):
+    L = native.lib()
+    out = {}
+    for nm in names:
+        la, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
+        if L.egtb_stats_get(nm.encode(), ctypes.byref(la), ctypes.byref(ms), ctypes.byref(by)) == 0:
+            out[nm] = dict(launches=int(la.value), ms=ms.value, bytes=by.value)
+    return out
+
+
+def cpu_baseline(pattern, target, budget_threads, n_states):
+    """Time the reference's own classic builder (compiled from its sources
+    into oracle/_ref) on the same workload; fall back to the oracle port on
+    a smaller target if that library is absent."""
+    pkg = importlib.import_module("2048endgametablebase_amd")
+    p = pkg.patterns.build_params(pattern, target)
+    threads = max(4, min(32, budget_threads))
+    work = tempfile.mkdtemp(prefix="egtb_cpu_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        from oracle.ref import ref
+
+        if ref.available():
+            t0 = time.perf_counter()
+            ref.run_build(p["seeds"], p["masks"], p["shifts"], p["symm_mode"], target, p["steps"],
+                          p["docheck_step"], os.path.join(work, f"{pattern}_{2**target}_"), is_free=p["is_free"],
+                          threads=threads)
+            dt = time.perf_counter() - t0
+            # same layer sets as the GPU build (golden-checked), so the same count
+            return dict(value=round(n_states / dt, 1), unit="states/s", cores=threads, kind="reference",
+                        sample=f"full {pattern}@{2**target} build by the reference's run_pattern_build_cpp "
+                               f"(g++ -Ofast, {threads} OpenMP threads, layer files on tmpfs): "
+                               f"{n_states} states in {dt:.1f} s")
+        from oracle import oracle as orc
+
+        t_small = target - 1
+        ps = pkg.patterns.build_params(pattern, t_small)
+        t0 = time.perf_counter()
+        ob = orc.OracleBuild(ps["seeds"], ps["masks"], ps["shifts"], ps["symm_mode"], t_small, ps["steps"],
+                             ps["docheck_step"], threads=threads)
+        dt = time.perf_counter() - t0
+        n = sum(ob.raw(i).size for i in range(ps["steps"]))
+        ob.close()
+        return dict(value=round(n / dt, 1), unit="states/s", cores=threads, kind="port",
+                    sample=f"oracle port, {pattern}@{2**t_small} in memory, {n} states in {dt:.1f} s")
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pattern", default="442")
+    ap.add_argument("--target", type=int, default=9)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
+              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    pkg = importlib.import_module("2048endgametablebase_amd")
+    native = pkg.native
+    L = native.lib()
+    torch.cuda.init()
+
+    def one_build():
+        return pkg.build_in_memory(args.pattern, args.target, success_rate_dtype="float32")
+
+    for _ in range(args.warmup):
+        one_build().close()
+    torch.cuda.synchronize()
+
+    L.egtb_stats_reset()
+    L.egtb_stats_enable(1)
+    totals = []
+    fwd = bwd = 0.0
+    n_states = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = one_build()
+        f, b, tt = res.timing()
+        fwd += f
+        bwd += b
+        totals.append(tt)
+        if n_states == 0:
+            layers = res.steps
+            n_states = sum(res.layer_info(i)[0] for i in range(res.steps))
+            live = sum(res.layer_info(i)[1] for i in range(res.steps))
+            cands = sum(res.layer_info(i)[2] + res.layer_info(i)[3] for i in range(res.steps))
+        res.close()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    L.egtb_stats_enable(0)
+    stats = kernel_stats(native, ["expand_count", "expand", "sort", "unique", "solve", "compact"])
+
+    ms_per_step = wall * 1000.0 / args.steps
+    value = n_states / (wall / args.steps)
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"]) if stats else None
+    roofline = None
+    if dom:
+        name, st = dom
+        avg_ms = st["ms"] / max(st["launches"], 1)
+        achieved = (st["bytes"] / st["launches"]) / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
+                    "avg_launch_ms": round(avg_ms, 5), "launches_per_step": st["launches"] // args.steps}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.pattern, args.target, min(os.cpu_count() or 16, 16), n_states)
+        except Exception as exc:  # reported, never fatal for the GPU number
+            cpu = {"error": str(exc)}
+
+    line = {
+        "metric": "solved states/s for 442@512 build (plain GPU schedule)",
+        "value": round(value, 1),
+        "unit": "states/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 keys / f32 rates (f64 accumulate)",
+        "data": "synthetic: fully determined by pattern 442, target 512, seeds of the catalogue",
+        "config": {"workload": f"{args.pattern}@{2**args.target} classic build, float32 rates, p4=0.1",
+                   "layers": layers, "states_per_build": n_states,
+                   "live_records": live, "candidates": cands, "fwd_ms": round(fwd / args.steps, 2),
+                   "bwd_ms": round(bwd / args.steps, 2),
+                   "kernels": {k: {"ms_per_step": round(v["ms"] / args.steps, 3),
+                                   "launches_per_step": v["launches"] // args.steps} for k, v in stats.items()}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

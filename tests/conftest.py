@@ -1,0 +1,59 @@
+import hashlib
+import importlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X) and libegtb_hip.so")
+    config.addinivalue_line("markers", "slow: longer CPU test")
+
+
+@pytest.fixture(scope="session")
+def pkg():
+    return importlib.import_module("2048endgametablebase_amd")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle
+
+    oracle.lib()
+    return oracle
+
+
+def load_golden(tag):
+    with open(os.path.join(GOLDEN, f"{tag}.json")) as fh:
+        return json.load(fh)
+
+
+def golden_tags():
+    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))
+
+
+def book_bytes(keys, rates):
+    """Packed reference records {u64 board; f32 rate} (12 bytes each)."""
+    rec = np.empty(keys.size, dtype=np.dtype([("board", "<u8"), ("rate", "<f4")]))
+    rec["board"] = keys
+    rec["rate"] = rates
+    return rec.tobytes()
+
+
+def book_sha(keys, rates):
+    return hashlib.sha256(book_bytes(keys, rates)).hexdigest()
+
+
+def golden_params(g):
+    return dict(seeds=[int(s, 16) for s in g["seeds"]], masks=[int(m, 16) for m in g["masks"]],
+                shifts=list(g["shifts"]), symm=g["symm"], target=g["target"], steps=g["steps"],
+                docheck=g["docheck"])

@@ -1,0 +1,125 @@
+"""GPU parity of the plain build: the HIP path (through the C ABI) against
+the CPU oracle (bit-exact raw layers, solved keys and float32 rate bits) and
+against golden digests of the reference's own compiled builder."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import book_sha, golden_params, golden_tags, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(pkg, gp, dtype="float32", keep_raw=False, radix_bits=0):
+    return pkg.run_pattern_build(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                                 gp["docheck"], "", success_rate_dtype=dtype, write_files=False, keep=True,
+                                 keep_raw=keep_raw, radix_bits=radix_bits)
+
+
+@pytest.mark.parametrize("pattern,target", [("L3", 7), ("442", 7), ("LL", 6), ("444", 6), ("t", 6)])
+def test_layers_match_oracle(pkg, orc, pattern, target):
+    p = pkg.patterns.build_params(pattern, target)
+    gp = dict(seeds=p["seeds"], masks=p["masks"], shifts=p["shifts"], symm=p["symm_mode"], target=target,
+              steps=p["steps"], docheck=p["docheck_step"])
+    print(f"case {pattern}@{2**target}", flush=True)
+    res = _build(pkg, gp, keep_raw=True)
+    ob = orc.OracleBuild(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], target, gp["steps"], gp["docheck"])
+    try:
+        for i in range(gp["steps"]):
+            assert np.array_equal(res.raw(i), ob.raw(i)), f"raw layer {i}"
+            n_raw, n_live, c2, c4 = res.layer_info(i)
+            oc2, oc4 = ob.candidates(i)
+            if 0 < i <= gp["steps"] - 2:
+                assert (c2, c4) == (oc2, oc4), f"candidate counts of step {i}"
+            gk, gr = res.book(i)
+            ok, orr = ob.book(i)
+            assert np.array_equal(gk, ok), f"book keys {i}"
+            assert np.array_equal(gr.view(np.uint32), orr.view(np.uint32)), f"book rates {i}"
+            assert n_live == gk.size
+    finally:
+        res.close()
+        ob.close()
+
+
+@pytest.mark.parametrize("tag", golden_tags())
+def test_matches_reference_golden(pkg, tag):
+    """Every solved layer's packed bytes hash to the reference's .book file."""
+    g = load_golden(tag)
+    gp = golden_params(g)
+    print(f"case golden {tag}", flush=True)
+    res = _build(pkg, gp)
+    try:
+        bad = []
+        for layer in g["layers"]:
+            keys, rates = res.book(layer["i"])
+            if keys.size != layer["live"] or book_sha(keys, rates) != layer["sha256"]:
+                bad.append(layer["i"])
+            n_raw = res.layer_info(layer["i"])[0]
+            if layer["n"] is not None:
+                assert n_raw == layer["n"], f"raw size of layer {layer['i']}"
+        assert not bad, f"layers differing from the reference: {bad[:10]}"
+    finally:
+        res.close()
+
+
+@pytest.mark.parametrize("bits", [4, 6, 8, 11])
+def test_radix_widths_agree(pkg, bits):
+    """The digit width is a tuning knob: every width gives the same table."""
+    p = pkg.patterns.build_params("442", 7)
+    gp = dict(seeds=p["seeds"], masks=p["masks"], shifts=p["shifts"], symm=p["symm_mode"], target=7,
+              steps=p["steps"], docheck=p["docheck_step"])
+    g = load_golden("442_128")
+    res = _build(pkg, gp, radix_bits=bits)
+    try:
+        for layer in g["layers"]:
+            keys, rates = res.book(layer["i"])
+            assert book_sha(keys, rates) == layer["sha256"], f"layer {layer['i']} with {bits}-bit digits"
+    finally:
+        res.close()
+
+
+def test_start_build_writes_reference_layout(pkg, tmp_path):
+    """start_build writes config.txt, stats.txt and every <i>.book with the
+    reference's bytes (442 at 128, float32)."""
+    cfg = pkg.config.SingletonConfig().config
+    old = cfg["success_rate_dtype"]
+    cfg["success_rate_dtype"] = "float32"
+    try:
+        prefix = str(tmp_path / "442_128_")
+        assert pkg.start_build("442", 7, prefix) is True
+    finally:
+        cfg["success_rate_dtype"] = old
+    g = load_golden("442_128")
+    for layer in g["layers"]:
+        data = open(prefix + f"{layer['i']}.book", "rb").read()
+        assert hashlib.sha256(data).hexdigest() == layer["sha256"], layer["i"]
+    assert not [f for f in os.listdir(tmp_path) if f.endswith(".tmp")]
+    stats = open(prefix + "stats.txt").read().splitlines()
+    assert stats[0] == "layer,length,max success rate,speed,deletion_threshold,time"
+    rows = [r.split(",") for r in stats[1:]]
+    assert [int(r[0]) for r in rows] == list(range(g["steps"] - 3, -1, -1))[: len(rows)]
+    by_layer = {int(r[0]): int(r[1]) for r in rows}
+    for layer in g["layers"]:
+        if layer["n"] is not None and layer["live"]:
+            assert by_layer[layer["i"]] == layer["n"]
+    cfg_txt = open(prefix + "config.txt").read()
+    assert "success_rate_dtype: float32" in cfg_txt and "4_spawn_rate: 0.1" in cfg_txt
+
+
+def test_dedup_primitives_match_numpy(pkg):
+    """sort_uint64 / unique_sorted_u64_inplace (bookgen_native.so names)."""
+    L = pkg.native.lib()
+    rng = np.random.default_rng(0)
+    for n in [0, 1, 2, 1000, 4097, 300001]:
+        a = rng.integers(0, 2**63, size=n, dtype=np.uint64) | np.uint64(1 << 63) * (rng.integers(0, 2, n) > 0)
+        a = np.concatenate([a, a[: n // 3]])
+        want = np.sort(a)
+        L.sort_uint64(a.ctypes.data_as(pkg.native.u64p), a.size, False)
+        assert np.array_equal(a, want)
+        m = L.unique_sorted_u64_inplace(a.ctypes.data_as(pkg.native.u64p), a.size)
+        assert np.array_equal(a[:m], np.unique(want))
+    b = rng.integers(0, 2**40, size=5000, dtype=np.uint64)
+    L.sort_uint64(b.ctypes.data_as(pkg.native.u64p), b.size, True)
+    assert np.array_equal(b, np.sort(b)[::-1])
