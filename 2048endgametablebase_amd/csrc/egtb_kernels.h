@@ -15,10 +15,31 @@ struct TableDev {
     const uint64_t *keys;
     const T *rates;
     const uint32_t *idx;
-    uint64_t n;
+    const uint64_t *n_ptr;  // live size in device memory (read when non-null)
+    uint64_t n;             // live size when n_ptr is null
     uint32_t bits;
-    uint32_t pad;
+    uint32_t present;       // 1 when the layer exists (reader: its file exists)
 };
+
+// Exact-match lookup of `key` in a sorted table of n records; absent -> 0
+// (reference search_arr, BookSolver.cpp:317-327).
+template <typename T>
+__device__ __forceinline__ T table_lookup_n(const TableDev<T> &t, uint64_t n, uint64_t key) {
+    if (n == 0) return (T)0;
+    const uint64_t bk = key >> (64 - t.bits);
+    uint64_t lo = t.idx[bk], hi = t.idx[bk + 1];
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (t.keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && t.keys[lo] == key) ? t.rates[lo] : (T)0;
+}
+
+template <typename T>
+__device__ __forceinline__ T table_lookup_dev(const TableDev<T> &t, uint64_t key) {
+    return table_lookup_n(t, t.n, key);
+}
 
 struct LaunchShape {
     uint32_t blocks;   // persistent grid size (upper bound)
@@ -27,13 +48,17 @@ struct LaunchShape {
 
 void configure_kernels();
 void launch_build_lut(hipStream_t s, uint16_t *lut);
-void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n,
-                   const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *counts,
-                   const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
+// Sizes: *_bound is a host-side upper bound (launch geometry); when the
+// matching n_ptr is non-null the kernel reads the true size from device
+// memory, so a launch can be queued before the producer has finished.
+void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
+                   const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
+                   uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
                    uint64_t *or_and4);
 
 uint64_t scan_partials_needed(uint64_t n);
-void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *partial);
+void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n_bound, const uint64_t *n_ptr, uint64_t *out,
+                     uint64_t *partial, uint64_t *total_out);
 void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out, uint32_t *partial);
 
 uint32_t radix_tile(int bits);
@@ -42,9 +67,10 @@ void launch_radix_pass(hipStream_t s, int bits, const uint64_t *src, uint64_t *d
 
 uint64_t unique_tiles(uint64_t n);
 void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts, uint64_t *tile_offs,
-                   uint64_t *partial, uint64_t *out);
+                   uint64_t *partial, uint64_t *out, uint64_t *count_out);
 
-void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n, uint32_t bits, uint32_t *idx);
+void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, const uint64_t *n_ptr,
+                         uint32_t bits, uint32_t *idx);
 
 template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
@@ -59,9 +85,13 @@ uint64_t compact_tiles(uint64_t n);
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
                     uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
-                    unsigned long long *max_bits);
+                    unsigned long long *max_bits, uint64_t *count_out);
 
 template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);
+
+// Packed {u64 board; rate} records (rate_size 4 or 8 bytes) -> SoA.
+void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int rate_size, uint64_t *keys,
+                           void *rates);
 
 }  // namespace egtb

@@ -109,7 +109,8 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T *scratch, T &total) {
 // WRITE = true: write children at the exclusive prefix `offs` (packed the
 //   same way) into out2 / out4.
 template <bool WRITE>
-__global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ par, uint64_t n,
+__global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ par, uint64_t n_bound,
+                                                 const uint64_t *__restrict__ n_ptr,
                                                  const uint16_t *__restrict__ glut, PatternDev p,
                                                  int do_check, uint64_t *__restrict__ counts,
                                                  const uint64_t *__restrict__ offs,
@@ -118,7 +119,9 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                                                  uint64_t *__restrict__ or_and2,
                                                  uint64_t *__restrict__ or_and4) {
     extern __shared__ uint16_t lut[];
+    __shared__ uint64_t red[4][16];
     stage_lut(lut, glut);
+    const uint64_t n = n_ptr ? (*n_ptr < n_bound ? *n_ptr : n_bound) : n_bound;
     uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -169,7 +172,21 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
             or4 |= __shfl_xor(or4, off, 64);
             and4 &= __shfl_xor(and4, off, 64);
         }
+        const uint32_t wave = threadIdx.x >> 6;
         if (lane_id() == 0) {
+            red[0][wave] = or2;
+            red[1][wave] = and2;
+            red[2][wave] = or4;
+            red[3][wave] = and4;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+                or2 |= red[0][w];
+                and2 &= red[1][w];
+                or4 |= red[2][w];
+                and4 &= red[3][w];
+            }
             atomicOr((unsigned long long *)&or_and2[0], (unsigned long long)or2);
             atomicAnd((unsigned long long *)&or_and2[1], (unsigned long long)and2);
             atomicOr((unsigned long long *)&or_and4[0], (unsigned long long)or4);
@@ -186,10 +203,18 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
+__device__ __forceinline__ uint64_t dev_n(uint64_t bound, const uint64_t *n_ptr) {
+    if (!n_ptr) return bound;
+    const uint64_t v = *n_ptr;
+    return v < bound ? v : bound;
+}
+
 template <typename TI, typename TO>
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI *__restrict__ in, uint64_t n,
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI *__restrict__ in, uint64_t n_bound,
+                                                              const uint64_t *__restrict__ n_ptr,
                                                               TO *__restrict__ partial) {
     __shared__ TO scratch[kScanThreads / 64];
+    const uint64_t n = dev_n(n_bound, n_ptr);
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     TO s = 0;
 #pragma unroll
@@ -223,12 +248,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(TO *__restrict__
 }
 
 template <typename TI, typename TO>
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI *__restrict__ in, uint64_t n,
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI *__restrict__ in, uint64_t n_bound,
+                                                            const uint64_t *__restrict__ n_ptr,
                                                             TO *__restrict__ out,
                                                             const TO *__restrict__ partial,
-                                                            int write_total) {
+                                                            TO *__restrict__ total_out) {
     __shared__ TO tile[kScanTile];
     __shared__ TO scratch[kScanThreads / 64];
+    const uint64_t n = dev_n(n_bound, n_ptr);
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
@@ -253,8 +280,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI *__restrict
         const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
         if (idx < n) out[idx] = tile[j * kScanThreads + threadIdx.x];
     }
-    if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
-        out[n] = partial[blockIdx.x] + total;
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = partial[blockIdx.x] + total;
 }
 
 // ---------------------------------------------------------------------------
@@ -422,7 +448,8 @@ __global__ __launch_bounds__(kUniqThreads) void k_unique_count(const uint64_t *_
 // offsets: exclusive scan of the tile counts; out_n (optional) gets the total.
 __global__ __launch_bounds__(kUniqThreads) void k_unique_write(const uint64_t *__restrict__ a, uint64_t n,
                                                                const uint64_t *__restrict__ offsets,
-                                                               uint64_t *__restrict__ out) {
+                                                               uint64_t *__restrict__ out,
+                                                               uint64_t *__restrict__ count_out) {
     __shared__ uint64_t scratch[kUniqThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kUniqTile + (uint64_t)threadIdx.x * kUniqItems;
     uint64_t v[kUniqItems];
@@ -438,6 +465,7 @@ __global__ __launch_bounds__(kUniqThreads) void k_unique_write(const uint64_t *_
     }
     uint64_t total;
     uint64_t pos = block_exclusive_sum(cnt, scratch, total) + offsets[blockIdx.x];
+    if (count_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count_out = offsets[blockIdx.x] + total;
 #pragma unroll
     for (int j = 0; j < kUniqItems; ++j)
         if ((f >> j) & 1u) out[pos++] = v[j];
@@ -449,8 +477,9 @@ __global__ __launch_bounds__(kUniqThreads) void k_unique_write(const uint64_t *_
 // AdaptiveIndex level 1 over the top key bits, AdaptiveIndex.cpp:89-91).
 // ---------------------------------------------------------------------------
 
-__global__ void k_bucket_index(const uint64_t *__restrict__ keys, uint64_t n, uint32_t bits,
-                               uint32_t *__restrict__ idx) {
+__global__ void k_bucket_index(const uint64_t *__restrict__ keys, uint64_t n_bound,
+                               const uint64_t *__restrict__ n_ptr, uint32_t bits, uint32_t *__restrict__ idx) {
+    const uint64_t n = dev_n(n_bound, n_ptr);
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t nb = 1ull << bits;
     if (v > nb) return;
@@ -472,19 +501,6 @@ __global__ void k_bucket_index(const uint64_t *__restrict__ keys, uint64_t n, ui
 // Backward solve (reference recalculate_layer, BookSolver.cpp:413-470)
 // ---------------------------------------------------------------------------
 
-template <typename T>
-__device__ __forceinline__ T table_lookup(const TableDev<T> &t, uint64_t key) {
-    if (t.n == 0) return (T)0;
-    const uint64_t bk = key >> (64 - t.bits);
-    uint64_t lo = t.idx[bk], hi = t.idx[bk + 1];
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (t.keys[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return (lo < t.n && t.keys[lo] == key) ? t.rates[lo] : (T)0;
-}
-
 // rate(b) = max_scale when do_check and b is a success board; otherwise
 // with E empty nibbles visited in ascending order,
 //   sp = fma(best2_k, 1 - p4, sp); sp = fma(best4_k, p4, sp)   (double)
@@ -501,6 +517,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                                                 T *__restrict__ rates) {
     extern __shared__ uint16_t lut[];
     stage_lut(lut, glut);
+    const uint64_t n1 = t1.n_ptr ? *t1.n_ptr : t1.n, n2 = t2.n_ptr ? *t2.n_ptr : t2.n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t b = keys[i];
@@ -524,7 +541,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                 for (int d = 0; d < 4; ++d) {
                     if (mv[d] != sp_board && is_pattern(mv[d], p)) {
                         const uint64_t c = canonical(mv[d], p.symm);
-                        const T r = s == 0 ? table_lookup(t1, c) : table_lookup(t2, c);
+                        const T r = s == 0 ? table_lookup_n(t1, n1, c) : table_lookup_n(t2, n2, c);
                         best[s] = best[s] < r ? r : best[s];
                     }
                 }
@@ -586,7 +603,8 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *
                                                                 const uint64_t *__restrict__ offsets,
                                                                 uint64_t *__restrict__ out_keys,
                                                                 T *__restrict__ out_rates,
-                                                                unsigned long long *__restrict__ max_bits) {
+                                                                unsigned long long *__restrict__ max_bits,
+                                                                uint64_t *__restrict__ count_out) {
     __shared__ uint64_t scratch[kCompThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kCompTile + (uint64_t)threadIdx.x * kCompItems;
     T r[kCompItems];
@@ -601,6 +619,7 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *
     }
     uint64_t total;
     uint64_t pos = block_exclusive_sum(cnt, scratch, total) + offsets[blockIdx.x];
+    if (count_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count_out = offsets[blockIdx.x] + total;
 #pragma unroll
     for (int j = 0; j < kCompItems; ++j) {
         if (r[j] > (T)0) {
@@ -630,6 +649,17 @@ __global__ void k_pack_records(const uint64_t *__restrict__ keys, const T *__res
     if constexpr (W == 4) rec[3] = (uint32_t)(rb >> 32);
 }
 
+// Packed records -> SoA (the reader's table load).
+__global__ void k_unpack_records(const uint32_t *__restrict__ recs, uint64_t n, int words, uint64_t *__restrict__ keys,
+                                 uint32_t *__restrict__ rates) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t *r = recs + i * words;
+    keys[i] = (uint64_t)r[0] | ((uint64_t)r[1] << 32);
+    rates[i * (words - 2)] = r[2];
+    if (words == 4) rates[i * 2 + 1] = r[3];
+}
+
 // ===========================================================================
 // Launch wrappers
 // ===========================================================================
@@ -650,43 +680,47 @@ void launch_build_lut(hipStream_t s, uint16_t *lut) {
     hipLaunchKernelGGL(k_build_lut, dim3(kLutEntries / 256), dim3(256), 0, s, lut);
 }
 
-void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n,
-                   const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *counts,
-                   const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
+void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
+                   const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
+                   uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
                    uint64_t *or_and4) {
-    if (n == 0) return;
-    const uint32_t want = blocks_for(n, ls.threads);
+    if (n_bound == 0) return;
+    const uint32_t want = blocks_for(n_bound, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     const size_t lds = kLutEntries * sizeof(uint16_t);
     if (write)
-        hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(ls.threads), lds, s, par, n, lut, p, do_check,
-                           counts, offs, out2, out4, or_and2, or_and4);
+        hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
+                           do_check, counts, offs, out2, out4, or_and2, or_and4);
     else
-        hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(ls.threads), lds, s, par, n, lut, p, do_check,
-                           counts, offs, out2, out4, or_and2, or_and4);
+        hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
+                           do_check, counts, offs, out2, out4, or_and2, or_and4);
 }
 
 template <typename TI, typename TO>
-static void scan_impl(hipStream_t s, const TI *in, uint64_t n, TO *out, TO *partial, bool write_total) {
-    const uint32_t tiles = blocks_for(n ? n : 1, kScanTile);
-    hipLaunchKernelGGL((k_scan_reduce<TI, TO>), dim3(tiles), dim3(kScanThreads), 0, s, in, n, partial);
+static void scan_impl(hipStream_t s, const TI *in, uint64_t n_bound, const uint64_t *n_ptr, TO *out, TO *partial,
+                      TO *total_out) {
+    const uint32_t tiles = blocks_for(n_bound ? n_bound : 1, kScanTile);
+    hipLaunchKernelGGL((k_scan_reduce<TI, TO>), dim3(tiles), dim3(kScanThreads), 0, s, in, n_bound, n_ptr,
+                       partial);
     hipLaunchKernelGGL((k_scan_partials<TO>), dim3(1), dim3(kScanThreads), 0, s, partial, (uint64_t)tiles);
-    hipLaunchKernelGGL((k_scan_down<TI, TO>), dim3(tiles), dim3(kScanThreads), 0, s, in, n, out, partial,
-                       write_total ? 1 : 0);
+    hipLaunchKernelGGL((k_scan_down<TI, TO>), dim3(tiles), dim3(kScanThreads), 0, s, in, n_bound, n_ptr, out,
+                       partial, total_out);
 }
 
 uint64_t scan_partials_needed(uint64_t n) { return blocks_for(n ? n : 1, kScanTile); }
 
-void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *partial) {
-    scan_impl<uint64_t, uint64_t>(s, in, n, out, partial, true);
+void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n_bound, const uint64_t *n_ptr, uint64_t *out,
+                     uint64_t *partial, uint64_t *total_out) {
+    scan_impl<uint64_t, uint64_t>(s, in, n_bound, n_ptr, out, partial, total_out);
 }
 
 void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out, uint32_t *partial) {
-    scan_impl<uint32_t, uint32_t>(s, in, n, out, partial, false);
+    scan_impl<uint32_t, uint32_t>(s, in, n, nullptr, out, partial, nullptr);
 }
 
 uint32_t radix_tile(int bits) {
-    return kSortThreads * (bits >= 11 ? 16 : 16);
+    (void)bits;
+    return kSortThreads * 16;
 }
 
 template <int BITS>
@@ -716,17 +750,21 @@ void launch_radix_pass(hipStream_t s, int bits, const uint64_t *src, uint64_t *d
 uint64_t unique_tiles(uint64_t n) { return blocks_for(n ? n : 1, kUniqTile); }
 
 void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts, uint64_t *tile_offs,
-                   uint64_t *partial, uint64_t *out) {
-    if (n == 0) return;
+                   uint64_t *partial, uint64_t *out, uint64_t *count_out) {
+    if (n == 0) {
+        if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
+        return;
+    }
     const uint32_t tiles = (uint32_t)unique_tiles(n);
     hipLaunchKernelGGL(k_unique_count, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_counts);
-    launch_scan_u64(s, tile_counts, tiles, tile_offs, partial);
-    hipLaunchKernelGGL(k_unique_write, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_offs, out);
+    launch_scan_u64(s, tile_counts, tiles, nullptr, tile_offs, partial, tile_offs + tiles);
+    hipLaunchKernelGGL(k_unique_write, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_offs, out, count_out);
 }
 
-void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n, uint32_t bits, uint32_t *idx) {
+void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, const uint64_t *n_ptr,
+                         uint32_t bits, uint32_t *idx) {
     const uint64_t nb = (1ull << bits) + 1;
-    hipLaunchKernelGGL(k_bucket_index, dim3(blocks_for(nb, 256)), dim3(256), 0, s, keys, n, bits, idx);
+    hipLaunchKernelGGL(k_bucket_index, dim3(blocks_for(nb, 256)), dim3(256), 0, s, keys, n_bound, n_ptr, bits, idx);
 }
 
 template <typename T>
@@ -753,13 +791,16 @@ uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
                     uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
-                    unsigned long long *max_bits) {
-    if (n == 0) return;
+                    unsigned long long *max_bits, uint64_t *count_out) {
+    if (n == 0) {
+        if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
+        return;
+    }
     const uint32_t tiles = (uint32_t)compact_tiles(n);
     hipLaunchKernelGGL((k_compact_count<T>), dim3(tiles), dim3(kCompThreads), 0, s, rates, n, tile_counts);
-    launch_scan_u64(s, tile_counts, tiles, tile_offs, partial);
+    launch_scan_u64(s, tile_counts, tiles, nullptr, tile_offs, partial, tile_offs + tiles);
     hipLaunchKernelGGL((k_compact_write<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, tile_offs,
-                       out_keys, out_rates, max_bits);
+                       out_keys, out_rates, max_bits, count_out);
 }
 
 template <typename T>
@@ -768,13 +809,21 @@ void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, ui
     hipLaunchKernelGGL((k_pack_records<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, rates, n, out);
 }
 
+void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int rate_size, uint64_t *keys,
+                           void *rates) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_unpack_records, dim3(blocks_for(n, 256)), dim3(256), 0, s, recs, n, (8 + rate_size) / 4,
+                       keys, (uint32_t *)rates);
+}
+
 #define EGTB_INSTANTIATE(T)                                                                                   \
     template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
                                   const uint16_t *, const PatternDev &, int, const TableDev<T> &,             \
                                   const TableDev<T> &, double, double, T, T *);                               \
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *,           \
-                                    uint64_t *, uint64_t *, uint64_t *, T *, unsigned long long *);           \
+                                    uint64_t *, uint64_t *, uint64_t *, T *, unsigned long long *,            \
+                                    uint64_t *);                                                              \
     template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
 
 EGTB_INSTANTIATE(float)

@@ -94,12 +94,48 @@ class Timer {
     hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
-// A solved layer resident in HBM: SoA keys/rates plus its bucket index.
+// Bump allocator over large device chunks: layers are carved out in build
+// order, the newest allocation can be trimmed once its true size is known,
+// and everything is released at once.
+class Arena {
+  public:
+    explicit Arena(size_t min_chunk = (size_t)1 << 30) : min_chunk_(min_chunk) {}
+    Arena(const Arena &) = delete;
+    Arena &operator=(const Arena &) = delete;
+    ~Arena() { release(); }
+    void *alloc(size_t bytes);
+    void trim_last(void *p, size_t bytes);  // shrink the newest allocation
+    void release();
+    size_t used() const;
+
+  private:
+    struct Chunk {
+        char *p;
+        size_t cap, used;
+    };
+    std::vector<Chunk> chunks_;
+    size_t min_chunk_;
+};
+
+// Per-layer device record: sizes, key span and stats of layer i.
+struct LayerDev {
+    uint64_t n_raw;    // unique raw size N_i
+    uint64_t n_live;   // records with rate > 0
+    uint64_t or_all;   // OR of every candidate key of layer i
+    uint64_t and_all;  // AND of every candidate key of layer i
+    uint64_t maxbits;  // bits of the largest rate
+    uint64_t tot;      // candidates of forward step i: c2 | c4 << 32
+    uint64_t pad[2];
+};
+
+// A solved layer resident in HBM: SoA keys/rates (arena) plus its bucket index.
 struct Book {
-    DBuf keys, rates, idx;
-    uint64_t n = 0;
-    uint32_t bits = 0;
-    uint32_t rates_size = 4;
+    uint64_t *keys = nullptr;
+    void *rates = nullptr;
+    uint32_t *idx = nullptr;
+    uint64_t *n_dev = nullptr;  // &LayerDev::n_live
+    uint64_t n = 0;             // host copy, valid once the build finished
+    uint32_t bits = 1;
 };
 
 struct LayerInfo {
@@ -109,19 +145,22 @@ struct LayerInfo {
 struct Result {
     int steps = 0;
     int rate_size = 4;
+    bool rate_is_float = true;
+    int dev = 0;
     bool keep_tables = true;
-    std::vector<DBuf> raw;
+    Arena raw_arena, book_arena;
+    DBuf meta;  // LayerDev[steps + 1]
+    std::vector<uint64_t *> raw;
     std::vector<Book> books;
     std::vector<LayerInfo> info;
     double fwd_ms = 0, bwd_ms = 0, total_ms = 0;
+    LayerDev *meta_dev() const { return (LayerDev *)meta.p; }
 };
 
 PatternDev make_pattern(const egtb_pattern *pat, int target);
 int choose_radix_bits(int span, int requested);
 uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
                      int bits_req, SortWork &w);
-uint64_t sort_unique(hipStream_t s, DBuf &keys, DBuf &tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
-                     int bits_req, SortWork &w, DBuf &out);
 void write_file_atomic(const std::string &path, const void *data, size_t bytes);
 void progress_set(uint32_t cur, uint32_t total);
 void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *o,

@@ -17,6 +17,7 @@
 #include <string>
 #include <sys/stat.h>
 #include <unistd.h>
+#include <type_traits>
 #include <vector>
 
 #include "egtb.h"
@@ -156,6 +157,14 @@ static void resolve_pending_locked() {
     g_pending.clear();
 }
 
+// Add algorithmic bytes known only after a build to a kernel family.
+static void stats_add_bytes(const char *name, double bytes) {
+    if (!g_stats_on.load(std::memory_order_relaxed)) return;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    resolve_pending_locked();
+    g_stats[name].bytes += bytes;
+}
+
 Timer::Timer(hipStream_t s, const char *name, double bytes) : s_(s), name_(name), bytes_(bytes) {
     if (!g_stats_on.load(std::memory_order_relaxed)) return;
     on_ = true;
@@ -205,43 +214,6 @@ uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, u
     }
     HIPCHK(hipGetLastError());
     return src;
-}
-
-// Sort keys[0..n) (keys and tmp both n long), unique into `out` (resized),
-// return the unique count.  Synchronises the stream to read the count.
-uint64_t sort_unique(hipStream_t s, DBuf &keys, DBuf &tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
-                     int bits_req, SortWork &w, DBuf &out) {
-    if (n == 0) {
-        out.alloc(0, s);
-        return 0;
-    }
-    uint64_t *sorted;
-    {
-        Timer t(s, "sort", 16.0 * (double)n);
-        sorted = radix_sort(s, (uint64_t *)keys.p, (uint64_t *)tmp.p, n, or_all, and_all, bits_req, w);
-    }
-    const uint64_t tiles = unique_tiles(n);
-    w.tile_counts.ensure(tiles * 8, s);
-    w.tile_offs.ensure((tiles + 1) * 8, s);
-    w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
-    DBuf tmp_out;
-    tmp_out.alloc(n * 8, s);
-    {
-        Timer t(s, "unique", 16.0 * (double)n);
-        launch_unique(s, sorted, n, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                      (uint64_t *)w.partial.p, (uint64_t *)tmp_out.p);
-    }
-    uint64_t m = 0;
-    HIPCHK(hipMemcpyAsync(&m, (uint64_t *)w.tile_offs.p + tiles, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (m == n) {
-        out = std::move(tmp_out);
-        out.bytes = n * 8;
-    } else {
-        out.alloc(m * 8, s);
-        if (m) HIPCHK(hipMemcpyAsync(out.p, tmp_out.p, m * 8, hipMemcpyDeviceToDevice, s));
-    }
-    return m;
 }
 
 // ---------------------------------------------------------------------------
@@ -344,90 +316,19 @@ PatternDev make_pattern(const egtb_pattern *pat, int target) {
     return p;
 }
 
-static uint32_t index_bits_for(uint64_t n) {
-    uint32_t b = 1;
-    while (b < 24 && (1ull << (b + 3)) < n) ++b;
-    return b;
-}
-
-template <typename T>
-void make_table(hipStream_t s, Book &bk) {
-    bk.bits = index_bits_for(bk.n);
-    bk.idx.alloc(((1ull << bk.bits) + 1) * 4, s);
-    launch_bucket_index(s, (const uint64_t *)bk.keys.p, bk.n, bk.bits, (uint32_t *)bk.idx.p);
-}
-
-template <typename T>
-TableDev<T> table_of(const Book &bk) {
-    TableDev<T> t;
-    t.keys = (const uint64_t *)bk.keys.p;
-    t.rates = (const T *)bk.rates.p;
-    t.idx = (const uint32_t *)bk.idx.p;
-    t.n = bk.n;
-    t.bits = bk.bits;
-    t.pad = 0;
-    return t;
-}
-
-// keys[0..n) with rates -> compacted book (rate > 0); returns max rate bits.
-template <typename T>
-uint64_t compact_into(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, SortWork &w, Book &out) {
-    out.rates_size = sizeof(T);
-    if (n == 0) {
-        out.n = 0;
-        out.keys.alloc(0, s);
-        out.rates.alloc(0, s);
-        return 0;
-    }
-    const uint64_t tiles = compact_tiles(n);
-    w.tile_counts.ensure(tiles * 8, s);
-    w.tile_offs.ensure((tiles + 1) * 8, s);
-    w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
-    w.maxbits.ensure(8, s);
-    HIPCHK(hipMemsetAsync(w.maxbits.p, 0, 8, s));
-    DBuf k, r;
-    k.alloc(n * 8, s);
-    r.alloc(n * sizeof(T), s);
-    {
-        Timer t(s, "compact", (double)n * (sizeof(T) + 8) * 2);
-        launch_compact<T>(s, keys, rates, n, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                          (uint64_t *)w.partial.p, (uint64_t *)k.p, (T *)r.p,
-                          (unsigned long long *)w.maxbits.p);
-    }
-    uint64_t host[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&host[0], (uint64_t *)w.tile_offs.p + tiles, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&host[1], w.maxbits.p, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    out.n = host[0];
-    out.keys = std::move(k);
-    out.rates = std::move(r);
-    return host[1];
-}
-
-template <typename T>
-void write_book_file(hipStream_t s, const std::string &path, const Book &bk, HostStage &stage) {
-    constexpr size_t rec = 8 + sizeof(T);
-    const size_t bytes = bk.n * rec;
-    DBuf packed;
-    if (bk.n) {
-        packed.alloc(bytes, s);
-        launch_pack_records<T>(s, (const uint64_t *)bk.keys.p, (const T *)bk.rates.p, bk.n, (uint32_t *)packed.p);
-        stage.ensure(bytes);
-        HIPCHK(hipMemcpyAsync(stage.p, packed.p, bytes, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-    }
-    write_file_atomic(path, stage.p, bytes);
-}
+// ---------------------------------------------------------------------------
+// Pinned host staging
+// ---------------------------------------------------------------------------
 
 void HostStage::ensure(size_t n) {
-    if (n <= bytes) return;
+    if (n <= bytes && p) return;
     if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
     const size_t want = n + n / 4 + 4096;
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
         p = nullptr;
-        throw Error(EGTB_ERR_MEMORY, "pinned host allocation failed");
+        throw Error(EGTB_ERR_MEMORY, "pinned host allocation of " + std::to_string(want) + " bytes failed");
     }
     bytes = want;
 }
@@ -436,6 +337,136 @@ HostStage::~HostStage() {
     if (p) (void)hipHostFree(p);
 }
 
+// ---------------------------------------------------------------------------
+// Arena
+// ---------------------------------------------------------------------------
+
+void *Arena::alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (chunks_.empty() || chunks_.back().cap - chunks_.back().used < bytes) {
+        size_t cap = bytes > min_chunk_ ? bytes : min_chunk_;
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, cap);
+        if (e != hipSuccess)
+            throw Error(EGTB_ERR_MEMORY, "device allocation of " + std::to_string(cap) + " bytes failed: " +
+                                             hipGetErrorString(e));
+        chunks_.push_back(Chunk{(char *)p, cap, 0});
+    }
+    Chunk &c = chunks_.back();
+    void *p = c.p + c.used;
+    c.used += bytes;
+    return p;
+}
+
+void Arena::trim_last(void *p, size_t bytes) {
+    if (chunks_.empty() || !p) return;
+    Chunk &c = chunks_.back();
+    const size_t off = (size_t)((char *)p - c.p);
+    if (off > c.used) return;  // not in the newest chunk
+    const size_t end = off + ((bytes + 255) & ~(size_t)255);
+    if (end < c.used) c.used = end;
+}
+
+void Arena::release() {
+    for (auto &c : chunks_) (void)hipFree(c.p);
+    chunks_.clear();
+}
+
+size_t Arena::used() const {
+    size_t u = 0;
+    for (auto &c : chunks_) u += c.used;
+    return u;
+}
+
+static uint32_t index_bits_for(uint64_t n) {
+    uint32_t b = 1;
+    while (b < 24 && (1ull << (b + 3)) < n) ++b;
+    return b;
+}
+
+template <typename T>
+static TableDev<T> table_of(const Book &bk) {
+    TableDev<T> t;
+    t.keys = bk.keys;
+    t.rates = (const T *)bk.rates;
+    t.idx = bk.idx;
+    t.n_ptr = bk.n_dev;
+    t.n = bk.n;
+    t.bits = bk.bits;
+    t.present = 1;
+    return t;
+}
+
+// Carve book storage for a layer of at most `bound` live records, compact
+// (keys, rates) into it (stable, rate > 0) and index it.  Sizes stay on the
+// device; nothing is synchronised.
+template <typename T>
+static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *keys, const T *rates, uint64_t bound,
+                      SortWork &w, bool index) {
+    Book &bk = res.books[layer];
+    LayerDev *md = res.meta_dev() + layer;
+    bk.n_dev = &md->n_live;
+    bk.keys = (uint64_t *)res.book_arena.alloc(bound * 8);
+    bk.rates = res.book_arena.alloc(bound * sizeof(T));
+    if (bound) {
+        const uint64_t tiles = compact_tiles(bound);
+        w.tile_counts.ensure(tiles * 8, s);
+        w.tile_offs.ensure((tiles + 1) * 8, s);
+        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+    }
+    {
+        Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2);
+        launch_compact<T>(s, keys, rates, bound, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
+                          (uint64_t *)w.partial.p, bk.keys, (T *)bk.rates, (unsigned long long *)&md->maxbits,
+                          &md->n_live);
+    }
+    if (index) {
+        bk.bits = index_bits_for(bound);
+        bk.idx = (uint32_t *)res.book_arena.alloc(((1ull << bk.bits) + 1) * 4);
+        launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
+    }
+}
+
+template <typename T>
+static void write_book_file(hipStream_t s, const std::string &path, const Book &bk, uint64_t n, HostStage &stage,
+                            DBuf &packed) {
+    constexpr size_t rec = 8 + sizeof(T);
+    const size_t bytes = n * rec;
+    if (n) {
+        packed.ensure(bytes, s);
+        launch_pack_records<T>(s, bk.keys, (const T *)bk.rates, n, (uint32_t *)packed.p);
+        stage.ensure(bytes);
+        HIPCHK(hipMemcpyAsync(stage.p, packed.p, bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    write_file_atomic(path, stage.p, bytes);
+}
+
+template <typename T>
+static double rate_unit(uint64_t bits) {
+    T v;
+    if (sizeof(T) == 4) {
+        uint32_t b32 = (uint32_t)bits;
+        std::memcpy(&v, &b32, 4);
+    } else {
+        std::memcpy(&v, &bits, 8);
+    }
+    return RateTraits<T>::to_unit(v);
+}
+
+// Copy layer records [lo, hi] of the device metadata into the pinned mirror
+// and wait for them.
+static void read_meta(hipStream_t s, Result &res, LayerDev *host, int lo, int hi) {
+    HIPCHK(hipMemcpyAsync(host + lo, res.meta_dev() + lo, (size_t)(hi - lo + 1) * sizeof(LayerDev),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// The plain per-layer schedule.  Forward: one host synchronisation per step
+// (after the count pass, to size the candidate streams); the spawn-4 stream
+// of step i lands at the front of the candidate buffer of layer i+1 and the
+// spawn-2 stream of step i+1 is appended behind it, so layer i+1 is one sort
+// + unique of that buffer.  Backward: sizes stay on the device, no sync.
 template <typename T>
 void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const egtb_options &o, Result &res) {
     DeviceCtx &C = device_ctx(o.device);
@@ -448,143 +479,173 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     const uint32_t prog_total = (uint32_t)(2 * steps);
     progress_set(0, prog_total);
 
+    res.dev = o.device;
     res.steps = steps;
     res.rate_size = sizeof(T);
-    res.raw.clear();
-    res.raw.resize(steps);
-    res.books.clear();
-    res.books.resize(steps);
+    res.rate_is_float = std::is_floating_point<T>::value;
+    res.raw.assign(steps, nullptr);
+    res.books.assign(steps, Book{});
     res.info.assign(steps, LayerInfo{});
     SortWork w;
-    HostStage stage;
+    HostStage stage, meta_host_buf;
+    DBuf cand[2], sort_tmp, counts, offs, rates, packed;
 
     const auto t_begin = std::chrono::steady_clock::now();
 
+    // per-layer device records: zero sizes, OR = 0, AND = all ones
+    meta_host_buf.ensure((size_t)(steps + 1) * sizeof(LayerDev));
+    LayerDev *H = (LayerDev *)meta_host_buf.p;
+    for (int i = 0; i <= steps; ++i) {
+        std::memset(&H[i], 0, sizeof(LayerDev));
+        H[i].and_all = ~0ull;
+    }
+    H[0].n_raw = n_seeds;
+    res.meta.alloc((size_t)(steps + 1) * sizeof(LayerDev), s);
+    HIPCHK(hipMemcpyAsync(res.meta.p, H, (size_t)(steps + 1) * sizeof(LayerDev), hipMemcpyHostToDevice, s));
+    LayerDev *M = res.meta_dev();
+
     // ---- forward (reference generate_process, BookGenerator.cpp:963-1121)
-    std::vector<DBuf> &raw = res.raw;
-    raw[0].alloc(n_seeds * 8, s);
-    if (n_seeds) HIPCHK(hipMemcpyAsync(raw[0].p, seeds, n_seeds * 8, hipMemcpyHostToDevice, s));
+    res.raw[0] = (uint64_t *)res.raw_arena.alloc(n_seeds * 8);
+    if (n_seeds) HIPCHK(hipMemcpyAsync(res.raw[0], seeds, n_seeds * 8, hipMemcpyHostToDevice, s));
     res.info[0].n_raw = n_seeds;
 
-    DBuf partial;  // unsorted spawn-4 children of the previous step (next layer)
-    uint64_t partial_n = 0, partial_or = 0, partial_and = ~0ull;
-    DBuf counts, offs, spans;
-    spans.alloc(4 * 8, s);
+    auto launch_count = [&](int step, uint64_t bound) {
+        // count pass of forward step `step` over layer step-1 (size on device)
+        counts.ensure(bound * 8 + 8, s);
+        offs.ensure(bound * 8 + 8, s);
+        w.partial.ensure(scan_partials_needed(bound) * 8 + 64, s);
+        {
+            Timer t(s, "expand_count", 8.0 * (double)bound);
+            launch_expand(s, false, C.shape, res.raw[step - 1], bound, &M[step - 1].n_raw, C.lut, p,
+                          step > o.docheck_step ? 1 : 0, (uint64_t *)counts.p, nullptr, nullptr, nullptr,
+                          &M[step].or_all, &M[step + 1].or_all);
+        }
+        launch_scan_u64(s, (const uint64_t *)counts.p, bound, &M[step - 1].n_raw, (uint64_t *)offs.p,
+                        (uint64_t *)w.partial.p, &M[step].tot);
+    };
+
+    uint64_t c4_prev = 0;
+    if (steps >= 3) launch_count(1, n_seeds);
     for (int i = 1; i <= steps - 2; ++i) {
         progress_set((uint32_t)i, prog_total);
-        const uint64_t n = res.info[i - 1].n_raw;
-        const int do_check = i > o.docheck_step ? 1 : 0;
-        uint64_t c2 = 0, c4 = 0;
-        uint64_t span[4] = {0, ~0ull, 0, ~0ull};
-        if (n) {
-            counts.ensure(n * 8, s);
-            offs.ensure((n + 1) * 8, s);
-            w.partial.ensure(scan_partials_needed(n) * 8 + 64, s);
-            HIPCHK(hipMemcpyAsync(spans.p, span, sizeof span, hipMemcpyHostToDevice, s));
-            {
-                Timer t(s, "expand_count", 8.0 * (double)n);
-                launch_expand(s, false, C.shape, (const uint64_t *)raw[i - 1].p, n, C.lut, p, do_check,
-                              (uint64_t *)counts.p, nullptr, nullptr, nullptr, (uint64_t *)spans.p,
-                              (uint64_t *)spans.p + 2);
-            }
-            launch_scan_u64(s, (const uint64_t *)counts.p, n, (uint64_t *)offs.p, (uint64_t *)w.partial.p);
-            uint64_t tot = 0;
-            HIPCHK(hipMemcpyAsync(&tot, (uint64_t *)offs.p + n, 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(span, spans.p, sizeof span, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            c2 = tot & 0xFFFFFFFFull;
-            c4 = tot >> 32;
-        }
-        DBuf cand, next_partial;
-        cand.alloc((partial_n + c2) * 8, s);
-        next_partial.alloc(c4 * 8, s);
-        if (partial_n) HIPCHK(hipMemcpyAsync(cand.p, partial.p, partial_n * 8, hipMemcpyDeviceToDevice, s));
-        if (n && (c2 || c4)) {
-            Timer t(s, "expand", 8.0 * (double)n + 8.0 * (double)(c2 + c4));
-            launch_expand(s, true, C.shape, (const uint64_t *)raw[i - 1].p, n, C.lut, p, do_check, nullptr,
-                          (const uint64_t *)offs.p, (uint64_t *)cand.p + partial_n, (uint64_t *)next_partial.p,
-                          nullptr, nullptr);
-        }
-        HIPCHK(hipGetLastError());
-        partial.release();
-        DBuf tmp;
-        tmp.alloc((partial_n + c2) * 8, s);
-        const uint64_t m = sort_unique(s, cand, tmp, partial_n + c2, span[0] | partial_or, span[1] & partial_and,
-                                       o.radix_bits, w, raw[i]);
-        res.info[i].n_raw = m;
+        read_meta(s, res, H, i - 1, i);
+        const uint64_t n_par = H[i - 1].n_raw;
+        if (i > 1) res.raw_arena.trim_last(res.raw[i - 1], n_par * 8);
+        res.info[i - 1].n_raw = n_par;
+        const uint64_t c2 = H[i].tot & 0xFFFFFFFFull, c4 = H[i].tot >> 32;
         res.info[i].c2 = c2;
         res.info[i].c4 = c4;
-        partial = std::move(next_partial);
-        partial_n = c4;
-        partial_or = span[2];
-        partial_and = span[3];
+        const uint64_t n_c = c4_prev + c2;
+        DBuf &cur = cand[i & 1], &nxt = cand[(i + 1) & 1];
+        if (cur.bytes < n_c * 8) {  // grow, keeping the spawn-4 prefix
+            DBuf g;
+            g.alloc(n_c * 8 + n_c * 4 + 256, s);
+            if (c4_prev) HIPCHK(hipMemcpyAsync(g.p, cur.p, c4_prev * 8, hipMemcpyDeviceToDevice, s));
+            cur = std::move(g);
+        }
+        nxt.ensure(c4 * 8, s);
+        sort_tmp.ensure(n_c * 8, s);
+        if (n_par && (c2 || c4)) {
+            Timer t(s, "expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
+            launch_expand(s, true, C.shape, res.raw[i - 1], n_par, nullptr, C.lut, p, i > o.docheck_step ? 1 : 0,
+                          nullptr, (const uint64_t *)offs.p, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p,
+                          nullptr, nullptr);
+        }
+        uint64_t *sorted;
+        {
+            Timer t(s, "sort", 16.0 * (double)n_c);
+            sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all,
+                                o.radix_bits, w);
+        }
+        res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
+        {
+            const uint64_t tiles = unique_tiles(n_c);
+            w.tile_counts.ensure(tiles * 8, s);
+            w.tile_offs.ensure((tiles + 1) * 8, s);
+            w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+            Timer t(s, "unique", 16.0 * (double)n_c);
+            launch_unique(s, sorted, n_c, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
+                          (uint64_t *)w.partial.p, res.raw[i], &M[i].n_raw);
+        }
+        if (i + 1 <= steps - 2) launch_count(i + 1, n_c);
+        HIPCHK(hipGetLastError());
+        c4_prev = c4;
     }
+    // layer steps-1: only the spawn-4 children of layer steps-3
     {
-        DBuf tmp;
-        tmp.alloc(partial_n * 8, s);
-        res.info[steps - 1].n_raw =
-            sort_unique(s, partial, tmp, partial_n, partial_or, partial_and, o.radix_bits, w, raw[steps - 1]);
-        partial.release();
+        const int L = steps - 1;
+        read_meta(s, res, H, L - 1, L);
+        res.raw_arena.trim_last(res.raw[L - 1], H[L - 1].n_raw * 8);
+        res.info[L - 1].n_raw = H[L - 1].n_raw;
+        const uint64_t n_c = c4_prev;
+        DBuf &cur = cand[L & 1];
+        sort_tmp.ensure(n_c * 8, s);
+        uint64_t *sorted;
+        {
+            Timer t(s, "sort", 16.0 * (double)n_c);
+            sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all,
+                                o.radix_bits, w);
+        }
+        res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
+        const uint64_t tiles = unique_tiles(n_c);
+        w.tile_counts.ensure(tiles * 8, s);
+        w.tile_offs.ensure((tiles + 1) * 8, s);
+        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+        launch_unique(s, sorted, n_c, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
+                      (uint64_t *)w.partial.p, res.raw[L], &M[L].n_raw);
+        read_meta(s, res, H, L, L);
+        res.raw_arena.trim_last(res.raw[L], H[L].n_raw * 8);
+        res.info[L].n_raw = H[L].n_raw;
     }
+    cand[0].release();
+    cand[1].release();
+    sort_tmp.release();
     counts.release();
     offs.release();
-    HIPCHK(hipStreamSynchronize(s));
     const auto t_fwd = std::chrono::steady_clock::now();
 
     // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
+    uint64_t max_n = 0;
+    for (int i = 0; i < steps; ++i) max_n = std::max(max_n, res.info[i].n_raw);
+    rates.ensure(max_n * sizeof(T) + 8, s);
     for (int i = steps - 2; i <= steps - 1; ++i) {
         const uint64_t n = res.info[i].n_raw;
-        DBuf r;
-        r.alloc(n * sizeof(T), s);
-        launch_final_rates<T>(s, (const uint64_t *)raw[i].p, n, p, max_scale, (T *)r.p);
-        compact_into<T>(s, (const uint64_t *)raw[i].p, (const T *)r.p, n, w, res.books[i]);
-        res.info[i].n_live = res.books[i].n;
-        make_table<T>(s, res.books[i]);
-        if (files) write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], stage);
-        if (!o.keep_raw) raw[i].release();
+        launch_final_rates<T>(s, res.raw[i], n, p, max_scale, (T *)rates.p);
+        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, true);
+    }
+    if (files) {
+        read_meta(s, res, H, steps - 2, steps - 1);
+        for (int i = steps - 2; i <= steps - 1; ++i)
+            write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], H[i].n_live, stage, packed);
+        if (!file_exists(prefix + "stats.txt")) {
+            FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
+            if (!f) throw Error(EGTB_ERR_IO, "cannot open " + prefix + "stats.txt");
+            std::fputs("layer,length,max success rate,speed,deletion_threshold,time\n", f);
+            std::fclose(f);
+        }
     }
 
     // ---- backward (reference recalculate_process_impl, BookSolver.cpp:513-605)
-    if (files && !file_exists(prefix + "stats.txt")) {
-        const char *hdr = "layer,length,max success rate,speed,deletion_threshold,time\n";
-        FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
-        if (!f) throw Error(EGTB_ERR_IO, "cannot open " + prefix + "stats.txt");
-        std::fputs(hdr, f);
-        std::fclose(f);
-    }
     for (int i = steps - 3; i >= 0; --i) {
         progress_set(prog_total - (uint32_t)i - 2u, prog_total);
         const auto l0 = std::chrono::steady_clock::now();
         const uint64_t n = res.info[i].n_raw;
-        DBuf rates;
-        rates.alloc(n * sizeof(T), s);
         const Book &b1 = res.books[i + 1], &b2 = res.books[i + 2];
         {
-            Timer t(s, "solve", (double)n * (8 + sizeof(T)) + (double)(b1.n + b2.n) * (8 + sizeof(T)));
-            launch_solve<T>(s, C.shape, (const uint64_t *)raw[i].p, n, C.lut, p, i > o.docheck_step ? 1 : 0,
-                            table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p);
+            Timer t(s, "solve", (double)n * (8 + sizeof(T)));
+            launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, table_of<T>(b1),
+                            table_of<T>(b2), w2, w4, max_scale, (T *)rates.p);
         }
+        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0);
         HIPCHK(hipGetLastError());
-        const uint64_t maxbits = compact_into<T>(s, (const uint64_t *)raw[i].p, (const T *)rates.p, n, w,
-                                                 res.books[i]);
-        res.info[i].n_live = res.books[i].n;
-        if (i > 0) make_table<T>(s, res.books[i]);
-        rates.release();
-        if (!o.keep_raw) raw[i].release();
-        const auto l1 = std::chrono::steady_clock::now();
         if (files) {
-            if (res.books[i].n) {
-                T mx;
-                if (sizeof(T) == 4) {
-                    uint32_t b32 = (uint32_t)maxbits;
-                    std::memcpy(&mx, &b32, 4);
-                } else {
-                    std::memcpy(&mx, &maxbits, 8);
-                }
+            read_meta(s, res, H, i, i);
+            const auto l1 = std::chrono::steady_clock::now();
+            if (H[i].n_live) {
                 const double dt = std::chrono::duration<double>(l1 - l0).count();
                 char row[256];
                 std::snprintf(row, sizeof row, "%d,%llu,%g,%g mbps,%g,%s\n", i, (unsigned long long)n,
-                              RateTraits<T>::to_unit(mx), round_to_2((double)n / std::max(dt, 0.01) / 1e6), 0.0,
+                              rate_unit<T>(H[i].maxbits), round_to_2((double)n / std::max(dt, 0.01) / 1e6), 0.0,
                               now_string().c_str());
                 FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
                 if (f) {
@@ -592,12 +653,27 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                     std::fclose(f);
                 }
             }
-            write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], stage);
+            write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], H[i].n_live, stage, packed);
         }
-        if (!res.keep_tables) res.books[i + 2] = Book{};
     }
-    HIPCHK(hipStreamSynchronize(s));
+    read_meta(s, res, H, 0, steps - 1);
+    for (int i = 0; i < steps; ++i) {
+        res.info[i].n_live = H[i].n_live;
+        res.books[i].n = H[i].n_live;
+    }
+    {
+        // each solve launch streams the two tables it looks up in once
+        double table_bytes = 0;
+        for (int i = 0; i <= steps - 3; ++i)
+            table_bytes += (double)(H[i + 1].n_live + H[i + 2].n_live) * (8 + sizeof(T));
+        stats_add_bytes("solve", table_bytes);
+    }
     const auto t_end = std::chrono::steady_clock::now();
+    if (!o.keep_raw) {
+        res.raw_arena.release();
+        res.raw.assign(steps, nullptr);
+    }
+    if (!res.keep_tables) res.book_arena.release();
     res.fwd_ms = std::chrono::duration<double, std::milli>(t_fwd - t_begin).count();
     res.bwd_ms = std::chrono::duration<double, std::milli>(t_end - t_fwd).count();
     res.total_ms = std::chrono::duration<double, std::milli>(t_end - t_begin).count();
@@ -703,10 +779,10 @@ int egtb_result_raw(const egtb_result *r, int layer, uint64_t *keys, size_t cap,
         if (!r || layer < 0 || layer >= r->steps) throw Error(EGTB_ERR_ARG, "layer out of range");
         const uint64_t m = r->info[layer].n_raw;
         if (n) *n = m;
-        if (m && !r->raw[layer].p) throw Error(EGTB_ERR_ARG, "raw layers were not kept (options.keep_raw)");
+        if (m && !r->raw[layer]) throw Error(EGTB_ERR_ARG, "raw layers were not kept (options.keep_raw)");
         if (!keys) return;
         if (cap < m) throw Error(EGTB_ERR_ARG, "buffer too small");
-        if (m) HIPCHK(hipMemcpy(keys, r->raw[layer].p, m * 8, hipMemcpyDeviceToHost));
+        if (m) HIPCHK(hipMemcpy(keys, r->raw[layer], m * 8, hipMemcpyDeviceToHost));
     });
 }
 
@@ -717,8 +793,8 @@ int egtb_result_book(const egtb_result *r, int layer, uint64_t *keys, void *rate
         if (n) *n = b.n;
         if (!keys && !rates) return;
         if (cap < b.n) throw Error(EGTB_ERR_ARG, "buffer too small");
-        if (b.n && keys) HIPCHK(hipMemcpy(keys, b.keys.p, b.n * 8, hipMemcpyDeviceToHost));
-        if (b.n && rates) HIPCHK(hipMemcpy(rates, b.rates.p, b.n * r->rate_size, hipMemcpyDeviceToHost));
+        if (b.n && keys) HIPCHK(hipMemcpy(keys, b.keys, b.n * 8, hipMemcpyDeviceToHost));
+        if (b.n && rates) HIPCHK(hipMemcpy(rates, b.rates, b.n * r->rate_size, hipMemcpyDeviceToHost));
     });
 }
 
@@ -733,10 +809,9 @@ int egtb_result_timing(const egtb_result *r, double *fwd_ms, double *bwd_ms, dou
 void egtb_result_free(egtb_result *r) {
     if (!r) return;
     try {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        delete r;
+        (void)hipSetDevice(r->dev);
         (void)hipDeviceSynchronize();
+        delete r;
     } catch (...) {
     }
 }
@@ -780,7 +855,7 @@ size_t unique_sorted_u64_inplace(uint64_t *data, size_t size) {
         w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
         out.alloc(size * 8, s);
         launch_unique(s, (const uint64_t *)a.p, size, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                      (uint64_t *)w.partial.p, (uint64_t *)out.p);
+                      (uint64_t *)w.partial.p, (uint64_t *)out.p, nullptr);
         uint64_t mm = 0;
         HIPCHK(hipMemcpyAsync(&mm, (uint64_t *)w.tile_offs.p + tiles, 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

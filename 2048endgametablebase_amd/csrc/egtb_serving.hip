@@ -1,0 +1,438 @@
+// Solved tables resident in HBM and batched position lookups.
+//
+// A table set holds every solved layer of one build (pattern, target) as
+// sorted keys + rates + bucket index.  A lookup answers the reference's
+// ClassicBookReader::move_on_dic (ReaderRuntime.cpp:739-840) for a batch of
+// boards: layer = (tile sum - seed tile sum) / 2; the first symmetry op
+// (0..7) under which the board is in the pattern and has a valid move gives
+// the four directional rates (NaN = invalid move), remapped to absolute
+// directions.  Boards that no op answers take a depth-2 expectimax fallback
+// with a fixed empty-cell heuristic (status EGTB_LOOKUP_FALLBACK).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "egtb.h"
+#include "egtb_device.h"
+#include "egtb_kernels.h"
+#include "egtb_runtime.h"
+#include "egtb_serving.h"
+
+namespace egtb {
+
+// absolute direction (left, right, up, down) of move d of the board under
+// symmetry op `op` (reference adjust_direction, ReaderRuntime.cpp:149-169)
+__constant__ uint8_t c_absdir[8][4] = {{0, 1, 2, 3}, {2, 3, 1, 0}, {1, 0, 3, 2}, {3, 2, 0, 1},
+                                       {1, 0, 2, 3}, {3, 2, 1, 0}, {0, 1, 3, 2}, {2, 3, 0, 1}};
+
+// The reader's board operations (apply_operation, ReaderRuntime.cpp:127-147)
+// on the nibble encoding: 1 rotate left, 2 rotate 180, 3 rotate right,
+// 4 flip horizontal, 5-7 flip horizontal after 1-3.
+__device__ __forceinline__ uint64_t apply_op(uint64_t b, int op) {
+    switch (op) {
+        case 1: return mirror_ud(transpose(b));
+        case 2: return mirror_ud(mirror_lr(b));
+        case 3: return mirror_lr(transpose(b));
+        case 4: return mirror_lr(b);
+        case 5: return mirror_lr(mirror_ud(transpose(b)));
+        case 6: return mirror_ud(b);
+        case 7: return transpose(b);
+        default: return b;
+    }
+}
+
+__device__ __forceinline__ uint32_t tile_sum(uint64_t b) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t v = (b >> (4 * k)) & 0xF;
+        s += v ? (1u << v) : 0u;
+    }
+    return s;
+}
+
+// Heuristic of the fallback: fraction of empty cells.
+__device__ __forceinline__ double fallback_h(uint64_t b) { return (double)__popc(empty_cells(b)) / 16.0; }
+
+// Depth-2 expectimax value of playing move d on b: mean over the spawns on
+// the moved board (2 with 1-p4, 4 with p4, uniform cell) of the best next
+// move's heuristic (0 when the spawned board has no move).
+__device__ double fallback_value(uint64_t moved, const uint16_t *lut, double p4) {
+    uint32_t em = empty_cells(moved);
+    const int e = __popc(em);
+    if (e == 0) return fallback_h(moved);
+    double acc = 0.0;
+    while (em) {
+        const int k = __builtin_ctz(em);
+        em &= em - 1;
+#pragma unroll
+        for (int s = 1; s <= 2; ++s) {
+            const uint64_t sp = moved | ((uint64_t)s << (4 * k));
+            uint64_t mv[4];
+            move4(sp, lut, mv);
+            double best = 0.0;
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+                if (mv[d] != sp) best = fmax(best, fallback_h(mv[d]));
+            acc += (s == 1 ? 1.0 - p4 : p4) * best;
+        }
+    }
+    return acc / (double)e;
+}
+
+template <typename T, typename OutT>
+__global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ boards, uint64_t n,
+                                                 const TableDev<T> *__restrict__ tabs, int steps,
+                                                 int64_t nums_adjust, PatternDev p, double scale, double p4,
+                                                 const uint16_t *__restrict__ glut, OutT *__restrict__ out4,
+                                                 int32_t *__restrict__ status, int32_t *__restrict__ op_out) {
+    extern __shared__ uint16_t lut[];
+    stage_lut(lut, glut);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        const uint64_t b = boards[q];
+        const OutT nanv = (OutT)NAN;
+        OutT v[4] = {nanv, nanv, nanv, nanv};
+        int st = EGTB_LOOKUP_FALLBACK, used = -1;
+        const int64_t nums = ((int64_t)tile_sum(b) + nums_adjust) / 2;
+        if (nums >= 0 && nums < steps && tabs[nums].present) {
+            const TableDev<T> &t = tabs[nums];
+            for (int op = 0; op < 8 && used < 0; ++op) {
+                const uint64_t x = apply_op(b, op);
+                if (!is_pattern(x, p)) continue;
+                uint64_t mv[4];
+                move4(x, lut, mv);
+                bool any = false;
+                OutT r[4] = {nanv, nanv, nanv, nanv};
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    if (mv[d] != x && is_pattern(mv[d], p)) {
+                        const T raw = table_lookup_dev(t, canonical(mv[d], p.symm));
+                        r[c_absdir[op][d]] = (OutT)((double)raw / scale);
+                        any = true;
+                    }
+                }
+                if (any) {
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) v[d] = r[d];
+                    used = op;
+                    st = EGTB_LOOKUP_OK;
+                }
+            }
+        }
+        if (used < 0) {
+            st = EGTB_LOOKUP_FALLBACK;
+            uint64_t mv[4];
+            move4(b, lut, mv);
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+                if (mv[d] != b) v[d] = (OutT)fallback_value(mv[d], lut, p4);
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) out4[q * 4 + d] = v[d];
+        if (status) status[q] = st;
+        if (op_out) op_out[q] = used;
+    }
+}
+
+void configure_serving_kernels() {
+    const int lds = kLutEntries * sizeof(uint16_t);
+    (void)hipFuncSetAttribute((const void *)k_lookup<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_lookup<float, double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_lookup<double, float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_lookup<double, double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lds);
+    (void)hipFuncSetAttribute((const void *)k_lookup<uint32_t, float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lds);
+    (void)hipFuncSetAttribute((const void *)k_lookup<uint32_t, double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_lookup<uint64_t, float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lds);
+    (void)hipFuncSetAttribute((const void *)k_lookup<uint64_t, double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+template <typename T, typename OutT>
+static void launch_lookup_t(hipStream_t s, const TableSet &ts, const DeviceCtx &C, const uint64_t *boards,
+                            uint64_t n, OutT *out4, int32_t *status, int32_t *op_out) {
+    // the 128 KiB LDS row table allows one block per CU: 1024 threads each
+    const uint32_t threads = 1024;
+    uint64_t want = (n + threads - 1) / threads;
+    const uint64_t cap = (uint64_t)C.cus;
+    const uint32_t grid = (uint32_t)(want < cap ? want : cap);
+    hipLaunchKernelGGL((k_lookup<T, OutT>), dim3(grid ? grid : 1), dim3(threads), kLutEntries * sizeof(uint16_t), s,
+                       boards, n, (const TableDev<T> *)ts.tabs.p, ts.steps, ts.nums_adjust, ts.pattern, ts.scale,
+                       ts.p4, C.lut, out4, status, op_out);
+}
+
+template <typename OutT>
+void launch_lookup(hipStream_t s, const TableSet &ts, const uint64_t *boards, uint64_t n, OutT *out4,
+                   int32_t *status, int32_t *op_out) {
+    if (n == 0) return;
+    const DeviceCtx &C = device_ctx(ts.dev);
+    switch (ts.rate_size * 10 + (ts.is_float ? 1 : 0)) {
+        case 41: launch_lookup_t<float, OutT>(s, ts, C, boards, n, out4, status, op_out); break;
+        case 81: launch_lookup_t<double, OutT>(s, ts, C, boards, n, out4, status, op_out); break;
+        case 40: launch_lookup_t<uint32_t, OutT>(s, ts, C, boards, n, out4, status, op_out); break;
+        default: launch_lookup_t<uint64_t, OutT>(s, ts, C, boards, n, out4, status, op_out); break;
+    }
+}
+
+template void launch_lookup<float>(hipStream_t, const TableSet &, const uint64_t *, uint64_t, float *, int32_t *,
+                                   int32_t *);
+template void launch_lookup<double>(hipStream_t, const TableSet &, const uint64_t *, uint64_t, double *,
+                                    int32_t *, int32_t *);
+
+// ---------------------------------------------------------------------------
+// Table sets
+// ---------------------------------------------------------------------------
+
+struct DTypeDesc {
+    int size;
+    bool is_float;
+    double scale;  // reader normalisation (ReaderRuntime.cpp:183-206,271-277)
+};
+
+static DTypeDesc dtype_desc(const std::string &dt) {
+    if (dt == "float32") return {4, true, 1.0};
+    if (dt == "float64") return {8, true, 1.0};
+    if (dt == "uint32") return {4, false, 4e9};
+    if (dt == "uint64") return {8, false, 1.6e18};
+    throw Error(EGTB_ERR_ARG, "success_rate_dtype '" + dt + "' is not supported");
+}
+
+static void finish_tables(TableSet &ts, hipStream_t s, const std::vector<TableDev<char>> &host_tabs) {
+    ts.tabs.alloc(host_tabs.size() * sizeof(TableDev<char>), s);
+    HIPCHK(hipMemcpyAsync(ts.tabs.p, host_tabs.data(), host_tabs.size() * sizeof(TableDev<char>),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+static uint32_t table_index_bits(uint64_t n) {
+    uint32_t b = 1;
+    while (b < 24 && (1ull << (b + 3)) < n) ++b;
+    return b;
+}
+
+std::unique_ptr<TableSet> table_from_files(const char *prefix, int steps, const egtb_pattern *pat, int target,
+                                           int64_t tile_sum, const char *dtype, double p4, int device) {
+    if (!prefix) throw Error(EGTB_ERR_ARG, "prefix is NULL");
+    if (steps < 1) throw Error(EGTB_ERR_ARG, "steps must be positive");
+    const DTypeDesc dd = dtype_desc(dtype ? dtype : "float32");
+    DeviceCtx &C = device_ctx(device);
+    configure_serving_kernels();
+    hipStream_t s = C.stream;
+    auto ts = std::make_unique<TableSet>();
+    ts->dev = device;
+    ts->steps = steps;
+    ts->pattern = make_pattern(pat, target);
+    ts->nums_adjust = -tile_sum;
+    ts->rate_size = dd.size;
+    ts->is_float = dd.is_float;
+    ts->scale = dd.scale;
+    ts->p4 = p4;
+    ts->n.assign(steps, 0);
+    const size_t rec = 8 + dd.size;
+    std::vector<TableDev<char>> tabs(steps);
+    HostStage stage;
+    DBuf recs;
+    for (int i = 0; i < steps; ++i) {
+        TableDev<char> &t = tabs[i];
+        std::memset(&t, 0, sizeof t);
+        t.bits = 1;
+        const std::string path = std::string(prefix) + std::to_string(i) + ".book";
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) continue;  // absent layer: lookups there read as off-table
+        t.present = 1;
+        std::fseek(f, 0, SEEK_END);
+        const long sz = std::ftell(f);
+        std::fseek(f, 0, SEEK_SET);
+        if (sz < 0 || (size_t)sz % rec) {
+            std::fclose(f);
+            throw Error(EGTB_ERR_IO, path + ": size is not a multiple of the record size");
+        }
+        const uint64_t n = (uint64_t)sz / rec;
+        stage.ensure((size_t)sz + 16);
+        const size_t got = n ? std::fread(stage.p, 1, (size_t)sz, f) : 0;
+        std::fclose(f);
+        if (got != (size_t)sz) throw Error(EGTB_ERR_IO, "short read of " + path);
+        ts->n[i] = n;
+        if (n == 0) continue;
+        recs.ensure((size_t)sz, s);
+        HIPCHK(hipMemcpyAsync(recs.p, stage.p, (size_t)sz, hipMemcpyHostToDevice, s));
+        uint64_t *keys = (uint64_t *)ts->storage.alloc(n * 8);
+        void *rates = ts->storage.alloc(n * dd.size);
+        launch_unpack_records(s, (const uint32_t *)recs.p, n, dd.size, keys, rates);
+        const uint32_t bits = table_index_bits(n);
+        uint32_t *idx = (uint32_t *)ts->storage.alloc(((1ull << bits) + 1) * 4);
+        launch_bucket_index(s, keys, n, nullptr, bits, idx);
+        t.keys = keys;
+        t.rates = (const char *)rates;
+        t.idx = idx;
+        t.n_ptr = nullptr;
+        t.n = n;
+        t.bits = bits;
+        HIPCHK(hipStreamSynchronize(s));  // the staging buffer is reused
+    }
+    finish_tables(*ts, s, tabs);
+    ts->bytes = ts->storage.used();
+    return ts;
+}
+
+std::unique_ptr<TableSet> table_from_result(const Result &r, const egtb_pattern *pat, int target, int64_t tile_sum,
+                                            double p4) {
+    DeviceCtx &C = device_ctx(r.dev);
+    configure_serving_kernels();
+    auto ts = std::make_unique<TableSet>();
+    ts->dev = r.dev;
+    ts->steps = r.steps;
+    ts->pattern = make_pattern(pat, target);
+    ts->nums_adjust = -tile_sum;
+    ts->rate_size = r.rate_size;
+    ts->is_float = r.rate_is_float;
+    ts->scale = r.rate_is_float ? 1.0 : (r.rate_size == 4 ? 4e9 : 1.6e18);
+    ts->p4 = p4;
+    ts->borrowed = &r;
+    ts->n.assign(r.steps, 0);
+    std::vector<TableDev<char>> tabs(r.steps);
+    for (int i = 0; i < r.steps; ++i) {
+        const Book &bk = r.books[i];
+        TableDev<char> &t = tabs[i];
+        std::memset(&t, 0, sizeof t);
+        t.bits = 1;
+        ts->n[i] = bk.n;
+        t.present = 1;
+        if (bk.n == 0) continue;
+        t.keys = bk.keys;
+        t.rates = (const char *)bk.rates;
+        if (bk.idx) {
+            t.idx = bk.idx;
+            t.bits = bk.bits;
+        } else {  // layer 0 has no index in a build; give it one
+            const uint32_t bits = table_index_bits(bk.n);
+            uint32_t *idx = (uint32_t *)ts->storage.alloc(((1ull << bits) + 1) * 4);
+            launch_bucket_index(C.stream, bk.keys, bk.n, nullptr, bits, idx);
+            t.idx = idx;
+            t.bits = bits;
+        }
+        t.n = bk.n;
+        ts->bytes += bk.n * (8 + r.rate_size);
+    }
+    finish_tables(*ts, C.stream, tabs);
+    ts->bytes += ts->storage.used();
+    return ts;
+}
+
+}  // namespace egtb
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+using namespace egtb;
+
+struct egtb_table : TableSet {};
+
+template <typename F>
+static int guarded_s(F &&f) {
+    try {
+        g_last_error.clear();
+        f();
+        return EGTB_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return EGTB_ERR_INTERNAL;
+    }
+}
+
+extern "C" {
+
+int egtb_table_open(const char *pathname, int steps, const egtb_pattern *pattern, int target, int64_t tile_sum,
+                    const char *success_rate_dtype, double spawn_rate4, int device, egtb_table **out) {
+    if (!out) return EGTB_ERR_ARG;
+    *out = nullptr;
+    return guarded_s([&] {
+        auto ts = table_from_files(pathname, steps, pattern, target, tile_sum, success_rate_dtype, spawn_rate4,
+                                   device);
+        *out = static_cast<egtb_table *>(ts.release());
+    });
+}
+
+int egtb_table_from_result(const egtb_result *result, const egtb_pattern *pattern, int target, int64_t tile_sum,
+                           double spawn_rate4, egtb_table **out) {
+    if (!out || !result) return EGTB_ERR_ARG;
+    *out = nullptr;
+    return guarded_s([&] {
+        auto ts = table_from_result(*reinterpret_cast<const Result *>(result), pattern, target, tile_sum,
+                                    spawn_rate4);
+        *out = static_cast<egtb_table *>(ts.release());
+    });
+}
+
+int egtb_table_info(const egtb_table *t, int *steps, uint64_t *records, uint64_t *bytes) {
+    if (!t) return EGTB_ERR_ARG;
+    if (steps) *steps = t->steps;
+    uint64_t r = 0;
+    for (uint64_t v : t->n) r += v;
+    if (records) *records = r;
+    if (bytes) *bytes = t->bytes;
+    return EGTB_OK;
+}
+
+int egtb_lookup_batch(egtb_table *t, const uint64_t *boards, size_t n, float *out4, int32_t *status) {
+    return guarded_s([&] {
+        if (!t) throw Error(EGTB_ERR_ARG, "table is NULL");
+        if (n == 0) return;
+        DeviceCtx &C = device_ctx(t->dev);
+        hipStream_t s = C.stream;
+        DBuf b, o, st;
+        b.alloc(n * 8, s);
+        o.alloc(n * 16, s);
+        st.alloc(n * 4, s);
+        HIPCHK(hipMemcpyAsync(b.p, boards, n * 8, hipMemcpyHostToDevice, s));
+        launch_lookup<float>(s, *t, (const uint64_t *)b.p, n, (float *)o.p, (int32_t *)st.p, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out4, o.p, n * 16, hipMemcpyDeviceToHost, s));
+        if (status) HIPCHK(hipMemcpyAsync(status, st.p, n * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int egtb_lookup_batch_f64(egtb_table *t, const uint64_t *boards, size_t n, double *out4, int32_t *status,
+                          int32_t *op_used) {
+    return guarded_s([&] {
+        if (!t) throw Error(EGTB_ERR_ARG, "table is NULL");
+        if (n == 0) return;
+        DeviceCtx &C = device_ctx(t->dev);
+        hipStream_t s = C.stream;
+        DBuf b, o, st, op;
+        b.alloc(n * 8, s);
+        o.alloc(n * 32, s);
+        st.alloc(n * 4, s);
+        op.alloc(n * 4, s);
+        HIPCHK(hipMemcpyAsync(b.p, boards, n * 8, hipMemcpyHostToDevice, s));
+        launch_lookup<double>(s, *t, (const uint64_t *)b.p, n, (double *)o.p, (int32_t *)st.p, (int32_t *)op.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out4, o.p, n * 32, hipMemcpyDeviceToHost, s));
+        if (status) HIPCHK(hipMemcpyAsync(status, st.p, n * 4, hipMemcpyDeviceToHost, s));
+        if (op_used) HIPCHK(hipMemcpyAsync(op_used, op.p, n * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+void egtb_table_free(egtb_table *t) {
+    if (!t) return;
+    (void)hipSetDevice(t->dev);
+    (void)hipDeviceSynchronize();
+    delete t;
+}
+
+}  // extern "C"

@@ -71,6 +71,18 @@ SIGNATURES = {
     "egtb_result_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "egtb_result_free": (None, [ctypes.c_void_p]),
+    "egtb_table_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(Pattern), ctypes.c_int,
+                                       ctypes.c_int64, ctypes.c_char_p, ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "egtb_table_from_result": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Pattern), ctypes.c_int,
+                                              ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
+    "egtb_table_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), u64p, u64p]),
+    "egtb_lookup_batch": (ctypes.c_int, [ctypes.c_void_p, u64p, ctypes.c_size_t, f32p,
+                                         ctypes.POINTER(ctypes.c_int32)]),
+    "egtb_lookup_batch_f64": (ctypes.c_int, [ctypes.c_void_p, u64p, ctypes.c_size_t,
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
+                                             ctypes.POINTER(ctypes.c_int32)]),
+    "egtb_table_free": (None, [ctypes.c_void_p]),
     "sort_uint64": (None, [u64p, ctypes.c_size_t, ctypes.c_bool]),
     "unique_sorted_u64_inplace": (ctypes.c_size_t, [u64p, ctypes.c_size_t]),
     "egtb_stats_enable": (None, [ctypes.c_int]),

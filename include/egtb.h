@@ -105,6 +105,37 @@ EGTB_EXPORT int egtb_result_book(const egtb_result *r, int layer, uint64_t *keys
 EGTB_EXPORT int egtb_result_timing(const egtb_result *r, double *fwd_ms, double *bwd_ms, double *total_ms);
 EGTB_EXPORT void egtb_result_free(egtb_result *r);
 
+/* ---- solved tables resident in HBM and batched lookups ------------------- */
+
+typedef struct egtb_table egtb_table;
+
+#define EGTB_LOOKUP_OK 0        /* answered from the table */
+#define EGTB_LOOKUP_FALLBACK 1  /* off-table: depth-2 expectimax fallback values */
+
+/* Load every <pathname><i>.book (i < steps) of a build into HBM.  A missing
+ * layer file reads as off-table, an empty one as all-zero rates, as in the
+ * reference's reader (ReaderRuntime.cpp:739-776).  tile_sum is the seed tile
+ * sum (minus the catalogue's nums_adjust).  Replaces the per-lookup on-disk
+ * binary search of find_success_entry_in_file (ReaderRuntime.cpp:279-316). */
+EGTB_EXPORT int egtb_table_open(const char *pathname, int steps, const egtb_pattern *pattern, int target,
+                                int64_t tile_sum, const char *success_rate_dtype, double spawn_rate4,
+                                int device, egtb_table **out);
+/* Serve the solved layers of an in-memory build (no copy; the result must
+ * outlive the table). */
+EGTB_EXPORT int egtb_table_from_result(const egtb_result *result, const egtb_pattern *pattern, int target,
+                                       int64_t tile_sum, double spawn_rate4, egtb_table **out);
+EGTB_EXPORT int egtb_table_info(const egtb_table *t, int *steps, uint64_t *records, uint64_t *bytes);
+/* For each board: four directional rates in (left, right, up, down) order,
+ * NaN for an invalid move, normalised to [0, 1] as the reference reader
+ * does (ReaderRuntime.cpp:271-277) -- i.e. ClassicBookReader::move_on_dic
+ * (ReaderRuntime.cpp:739-840) with a stateless symmetry order 0..7.
+ * status[i] is EGTB_LOOKUP_OK or EGTB_LOOKUP_FALLBACK.  Host buffers. */
+EGTB_EXPORT int egtb_lookup_batch(egtb_table *t, const uint64_t *boards, size_t n, float *out4, int32_t *status);
+/* Same in float64, also returning the symmetry op that answered (-1 none). */
+EGTB_EXPORT int egtb_lookup_batch_f64(egtb_table *t, const uint64_t *boards, size_t n, double *out4,
+                                      int32_t *status, int32_t *op_used);
+EGTB_EXPORT void egtb_table_free(egtb_table *t);
+
 /* ---- dedup primitives: drop-in for the reference's bookgen_native.so ----- */
 
 /* Ascending (or descending) sort of a host array on the GPU.  Same name and

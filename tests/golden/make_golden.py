@@ -35,7 +35,7 @@ patterns = importlib.import_module("2048endgametablebase_amd.patterns")
 # (pattern, log2 target); the full-symmetry case uses the free8 spec below.
 RUNS = [
     ("L3", 7), ("L3", 8), ("442", 7), ("442", 8), ("442", 9), ("t", 8), ("LL", 7),
-    ("4431", 7), ("444", 7), ("4432", 7), ("4441", 7), ("free8", 7),
+    ("4431", 7), ("LL", 6), ("444", 6), ("4432", 6), ("free8", 6), ("t", 6),
 ]
 
 # free8 (docs_and_configs/patterns_config.json): no masks, every nibble a

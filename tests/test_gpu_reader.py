@@ -1,0 +1,142 @@
+"""GPU reader / lookup parity: BookReader.move_on_dic over HBM tables
+against the CPU restatement of the reference reader (on-disk binary
+search), the fallback against its definition, and table sources agreeing."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from oracle import reader_oracle as ro
+
+pytestmark = pytest.mark.gpu
+
+PATTERN, TARGET = "442", 7
+
+
+def decode_values(b):
+    m = np.zeros((4, 4), dtype=np.int64)
+    for r in range(4):
+        for c in range(4):
+            e = (b >> (4 * ((3 - r) * 4 + (3 - c)))) & 0xF
+            m[r, c] = 0 if e == 0 else 1 << e
+    return m
+
+
+@pytest.fixture(scope="module")
+def built(pkg, tmp_path_factory):
+    cfg = pkg.config.SingletonConfig().config
+    old = cfg["success_rate_dtype"]
+    cfg["success_rate_dtype"] = "float32"
+    folder = tmp_path_factory.mktemp("tables")
+    prefix = str(folder / f"{PATTERN}_{2**TARGET}_")
+    try:
+        pkg.start_build(PATTERN, TARGET, prefix)
+    finally:
+        cfg["success_rate_dtype"] = old
+    cfg["filepath_map"][(f"{PATTERN}_{2**TARGET}", cfg["4_spawn_rate"])] = [(str(folder), "float32")]
+    yield str(folder), prefix
+    pkg.BookReader.clear()
+    cfg["filepath_map"].clear()
+
+
+def sample_boards(prefix, steps, n, seed):
+    """Positions to move from: stored states plus one spawned tile, some
+    rotated/mirrored (other symmetry ops), some with a pinned cell broken."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        i = int(rng.integers(1, steps - 2))
+        keys, _ = orc.read_book(prefix + f"{i}.book")
+        if keys.size == 0:
+            continue
+        b = int(keys[rng.integers(0, keys.size)])
+        empt = [k for k in range(16) if (b >> (4 * k)) & 0xF == 0]
+        if empt:
+            k = empt[rng.integers(0, len(empt))]
+            b |= (1 if rng.random() < 0.9 else 2) << (4 * k)
+        m = decode_values(b)
+        kind = rng.integers(0, 4)
+        if kind == 1:
+            m = np.rot90(m, int(rng.integers(1, 4)))
+        elif kind == 2:
+            m = np.fliplr(m)
+        elif kind == 3:
+            m = m.copy()
+            m[3, 3] = 2  # break a pinned 32768
+        out.append(np.ascontiguousarray(m))
+    return out
+
+
+def test_move_on_dic_matches_reference_reader(pkg, built):
+    folder, prefix = built
+    meta = pkg.patterns.pattern_catalog[PATTERN]
+    boards = sample_boards(prefix, pkg.patterns.build_params(PATTERN, TARGET)["steps"], 600, 1)
+    n_numeric = 0
+    for m in boards:
+        want = ro.move_on_dic(m, folder, f"{PATTERN}_{2**TARGET}", meta.nums_adjust, list(meta.pattern_masks),
+                              meta.symm_mode)
+        got = pkg.BookReader.move_on_dic(m, PATTERN, str(2**TARGET), f"{PATTERN}_{2**TARGET}")
+        assert got == want, (m.tolist(), got, want)
+        assert list(got[0].keys()) == list(want[0].keys())
+        n_numeric += want[1] != ""
+    assert n_numeric > 300
+
+
+def test_unknown_paths_and_negative_layers(pkg, built):
+    m = np.zeros((4, 4), dtype=np.int64)
+    assert pkg.BookReader.move_on_dic(m, PATTERN, "128", "442_128") == (
+        {"down": "", "right": "", "left": "", "up": ""}, "")
+    assert pkg.BookReader.move_on_dic(m, PATTERN, "128", "442_999") == ({"?": "?"}, "")
+
+
+def fallback_oracle(b, p4=0.1):
+    def h(x):
+        return sum(1 for k in range(16) if (x >> (4 * k)) & 0xF == 0) / 16.0
+
+    out = []
+    for d in range(4):
+        mv = orc.move(b, d)
+        if mv == b:
+            out.append(math.nan)
+            continue
+        empt = [k for k in range(16) if (mv >> (4 * k)) & 0xF == 0]
+        acc = 0.0
+        for k in empt:
+            for s, w in ((1, 1 - p4), (2, p4)):
+                sp = mv | (s << (4 * k))
+                best = 0.0
+                for d2 in range(4):
+                    m2 = orc.move(sp, d2)
+                    if m2 != sp:
+                        best = max(best, h(m2))
+                acc += w * best
+        out.append(acc / len(empt) if empt else h(mv))
+    return out
+
+
+def test_fallback_and_table_sources_agree(pkg, built):
+    folder, prefix = built
+    res = pkg.build_in_memory(PATTERN, TARGET)
+    t_mem = pkg.Table.from_result(res, PATTERN, TARGET)
+    t_file = pkg.Table.open(prefix, PATTERN, TARGET, "float32")
+    rng = np.random.default_rng(7)
+    boards = rng.integers(0, 2**63, size=4000, dtype=np.uint64)
+    boards &= np.uint64(0x7777777777777777)  # small tiles: mostly off-pattern
+    keys, _ = res.book(40)
+    boards[:2000] = keys[rng.integers(0, keys.size, 2000)] | np.uint64(1 << 60)
+    a, sa, oa = t_mem.lookup_f64(boards)
+    b, sb, ob = t_file.lookup_f64(boards)
+    assert np.array_equal(sa, sb) and np.array_equal(oa, ob)
+    assert np.array_equal(np.nan_to_num(a, nan=-1), np.nan_to_num(b, nan=-1))
+    f32, s32 = t_mem.lookup(boards)
+    assert np.array_equal(s32, sa)
+    assert np.array_equal(np.nan_to_num(f32, nan=-1), np.nan_to_num(a.astype(np.float32), nan=-1))
+    fb = np.nonzero(sa == 1)[0]
+    assert fb.size > 100
+    for i in fb[:200]:
+        want = fallback_oracle(int(boards[i]))
+        assert np.allclose(np.nan_to_num(a[i], nan=-1), np.nan_to_num(want, nan=-1), rtol=0, atol=1e-12)
+    t_mem.close()
+    t_file.close()
+    res.close()

@@ -1,0 +1,75 @@
+"""Host-side logic and the C ABI boundary (CPU only: no GPU compute)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+@pytest.mark.parametrize("pattern,target,steps,docheck", [
+    # SURVEY Appendix A, derived from BookBuilder.py:124-128
+    ("L3", 7, 112, 60), ("L3", 8, 176, 124), ("L3", 9, 304, 252), ("L3", 10, 560, 508),
+    ("442", 9, 304, 252), ("444", 8, 188, 127), ("444", 11, 1084, 1023),
+    ("4442", 11, 1072, 1020), ("LL", 9, 304, 249), ("4431", 9, 304, 249),
+])
+def test_steps_and_docheck(pkg, pattern, target, steps, docheck):
+    p = pkg.patterns.build_params(pattern, target)
+    assert (p["steps"], p["docheck_step"]) == (steps, docheck)
+
+
+def test_catalogue_entries(pkg):
+    cat = pkg.patterns.pattern_catalog
+    assert cat["L3"].success_shifts == tuple(range(0, 64, 4))
+    assert cat["442"].success_shifts == (24,)
+    assert cat["444"].success_shifts == (20, 24)
+    assert cat["444"].symm_mode == 3 and cat["LL"].symm_mode == 2 and cat["442"].symm_mode == 0
+    assert cat["L3"].tile_sum == 196616 and cat["444"].tile_sum == 131074
+    assert cat["442"].free_cells == 10 and cat["444"].free_cells == 12
+    assert pkg.patterns.build_params("442", 9)["is_free"] is False
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "egtb.h")).read()
+    return re.findall(r"EGTB_EXPORT\s+[\w\s\*]+?\b(\w+)\s*\(", text)
+
+
+def test_library_exports_every_header_symbol(pkg):
+    syms = _header_symbols()
+    assert "egtb_build" in syms and "sort_uint64" in syms and len(syms) >= 15
+    lib = ctypes.CDLL(pkg.native.LIB_PATH)
+    for name in syms:
+        assert hasattr(lib, name), name
+    assert set(syms) == set(pkg.native.SIGNATURES), "ctypes binding out of step with the header"
+
+
+def test_invalid_arguments_fail_before_device_use(pkg):
+    """Argument errors come back as codes + messages (no GPU touched)."""
+    with pytest.raises(pkg.native.EgtbError, match="symm_mode"):
+        pkg.run_pattern_build([1], [0xFFFF], [24], 5, 9, 304, 252, "", write_files=False)
+    with pytest.raises(pkg.native.EgtbError, match="target"):
+        pkg.run_pattern_build([1], [0xFFFF], [24], 0, 0, 304, 252, "", write_files=False)
+    with pytest.raises(pkg.native.EgtbError, match="steps"):
+        pkg.run_pattern_build([1], [0xFFFF], [24], 0, 9, 2, 0, "", write_files=False)
+    with pytest.raises(pkg.native.EgtbError, match="16 pattern masks"):
+        pkg.run_pattern_build([1], list(range(17)), [24], 0, 9, 304, 252, "", write_files=False)
+    with pytest.raises(pkg.native.EgtbError, match="success_rate_dtype"):
+        pkg.run_pattern_build([1], [0xFFFF], [24], 0, 9, 304, 252, "", write_files=False,
+                              success_rate_dtype="1-float32")
+    assert "success_rate_dtype" in pkg.native.last_error()
+
+
+def test_save_config_to_txt(pkg, tmp_path):
+    path = tmp_path / "x_config.txt"
+    pkg.BookBuilder.save_config_to_txt(str(path))
+    lines = path.read_text().splitlines()
+    assert lines[0] == "compress: False"
+    assert "4_spawn_rate: 0.1" in lines and "success_rate_dtype: uint32" in lines
+    assert len(lines) == 10
+
+
+def test_unknown_pattern_raises(pkg, tmp_path):
+    with pytest.raises(KeyError):
+        pkg.start_build("nope", 9, str(tmp_path / "x_"))

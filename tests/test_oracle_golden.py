@@ -1,0 +1,59 @@
+"""The oracle against the reference's own compiled builder: every solved
+layer's packed bytes must hash to the reference's .book file, and raw layer
+sizes must match its stats.txt.  CPU only (small golden cases)."""
+import pytest
+
+from conftest import book_sha, golden_params, golden_tags, load_golden
+
+# cases the oracle finishes in seconds on the CPU
+CPU_CASES = [t for t in ["442_128", "LL_128", "444_128", "4431_128", "4432_128", "4441_128", "free8_128"]
+             if t in golden_tags()]
+
+
+@pytest.mark.parametrize("tag", CPU_CASES)
+def test_oracle_matches_reference(orc, tag):
+    g = load_golden(tag)
+    gp = golden_params(g)
+    ob = orc.OracleBuild(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                         gp["docheck"], p4=g["spawn_rate4"], fma_mode=1)
+    try:
+        for layer in g["layers"]:
+            keys, rates = ob.book(layer["i"])
+            assert keys.size == layer["live"], f"live size of layer {layer['i']}"
+            assert book_sha(keys, rates) == layer["sha256"], f"bytes of layer {layer['i']}"
+            if layer["n"] is not None:
+                assert ob.raw(layer["i"]).size == layer["n"], f"raw size of layer {layer['i']}"
+    finally:
+        ob.close()
+
+
+def test_unfused_update_is_distinguishable(orc):
+    """Planted mistake: rounding the product before the add (no FMA) must
+    change some rate bits, so the digest check above does pin the FP order."""
+    g = load_golden("442_128")
+    gp = golden_params(g)
+    ob = orc.OracleBuild(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                         gp["docheck"], fma_mode=0)
+    try:
+        differing = sum(book_sha(*ob.book(l["i"])) != l["sha256"] for l in g["layers"])
+        assert differing > 0
+    finally:
+        ob.close()
+
+
+def test_golden_samples_consistent(orc):
+    """Sampled records inside each golden file agree with the oracle build."""
+    g = load_golden("442_128")
+    gp = golden_params(g)
+    ob = orc.OracleBuild(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                         gp["docheck"])
+    try:
+        import numpy as np
+
+        for layer in g["layers"]:
+            keys, rates = ob.book(layer["i"])
+            lookup = dict(zip(keys.tolist(), rates.view(np.uint32).tolist()))
+            for board_hex, bits in layer["sample"]:
+                assert lookup[int(board_hex, 16)] == bits
+    finally:
+        ob.close()
