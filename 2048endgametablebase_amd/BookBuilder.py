@@ -45,7 +45,7 @@ class _Args:
     """ctypes argument block kept alive for the duration of a native call."""
 
     def __init__(self, seeds, masks, shifts, symm, target, steps, docheck, pathname, is_free,
-                 spawn_rate4, dtype, write_files, keep_raw, device, radix_bits):
+                 spawn_rate4, dtype, write_files, keep_raw, device, radix_bits, sort_schedule=0):
         self.seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
         self.masks = np.ascontiguousarray(np.asarray(list(masks) or [0], dtype=np.uint64))
         self.shifts = np.ascontiguousarray(np.asarray(list(shifts) or [0], dtype=np.uint8))
@@ -67,6 +67,7 @@ class _Args:
         o.keep_raw = int(bool(keep_raw))
         o.device = int(device)
         o.radix_bits = int(radix_bits)
+        o.sort_schedule = int(sort_schedule)
 
 
 class BuildResult:
@@ -134,13 +135,13 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
                       steps: int, docheck_step: int, pathname: str = "", *, is_free: bool = False,
                       spawn_rate4: float = 0.1, success_rate_dtype: str = "float32",
                       write_files: bool = True, keep: bool = False, keep_raw: bool = False,
-                      device: int = 0, radix_bits: int = 0) -> Optional[BuildResult]:
+                      device: int = 0, radix_bits: int = 0, sort_schedule: int = 0) -> Optional[BuildResult]:
     """The native build call (reference formation_core.run_pattern_build,
     native_core/src/bindings_formation.cpp:220-229) with the PatternSpec and
     RunOptions fields spelled out.  ``keep=True`` returns the solved layers
     resident in HBM as a BuildResult."""
     a = _Args(arr_init, masks, shifts, symm_mode, target, steps, docheck_step, pathname, is_free,
-              spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits)
+              spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits, sort_schedule)
     L = native.lib()
     seeds_p = a.seeds.ctypes.data_as(native.u64p)
     if keep or keep_raw:
@@ -200,10 +201,11 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
 
 def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "float32",
                     spawn_rate4: float = 0.1, keep_raw: bool = False, device: int = 0,
-                    radix_bits: int = 0) -> BuildResult:
+                    radix_bits: int = 0, sort_schedule: int = 0) -> BuildResult:
     """Build a catalogue pattern and keep every solved layer in HBM."""
     p = build_params(pattern, target)
     return run_pattern_build(p["seeds"], p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"],
                              p["docheck_step"], "", is_free=p["is_free"], spawn_rate4=spawn_rate4,
                              success_rate_dtype=success_rate_dtype, write_files=False, keep=True,
-                             keep_raw=keep_raw, device=device, radix_bits=radix_bits)
+                             keep_raw=keep_raw, device=device, radix_bits=radix_bits,
+                             sort_schedule=sort_schedule)

@@ -61,13 +61,43 @@ void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n_bound, const 
                      uint64_t *partial, uint64_t *total_out);
 void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out, uint32_t *partial);
 
+// Device state of the single-pass (decoupled look-back) kernels of one
+// stream: status words tagged with a pass epoch (never cleared), cumulative
+// tile counters (never reset: each launch claims ids from `claimed` on),
+// double-buffered digit histograms, and a stall flag.
+constexpr int kHistWords = 16 * 2048;
+struct LookbackWork {
+    uint64_t *status_sort = nullptr;  // [tiles * 2048]
+    uint64_t *status_uniq = nullptr;  // [tiles]
+    unsigned long long *counters = nullptr;  // [0] sort tiles, [1] unique tiles
+    uint32_t *hist = nullptr;         // [2 * kHistWords]
+    uint32_t *err = nullptr;
+    unsigned long long claimed = 0, claimed_uniq = 0;
+    uint32_t epoch = 0;
+    int hist_parity = 0;
+    int dirty[2] = {0, 0};  // histogram words left non-zero in each buffer
+    uint32_t next_epoch() {
+        epoch = epoch + 1 >= (1u << 20) ? 1u : epoch + 1;
+        return epoch;
+    }
+};
+
 uint32_t radix_tile(int bits);
-void launch_radix_pass(hipStream_t s, int bits, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
-                       uint32_t *counts, uint32_t *offsets, uint32_t *partial);
+// Classic schedule: per digit a tile-histogram kernel, a scan of the
+// [digit][tile] counts and a scatter kernel.  counts/offsets hold
+// tiles << bits words, partial the scan partials.
+uint64_t *launch_classic_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
+                              uint32_t *counts, uint32_t *offsets, uint32_t *partial);
+// Stable LSD sort of keys[0..n) on bits [lo, hi); returns keys or tmp,
+// whichever holds the result.
+uint64_t *launch_onesweep_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
+                               LookbackWork &lw);
 
 uint64_t unique_tiles(uint64_t n);
-void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts, uint64_t *tile_offs,
-                   uint64_t *partial, uint64_t *out, uint64_t *count_out);
+void launch_unique_classic(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts,
+                           uint64_t *tile_offs, uint64_t *partial, uint64_t *out, uint64_t *count_out);
+void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *out, uint64_t *count_out,
+                   LookbackWork &lw);
 
 void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, const uint64_t *n_ptr,
                          uint32_t bits, uint32_t *idx);

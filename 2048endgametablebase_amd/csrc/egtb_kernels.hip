@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "egtb_device.h"
 #include "egtb_kernels.h"
 
@@ -284,11 +286,78 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const TI *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// LSD radix sort pass (stable), wave64 ballot multisplit
+// LSD radix sort (stable), onesweep schedule, wave64 ballot multisplit
+//
+//   k_os_hist     one read of the keys: digit histograms of every pass
+//   k_os_scan     exclusive scan of each pass's histogram (digit bases)
+//   k_os_scatter  one pass: tile ranks by wave multisplit, tile offsets by a
+//                 decoupled look-back over per-(tile, digit) status words,
+//                 digit runs written contiguously through LDS
+//
+// Status words: [epoch:20 | flag:2 | value:42]; flag 1 = tile aggregate,
+// 2 = inclusive prefix.  The epoch changes every pass, so the array is
+// never cleared; a word from another pass reads as "not ready".  Spins are
+// bounded: past the limit the kernel raises *err and carries on.
 // ---------------------------------------------------------------------------
 
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kMaxPasses = 16;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint32_t flag, uint64_t v) {
+    return ((uint64_t)epoch << 44) | ((uint64_t)flag << 42) | (v & ((1ull << 42) - 1));
+}
+
+__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of the values of tiles [0, tile) from their status words, reading up
+// to 16 predecessors per round so their loads are in flight together; waits
+// for aggregates that are not yet published, stops at the first inclusive
+// prefix (tile 0 always publishes one).
+__device__ __forceinline__ uint64_t lb_lookback(const uint64_t *status, uint64_t stride, uint32_t tile,
+                                                uint32_t epoch, uint32_t *err) {
+    constexpr int B = 16;
+    uint64_t prefix = 0;
+    int64_t j = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (j >= 0) {
+        uint64_t w[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) w[q] = (j - q >= 0) ? lb_load(status + (uint64_t)(j - q) * stride) : 0;
+        int q = 0;
+        bool stall = false;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            if (stall || j - r < 0) continue;
+            const uint32_t f = (uint32_t)(w[r] >> 42) & 3u;
+            if ((uint32_t)(w[r] >> 44) != epoch || f == 0) {
+                stall = true;
+                continue;
+            }
+            prefix += w[r] & ((1ull << 42) - 1);
+            if (f == 2) return prefix;
+            ++q;
+        }
+        j -= q;
+        if (stall) {
+            if (++spins > kSpinLimit) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return prefix;
+}
 
 // Lanes of this wave whose digit equals mine, among the lanes in `valid`.
 template <int BITS>
@@ -303,6 +372,158 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, uint64_t valid) 
     return m;
 }
 
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_os_hist(const uint64_t *__restrict__ keys, uint64_t n, int lo,
+                                                          int passes, uint32_t *__restrict__ hist) {
+    constexpr int BINS = 1 << BITS;
+    __shared__ uint32_t h[kMaxPasses * BINS > 12288 ? 12288 : kMaxPasses * BINS];
+    for (int i = threadIdx.x; i < passes * BINS; i += kSortThreads) h[i] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * kSortThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kSortThreads + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i] >> lo;
+        for (int q = 0; q < passes; ++q) atomicAdd(&h[q * BINS + ((k >> (q * BITS)) & (BINS - 1))], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * BINS; i += kSortThreads)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// One block: hist[q][*] -> exclusive digit bases, in place; also clears
+// the other histogram buffer for the next sort (its last reader finished
+// before this sort's histogram kernel started).
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_os_scan(uint32_t *__restrict__ hist, int passes,
+                                                          uint32_t *__restrict__ next_hist, int next_words) {
+    constexpr int BINS = 1 << BITS;
+    constexpr int PER = BINS >= kSortThreads ? BINS / kSortThreads : 1;
+    __shared__ uint32_t scratch[kSortWaves];
+    for (int i = threadIdx.x; i < next_words; i += kSortThreads) next_hist[i] = 0;
+    for (int q = 0; q < passes; ++q) {
+        uint32_t *hq = hist + q * BINS;
+        const int first = threadIdx.x * PER;
+        uint32_t local[PER];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            local[j] = sum;
+            sum += first + j < BINS ? hq[first + j] : 0u;
+        }
+        uint32_t total;
+        const uint32_t ex = block_exclusive_sum(sum, scratch, total);
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if (first + j < BINS) hq[first + j] = ex + local[j];
+        __syncthreads();
+    }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_os_scatter(const uint64_t *__restrict__ src,
+                                                             uint64_t *__restrict__ dst, uint64_t n, int shift,
+                                                             const uint32_t *__restrict__ digit_base,
+                                                             uint64_t *__restrict__ status,
+                                                             unsigned long long *__restrict__ tile_counter,
+                                                             unsigned long long tile_base, uint32_t epoch,
+                                                             uint32_t *__restrict__ err) {
+    constexpr int BINS = 1 << BITS;
+    constexpr int ITEMS = kSortItems;
+    constexpr int TILE = kSortTile;
+    __shared__ uint64_t stage[TILE];
+    __shared__ uint32_t whist[kSortWaves][BINS];
+    __shared__ uint32_t binbase[BINS];
+    __shared__ uint32_t gexcl[BINS];
+    __shared__ uint32_t scratch[kSortWaves];
+    __shared__ uint32_t s_tile;
+
+    if (threadIdx.x == 0) s_tile = (uint32_t)(atomicAdd(tile_counter, 1ull) - tile_base);
+    for (int i = threadIdx.x; i < kSortWaves * BINS; i += kSortThreads) (&whist[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t tile0 = (uint64_t)tile * TILE;
+    const uint64_t base = tile0 + (uint64_t)wave * ITEMS * 64;
+    uint64_t key[ITEMS];
+    uint32_t rank[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)j * 64 + lane;
+        key[j] = idx < n ? src[idx] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)j * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (uint32_t)((key[j] >> shift) & (BINS - 1));
+        const uint64_t m = match_digit<BITS>(d, __ballot(valid));
+        const uint32_t below = (uint32_t)__popcll(lanes_below(m));
+        uint32_t prev = 0;
+        if (valid) prev = whist[wave][d];
+        rank[j] = prev + below;
+        if (valid && below == 0) whist[wave][d] = prev + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    // per digit: wave-exclusive prefix, tile count; publish, then look back
+    for (int bin = threadIdx.x; bin < BINS; bin += kSortThreads) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            const uint32_t t = whist[w][bin];
+            whist[w][bin] = c;
+            c += t;
+        }
+        binbase[bin] = c;
+        uint64_t *st = status + (uint64_t)tile * BINS + bin;
+        if (tile == 0) {
+            lb_store(st, lb_pack(epoch, 2, c));
+            gexcl[bin] = 0;
+        } else {
+            lb_store(st, lb_pack(epoch, 1, c));
+        }
+    }
+    if (tile > 0) {
+        for (int bin = threadIdx.x; bin < BINS; bin += kSortThreads) {
+            const uint64_t pre = lb_lookback(status + bin, BINS, tile, epoch, err);
+            lb_store(status + (uint64_t)tile * BINS + bin, lb_pack(epoch, 2, pre + binbase[bin]));
+            gexcl[bin] = (uint32_t)pre;
+        }
+    }
+    __syncthreads();
+    {
+        constexpr int PER = BINS >= kSortThreads ? BINS / kSortThreads : 1;
+        const int first = threadIdx.x * PER;
+        uint32_t local[PER];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            local[q] = sum;
+            sum += (first + q < BINS) ? binbase[first + q] : 0u;
+        }
+        uint32_t total;
+        const uint32_t ex = block_exclusive_sum(sum, scratch, total);
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (first + q < BINS) binbase[first + q] = ex + local[q];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)j * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)((key[j] >> shift) & (BINS - 1));
+            stage[binbase[d] + whist[wave][d] + rank[j]] = key[j];
+        }
+    }
+    __syncthreads();
+    const uint64_t m = (n - tile0) < (uint64_t)TILE ? (n - tile0) : (uint64_t)TILE;
+    for (uint32_t i = threadIdx.x; i < m; i += kSortThreads) {
+        const uint64_t k = stage[i];
+        const uint32_t d = (uint32_t)((k >> shift) & (BINS - 1));
+        dst[(uint64_t)digit_base[d] + gexcl[d] + (i - binbase[d])] = k;
+    }
+}
+
+// Classic three-kernel pass (tile histogram, scan, scatter): no look-back.
 // Per-tile digit histogram -> counts[bin * ntiles + tile].
 template <int BITS, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ src, uint64_t n,
@@ -425,6 +646,55 @@ constexpr int kUniqThreads = 256;
 constexpr int kUniqItems = 8;
 constexpr int kUniqTile = kUniqThreads * kUniqItems;
 
+// One pass: each tile counts its kept elements, publishes the count, looks
+// back for its output offset, writes; the last tile stores the total.
+__global__ __launch_bounds__(kUniqThreads) void k_unique_lb(const uint64_t *__restrict__ a, uint64_t n,
+                                                            uint64_t *__restrict__ out,
+                                                            uint64_t *__restrict__ count_out,
+                                                            uint64_t *__restrict__ status,
+                                                            unsigned long long *__restrict__ tile_counter,
+                                                            unsigned long long tile_base, uint32_t epoch,
+                                                            uint32_t *__restrict__ err) {
+    __shared__ uint64_t scratch[kUniqThreads / 64];
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_off;
+    if (threadIdx.x == 0) s_tile = (uint32_t)(atomicAdd(tile_counter, 1ull) - tile_base);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * kUniqTile + (uint64_t)threadIdx.x * kUniqItems;
+    uint64_t v[kUniqItems];
+    uint32_t f = 0;
+    uint64_t cnt = 0;
+#pragma unroll
+    for (int j = 0; j < kUniqItems; ++j) {
+        const uint64_t idx = base + j;
+        v[j] = idx < n ? a[idx] : 0;
+        const bool keep = idx < n && (idx == 0 || v[j] != (j ? v[j - 1] : a[idx - 1]));
+        f |= (keep ? 1u : 0u) << j;
+        cnt += keep;
+    }
+    uint64_t total;
+    const uint64_t ex = block_exclusive_sum(cnt, scratch, total);
+    if (threadIdx.x == 0) {
+        if (tile == 0) {
+            lb_store(status, lb_pack(epoch, 2, total));
+            s_off = 0;
+        } else {
+            lb_store(status + tile, lb_pack(epoch, 1, total));
+            const uint64_t pre = lb_lookback(status, 1, tile, epoch, err);
+            lb_store(status + tile, lb_pack(epoch, 2, pre + total));
+            s_off = pre;
+        }
+    }
+    __syncthreads();
+    uint64_t pos = s_off + ex;
+    if (count_out && (uint64_t)(tile + 1) * kUniqTile >= n && threadIdx.x == 0) *count_out = s_off + total;
+#pragma unroll
+    for (int j = 0; j < kUniqItems; ++j)
+        if ((f >> j) & 1u) out[pos++] = v[j];
+}
+
+// Classic unique: tile counts, scan, write.
 __global__ __launch_bounds__(kUniqThreads) void k_unique_count(const uint64_t *__restrict__ a, uint64_t n,
                                                                uint64_t *__restrict__ counts) {
     __shared__ uint64_t scratch[kUniqThreads / 64];
@@ -531,7 +801,12 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
         while (em) {
             const int k = __builtin_ctz(em);
             em &= em - 1;
-            T best[2] = {(T)0, (T)0};
+            // the 8 successors of this cell: spawn 2 (j < 4) and spawn 4
+            // (j >= 4) times the four moves; their searches advance in
+            // lockstep so the loads of one step are all in flight together
+            uint64_t key[8];
+            uint32_t lo[8], hi[8];
+            uint32_t live = 0;
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const uint64_t sp_board = b | ((uint64_t)(s + 1) << (4 * k));
@@ -539,11 +814,42 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                 move4(sp_board, lut, mv);
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
-                    if (mv[d] != sp_board && is_pattern(mv[d], p)) {
-                        const uint64_t c = canonical(mv[d], p.symm);
-                        const T r = s == 0 ? table_lookup_n(t1, n1, c) : table_lookup_n(t2, n2, c);
-                        best[s] = best[s] < r ? r : best[s];
+                    const int j = s * 4 + d;
+                    key[j] = canonical(mv[d], p.symm);
+                    lo[j] = hi[j] = 0;
+                    const uint64_t nn = s == 0 ? n1 : n2;
+                    if (mv[d] != sp_board && is_pattern(mv[d], p) && nn) {
+                        const TableDev<T> &t = s == 0 ? t1 : t2;
+                        const uint64_t bk = key[j] >> (64 - t.bits);
+                        lo[j] = t.idx[bk];
+                        hi[j] = t.idx[bk + 1];
+                        live |= 1u << j;
                     }
+                }
+            }
+            uint32_t active = live;
+            while (active) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (lo[j] < hi[j]) {
+                        const uint32_t mid = (lo[j] + hi[j]) >> 1;
+                        const uint64_t km = (j < 4 ? t1.keys : t2.keys)[mid];
+                        if (km < key[j]) lo[j] = mid + 1;
+                        else hi[j] = mid;
+                    }
+                }
+                active = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) active |= (lo[j] < hi[j] ? 1u : 0u) << j;
+            }
+            T best[2] = {(T)0, (T)0};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if ((live >> j) & 1u) {
+                    const TableDev<T> &t = j < 4 ? t1 : t2;
+                    const uint64_t nn = j < 4 ? n1 : n2;
+                    const T r = (lo[j] < nn && t.keys[lo[j]] == key[j]) ? t.rates[lo[j]] : (T)0;
+                    best[j >> 2] = best[j >> 2] < r ? r : best[j >> 2];
                 }
             }
             sp = __fma_rn((double)best[0], w2, sp);
@@ -720,37 +1026,77 @@ void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *ou
 
 uint32_t radix_tile(int bits) {
     (void)bits;
-    return kSortThreads * 16;
-}
-
-template <int BITS>
-static void radix_pass_impl(hipStream_t s, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
-                            uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
-    constexpr int ITEMS = 16;
-    constexpr int TILE = kSortThreads * ITEMS;
-    const uint32_t ntiles = blocks_for(n, TILE);
-    hipLaunchKernelGGL((k_radix_hist<BITS, ITEMS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, n, shift,
-                       counts, ntiles);
-    launch_scan_u32(s, counts, (uint64_t)ntiles << BITS, offsets, partial);
-    hipLaunchKernelGGL((k_radix_scatter<BITS, ITEMS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst, n,
-                       shift, offsets, ntiles);
-}
-
-void launch_radix_pass(hipStream_t s, int bits, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
-                       uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
-    if (n == 0) return;
-    switch (bits) {
-        case 4: radix_pass_impl<4>(s, src, dst, n, shift, counts, offsets, partial); break;
-        case 6: radix_pass_impl<6>(s, src, dst, n, shift, counts, offsets, partial); break;
-        case 11: radix_pass_impl<11>(s, src, dst, n, shift, counts, offsets, partial); break;
-        default: radix_pass_impl<8>(s, src, dst, n, shift, counts, offsets, partial); break;
-    }
+    return kSortTile;
 }
 
 uint64_t unique_tiles(uint64_t n) { return blocks_for(n ? n : 1, kUniqTile); }
 
-void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts, uint64_t *tile_offs,
-                   uint64_t *partial, uint64_t *out, uint64_t *count_out) {
+template <int BITS>
+static uint64_t *onesweep_impl(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int passes,
+                               LookbackWork &lw) {
+    constexpr int BINS = 1 << BITS;
+    const uint32_t ntiles = blocks_for(n, kSortTile);
+    const int cur = lw.hist_parity, other = cur ^ 1;
+    uint32_t *hist = lw.hist + (cur ? kHistWords : 0);
+    uint32_t *next = lw.hist + (other ? kHistWords : 0);
+    lw.hist_parity = other;
+    const uint32_t hgrid = blocks_for(n, 8192) < 256 ? blocks_for(n, 8192) : 256;
+    hipLaunchKernelGGL((k_os_hist<BITS>), dim3(hgrid), dim3(kSortThreads), 0, s, keys, n, lo, passes, hist);
+    hipLaunchKernelGGL((k_os_scan<BITS>), dim3(1), dim3(kSortThreads), 0, s, hist, passes, next, lw.dirty[other]);
+    lw.dirty[other] = 0;
+    lw.dirty[cur] = passes * BINS;
+    uint64_t *src = keys, *dst = tmp;
+    for (int q = 0; q < passes; ++q) {
+        const uint32_t epoch = lw.next_epoch();
+        hipLaunchKernelGGL((k_os_scatter<BITS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst, n,
+                           lo + q * BITS, hist + q * BINS, lw.status_sort, lw.counters, lw.claimed, epoch, lw.err);
+        lw.claimed += ntiles;
+        std::swap(src, dst);
+    }
+    return src;
+}
+
+template <int BITS>
+static void classic_pass_impl(hipStream_t s, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
+                              uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
+    const uint32_t ntiles = blocks_for(n, kSortTile);
+    hipLaunchKernelGGL((k_radix_hist<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, n, shift,
+                       counts, ntiles);
+    launch_scan_u32(s, counts, (uint64_t)ntiles << BITS, offsets, partial);
+    hipLaunchKernelGGL((k_radix_scatter<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst, n,
+                       shift, offsets, ntiles);
+}
+
+uint64_t *launch_classic_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
+                              uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
+    if (n < 2 || hi <= lo) return keys;
+    uint64_t *src = keys, *dst = tmp;
+    for (int shift = lo; shift < hi; shift += bits) {
+        switch (bits) {
+            case 4: classic_pass_impl<4>(s, src, dst, n, shift, counts, offsets, partial); break;
+            case 6: classic_pass_impl<6>(s, src, dst, n, shift, counts, offsets, partial); break;
+            case 11: classic_pass_impl<11>(s, src, dst, n, shift, counts, offsets, partial); break;
+            default: classic_pass_impl<8>(s, src, dst, n, shift, counts, offsets, partial); break;
+        }
+        std::swap(src, dst);
+    }
+    return src;
+}
+
+uint64_t *launch_onesweep_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
+                               LookbackWork &lw) {
+    if (n < 2 || hi <= lo) return keys;
+    const int passes = (hi - lo + bits - 1) / bits;
+    switch (bits) {
+        case 4: return onesweep_impl<4>(s, keys, tmp, n, lo, passes, lw);
+        case 6: return onesweep_impl<6>(s, keys, tmp, n, lo, passes, lw);
+        case 11: return onesweep_impl<11>(s, keys, tmp, n, lo, passes, lw);
+        default: return onesweep_impl<8>(s, keys, tmp, n, lo, passes, lw);
+    }
+}
+
+void launch_unique_classic(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts,
+                           uint64_t *tile_offs, uint64_t *partial, uint64_t *out, uint64_t *count_out) {
     if (n == 0) {
         if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
         return;
@@ -759,6 +1105,19 @@ void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_
     hipLaunchKernelGGL(k_unique_count, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_counts);
     launch_scan_u64(s, tile_counts, tiles, nullptr, tile_offs, partial, tile_offs + tiles);
     hipLaunchKernelGGL(k_unique_write, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_offs, out, count_out);
+}
+
+void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *out, uint64_t *count_out,
+                   LookbackWork &lw) {
+    if (n == 0) {
+        if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
+        return;
+    }
+    const uint32_t tiles = (uint32_t)unique_tiles(n);
+    const uint32_t epoch = lw.next_epoch();
+    hipLaunchKernelGGL(k_unique_lb, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, out, count_out, lw.status_uniq,
+                       lw.counters + 1, lw.claimed_uniq, epoch, lw.err);
+    lw.claimed_uniq += tiles;
 }
 
 void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, const uint64_t *n_ptr,

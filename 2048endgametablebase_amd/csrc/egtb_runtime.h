@@ -72,9 +72,16 @@ struct HostStage {
     void ensure(size_t n);
 };
 
-// Scratch of the dedup / compaction pipelines.
+// Scratch of the dedup / compaction pipelines of one stream.
 struct SortWork {
     DBuf counts, offsets, partial, tile_counts, tile_offs, maxbits;
+    DBuf lb_status_sort, lb_status_uniq, lb_misc;
+    LookbackWork lw;
+    // grow the look-back state for a sort of `sort_tiles` tiles of `bins`
+    // digits and a unique of `uniq_tiles` tiles (0 = no change)
+    void ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles);
+    // stall flag of the look-back kernels (synchronises)
+    uint32_t read_error(hipStream_t s);
 };
 
 // HIP-event timing of one launch group, accumulated per kernel family when
@@ -159,8 +166,11 @@ struct Result {
 
 PatternDev make_pattern(const egtb_pattern *pat, int target);
 int choose_radix_bits(int span, int requested);
+int choose_schedule(uint64_t n, int requested);
 uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
-                     int bits_req, SortWork &w);
+                     int bits_req, SortWork &w, int schedule);
+void unique_into(hipStream_t s, const uint64_t *sorted, uint64_t n, uint64_t *out, uint64_t *count_out,
+                 SortWork &w, int schedule);
 void write_file_atomic(const std::string &path, const void *data, size_t bytes);
 void progress_set(uint32_t cur, uint32_t total);
 void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *o,

@@ -186,6 +186,21 @@ Timer::~Timer() {
 // Dedup pipeline: LSD radix sort over the varying-bit span + unique
 // ---------------------------------------------------------------------------
 
+// Sort tuning table: for keys counts up to max_n[i] (ascending) use
+// bits[i] / schedule[i]; installed by the autotuner (egtb_set_sort_tuning).
+static std::mutex g_tune_mu;
+struct TuneRow {
+    uint64_t max_n;
+    int bits, schedule;
+};
+static std::vector<TuneRow> g_tune;
+
+static const TuneRow *tune_row(uint64_t n) {
+    for (const TuneRow &r : g_tune)
+        if (n <= r.max_n) return &r;
+    return g_tune.empty() ? nullptr : &g_tune.back();
+}
+
 int choose_radix_bits(int span, int requested) {
     if (requested == 4 || requested == 6 || requested == 8 || requested == 11) return requested;
     if (span <= 0) return 8;
@@ -194,26 +209,91 @@ int choose_radix_bits(int span, int requested) {
     return p11 < p8 ? 11 : 8;
 }
 
+void SortWork::ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles) {
+    if (!lb_misc.p) {
+        // counters (2 x u64), error flag, double histogram
+        const size_t bytes = 64 + (size_t)2 * kHistWords * 4;
+        lb_misc.alloc(bytes, s);
+        HIPCHK(hipMemsetAsync(lb_misc.p, 0, bytes, s));
+        lw.counters = (unsigned long long *)lb_misc.p;
+        lw.err = (uint32_t *)((char *)lb_misc.p + 16);
+        lw.hist = (uint32_t *)((char *)lb_misc.p + 64);
+        lw.claimed = lw.claimed_uniq = 0;
+    }
+    const size_t want_sort = (size_t)sort_tiles * (size_t)bins * 8;
+    if (want_sort > lb_status_sort.bytes) {
+        lb_status_sort.alloc(want_sort + want_sort / 2, s);
+        HIPCHK(hipMemsetAsync(lb_status_sort.p, 0, lb_status_sort.bytes, s));
+        lw.status_sort = (uint64_t *)lb_status_sort.p;
+    }
+    const size_t want_uniq = (size_t)uniq_tiles * 8;
+    if (want_uniq > lb_status_uniq.bytes) {
+        lb_status_uniq.alloc(want_uniq + want_uniq / 2 + 64, s);
+        HIPCHK(hipMemsetAsync(lb_status_uniq.p, 0, lb_status_uniq.bytes, s));
+        lw.status_uniq = (uint64_t *)lb_status_uniq.p;
+    }
+}
+
+uint32_t SortWork::read_error(hipStream_t s) {
+    if (!lw.err) return 0;
+    uint32_t e = 0;
+    HIPCHK(hipMemcpyAsync(&e, lw.err, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return e;
+}
+
+int choose_schedule(uint64_t n, int requested) {
+    if (requested == 1 || requested == 2) return requested;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const TuneRow *r = tune_row(n);
+    return r && (r->schedule == 1 || r->schedule == 2) ? r->schedule : 1;
+}
+
+static int tuned_bits(uint64_t n, int span, int requested) {
+    if (requested) return choose_radix_bits(span, requested);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const TuneRow *r = tune_row(n);
+    return choose_radix_bits(span, r ? r->bits : 0);
+}
+
 uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all,
-                     uint64_t and_all, int bits_req, SortWork &w) {
+                     uint64_t and_all, int bits_req, SortWork &w, int schedule) {
     const uint64_t diff = or_all ^ and_all;
     if (n < 2 || diff == 0) return keys;
+    if (n >= (1ull << 32)) throw Error(EGTB_ERR_ARG, "a single sort is limited to 2^32 keys");
     const int lo = __builtin_ctzll(diff);
     const int hi = 64 - __builtin_clzll(diff);
-    const int bits = choose_radix_bits(hi - lo, bits_req);
+    const int bits = tuned_bits(n, hi - lo, bits_req);
     const uint64_t ntiles = (n + radix_tile(bits) - 1) / radix_tile(bits);
-    const uint64_t nc = ntiles << bits;
-    w.counts.ensure(nc * 4, s);
-    w.offsets.ensure(nc * 4, s);
-    w.partial.ensure(scan_partials_needed(nc) * 8 + 64, s);
-    uint64_t *src = keys, *dst = tmp;
-    for (int shift = lo; shift < hi; shift += bits) {
-        launch_radix_pass(s, bits, src, dst, n, shift, (uint32_t *)w.counts.p, (uint32_t *)w.offsets.p,
-                          (uint32_t *)w.partial.p);
-        std::swap(src, dst);
+    uint64_t *out;
+    if (choose_schedule(n, schedule) == 2) {
+        w.ensure_lookback(s, ntiles, 1 << bits, 0);
+        out = launch_onesweep_sort(s, bits, keys, tmp, n, lo, hi, w.lw);
+    } else {
+        const uint64_t nc = ntiles << bits;
+        w.counts.ensure(nc * 4, s);
+        w.offsets.ensure(nc * 4, s);
+        w.partial.ensure(scan_partials_needed(nc) * 8 + 64, s);
+        out = launch_classic_sort(s, bits, keys, tmp, n, lo, hi, (uint32_t *)w.counts.p, (uint32_t *)w.offsets.p,
+                                  (uint32_t *)w.partial.p);
     }
     HIPCHK(hipGetLastError());
-    return src;
+    return out;
+}
+
+void unique_into(hipStream_t s, const uint64_t *sorted, uint64_t n, uint64_t *out, uint64_t *count_out,
+                 SortWork &w, int schedule) {
+    if (choose_schedule(n, schedule) == 2) {
+        w.ensure_lookback(s, 0, 0, unique_tiles(n));
+        launch_unique(s, sorted, n, out, count_out, w.lw);
+    } else {
+        const uint64_t tiles = unique_tiles(n);
+        w.tile_counts.ensure(tiles * 8, s);
+        w.tile_offs.ensure((tiles + 1) * 8, s);
+        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+        launch_unique_classic(s, sorted, n, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
+                              (uint64_t *)w.partial.p, out, count_out);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -555,17 +635,12 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         {
             Timer t(s, "sort", 16.0 * (double)n_c);
             sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all,
-                                o.radix_bits, w);
+                                o.radix_bits, w, o.sort_schedule);
         }
         res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
         {
-            const uint64_t tiles = unique_tiles(n_c);
-            w.tile_counts.ensure(tiles * 8, s);
-            w.tile_offs.ensure((tiles + 1) * 8, s);
-            w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
             Timer t(s, "unique", 16.0 * (double)n_c);
-            launch_unique(s, sorted, n_c, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                          (uint64_t *)w.partial.p, res.raw[i], &M[i].n_raw);
+            unique_into(s, sorted, n_c, res.raw[i], &M[i].n_raw, w, o.sort_schedule);
         }
         if (i + 1 <= steps - 2) launch_count(i + 1, n_c);
         HIPCHK(hipGetLastError());
@@ -584,15 +659,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         {
             Timer t(s, "sort", 16.0 * (double)n_c);
             sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all,
-                                o.radix_bits, w);
+                                o.radix_bits, w, o.sort_schedule);
         }
         res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-        const uint64_t tiles = unique_tiles(n_c);
-        w.tile_counts.ensure(tiles * 8, s);
-        w.tile_offs.ensure((tiles + 1) * 8, s);
-        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
-        launch_unique(s, sorted, n_c, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                      (uint64_t *)w.partial.p, res.raw[L], &M[L].n_raw);
+        unique_into(s, sorted, n_c, res.raw[L], &M[L].n_raw, w, o.sort_schedule);
         read_meta(s, res, H, L, L);
         res.raw_arena.trim_last(res.raw[L], H[L].n_raw * 8);
         res.info[L].n_raw = H[L].n_raw;
@@ -656,6 +726,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], H[i].n_live, stage, packed);
         }
     }
+    if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     read_meta(s, res, H, 0, steps - 1);
     for (int i = 0; i < steps; ++i) {
         res.info[i].n_live = H[i].n_live;
@@ -827,9 +898,10 @@ void sort_uint64(uint64_t *arr, size_t arrsize, bool descending) {
         HIPCHK(hipMemcpyAsync(a.p, arr, arrsize * 8, hipMemcpyHostToDevice, s));
         SortWork w;
         // full 64-bit span: no key-range assumption for arbitrary input
-        uint64_t *out = radix_sort(s, (uint64_t *)a.p, (uint64_t *)t.p, arrsize, ~0ull, 0ull, 0, w);
+        uint64_t *out = radix_sort(s, (uint64_t *)a.p, (uint64_t *)t.p, arrsize, ~0ull, 0ull, 0, w, 0);
         HIPCHK(hipMemcpyAsync(arr, out, arrsize * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     });
     if (rc != EGTB_OK) {
         std::fprintf(stderr, "sort_uint64: %s\n", g_last_error.c_str());
@@ -849,15 +921,12 @@ size_t unique_sorted_u64_inplace(uint64_t *data, size_t size) {
         a.alloc(size * 8, s);
         HIPCHK(hipMemcpyAsync(a.p, data, size * 8, hipMemcpyHostToDevice, s));
         SortWork w;
-        const uint64_t tiles = unique_tiles(size);
-        w.tile_counts.ensure(tiles * 8, s);
-        w.tile_offs.ensure((tiles + 1) * 8, s);
-        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
         out.alloc(size * 8, s);
-        launch_unique(s, (const uint64_t *)a.p, size, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                      (uint64_t *)w.partial.p, (uint64_t *)out.p, nullptr);
+        DBuf cnt;
+        cnt.alloc(8, s);
+        unique_into(s, (const uint64_t *)a.p, size, (uint64_t *)out.p, (uint64_t *)cnt.p, w, 0);
         uint64_t mm = 0;
-        HIPCHK(hipMemcpyAsync(&mm, (uint64_t *)w.tile_offs.p + tiles, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&mm, cnt.p, 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         if (mm) HIPCHK(hipMemcpyAsync(data, out.p, mm * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -868,6 +937,64 @@ size_t unique_sorted_u64_inplace(uint64_t *data, size_t size) {
         std::abort();
     }
     return m;
+}
+
+int egtb_bench_sort(const uint64_t *keys, size_t n, int bits, int schedule, int reps, double *ms_per_sort) {
+    return guarded([&] {
+        if (!keys || n == 0 || reps <= 0 || !ms_per_sort) throw Error(EGTB_ERR_ARG, "invalid sort benchmark");
+        DeviceCtx &C = device_ctx(0);
+        hipStream_t s = C.stream;
+        DBuf src, a, t, cnt;
+        src.alloc(n * 8, s);
+        a.alloc(n * 8, s);
+        t.alloc(n * 8, s);
+        cnt.alloc(8, s);
+        HIPCHK(hipMemcpyAsync(src.p, keys, n * 8, hipMemcpyHostToDevice, s));
+        uint64_t o = 0, an = ~0ull;
+        for (size_t i = 0; i < n; ++i) {
+            o |= keys[i];
+            an &= keys[i];
+        }
+        SortWork w;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        double total = 0;
+        for (int r = 0; r <= reps; ++r) {  // first round warms up
+            HIPCHK(hipMemcpyAsync(a.p, src.p, n * 8, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipEventRecord(e0, s));
+            uint64_t *out = radix_sort(s, (uint64_t *)a.p, (uint64_t *)t.p, n, o, an, bits, w, schedule);
+            unique_into(s, out, n, (uint64_t *)t.p == out ? (uint64_t *)a.p : (uint64_t *)t.p, (uint64_t *)cnt.p, w,
+                        schedule);
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (r) total += ms;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
+        *ms_per_sort = total / reps;
+    });
+}
+
+int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32_t *schedule, int rows) {
+    if (rows < 0 || (rows > 0 && (!max_n || !bits || !schedule))) {
+        g_last_error = "invalid tuning table";
+        return EGTB_ERR_ARG;
+    }
+    std::vector<TuneRow> t;
+    for (int i = 0; i < rows; ++i) {
+        if (i && max_n[i] <= max_n[i - 1]) {
+            g_last_error = "tuning rows must have ascending max_n";
+            return EGTB_ERR_ARG;
+        }
+        t.push_back(TuneRow{max_n[i], bits[i], schedule[i]});
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune = std::move(t);
+    return EGTB_OK;
 }
 
 void egtb_stats_enable(int on) { g_stats_on.store(on ? 1 : 0); }

@@ -51,7 +51,8 @@ class Options(ctypes.Structure):
         ("keep_raw", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("radix_bits", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("sort_schedule", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -85,6 +86,10 @@ SIGNATURES = {
     "egtb_table_free": (None, [ctypes.c_void_p]),
     "sort_uint64": (None, [u64p, ctypes.c_size_t, ctypes.c_bool]),
     "unique_sorted_u64_inplace": (ctypes.c_size_t, [u64p, ctypes.c_size_t]),
+    "egtb_set_sort_tuning": (ctypes.c_int, [u64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                            ctypes.c_int]),
+    "egtb_bench_sort": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]),
     "egtb_stats_enable": (None, [ctypes.c_int]),
     "egtb_stats_reset": (None, []),
     "egtb_stats_get": (ctypes.c_int, [ctypes.c_char_p, u64p, ctypes.POINTER(ctypes.c_double),
@@ -118,7 +123,35 @@ def lib():
                 fn.restype = res
                 fn.argtypes = args
             _lib = L
+            _install_cached_tuning(L)
         return _lib
+
+
+TUNING_FILE = os.path.join(_HERE, "tuning", "sort_gfx950.json")
+
+
+def _install_cached_tuning(L) -> None:
+    """Install the autotuner's committed sort table, when there is one."""
+    import json
+
+    try:
+        with open(TUNING_FILE) as fh:
+            rows = json.load(fh)["rows"]
+    except (OSError, ValueError, KeyError):
+        return
+    install_sort_tuning(rows, L)
+
+
+def install_sort_tuning(rows, L=None) -> None:
+    """rows: [{"max_n": int, "bits": int, "schedule": int}, ...] ascending."""
+    L = L or lib()
+    n = len(rows)
+    max_n = (ctypes.c_uint64 * max(n, 1))(*[int(r["max_n"]) for r in rows])
+    bits = (ctypes.c_int32 * max(n, 1))(*[int(r["bits"]) for r in rows])
+    sched = (ctypes.c_int32 * max(n, 1))(*[int(r["schedule"]) for r in rows])
+    rc = L.egtb_set_sort_tuning(max_n, bits, sched, n)
+    if rc != 0:
+        raise EgtbError(rc, last_error())
 
 
 def last_error() -> str:

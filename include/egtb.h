@@ -60,7 +60,8 @@ typedef struct egtb_options {
     int32_t keep_raw;                /* 1: keep raw layers for egtb_result_raw */
     int32_t device;                  /* HIP device ordinal */
     int32_t radix_bits;              /* 0 = autotuned; else 4, 6, 8 or 11 */
-    int32_t reserved[7];
+    int32_t sort_schedule;           /* 0 = autotuned, 1 = per-digit 3 kernels, 2 = single-pass look-back */
+    int32_t reserved[6];
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file
@@ -146,6 +147,22 @@ EGTB_EXPORT void sort_uint64(uint64_t *arr, size_t arrsize, bool descending);
 /* Adjacent dedup of a sorted host array in place; returns the new length.
  * Same name and signature as native_core/src/unique_wrapper.cpp:12-14. */
 EGTB_EXPORT size_t unique_sorted_u64_inplace(uint64_t *data, size_t size);
+
+/* ---- tuning ---------------------------------------------------------------- */
+
+/* Install the sort tuning table used when options ask for autotuned values
+ * (radix_bits = 0, sort_schedule = 0): a sort of n keys uses the first row
+ * with n <= max_n[i] (rows ascending; the last row also covers larger n).
+ * rows = 0 restores the built-in defaults.  No counterpart in the reference
+ * (its sort has no tunables); written by the package's autotuner. */
+EGTB_EXPORT int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32_t *schedule,
+                                     int rows);
+
+/* Time sort + unique of n host keys on device 0 with a given digit width
+ * (4, 6, 8, 11) and schedule (1 per-digit kernels, 2 look-back); average
+ * milliseconds over `reps` runs after one warm-up.  For the autotuner. */
+EGTB_EXPORT int egtb_bench_sort(const uint64_t *keys, size_t n, int bits, int schedule, int reps,
+                                double *ms_per_sort);
 
 /* ---- kernel statistics (bench / profiling) ------------------------------- */
 

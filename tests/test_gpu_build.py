@@ -64,18 +64,21 @@ def test_matches_reference_golden(pkg, tag):
         res.close()
 
 
-@pytest.mark.parametrize("bits", [4, 6, 8, 11])
-def test_radix_widths_agree(pkg, bits):
-    """The digit width is a tuning knob: every width gives the same table."""
+@pytest.mark.parametrize("bits,schedule", [(b, s) for s in (1, 2) for b in (4, 6, 8, 11)])
+def test_sort_tunings_agree(pkg, bits, schedule):
+    """Digit width and sort schedule are tuning knobs: every choice gives the
+    reference's table."""
     p = pkg.patterns.build_params("442", 7)
     gp = dict(seeds=p["seeds"], masks=p["masks"], shifts=p["shifts"], symm=p["symm_mode"], target=7,
               steps=p["steps"], docheck=p["docheck_step"])
     g = load_golden("442_128")
-    res = _build(pkg, gp, radix_bits=bits)
+    res = pkg.run_pattern_build(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                                gp["docheck"], "", write_files=False, keep=True, radix_bits=bits,
+                                sort_schedule=schedule)
     try:
         for layer in g["layers"]:
             keys, rates = res.book(layer["i"])
-            assert book_sha(keys, rates) == layer["sha256"], f"layer {layer['i']} with {bits}-bit digits"
+            assert book_sha(keys, rates) == layer["sha256"], f"layer {layer['i']}, {bits}-bit, schedule {schedule}"
     finally:
         res.close()
 
