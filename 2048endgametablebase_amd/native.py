@@ -56,6 +56,35 @@ class Options(ctypes.Structure):
     ]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("slots", ctypes.c_int32),
+        ("max_batch", ctypes.c_uint32),
+        ("reserved0", ctypes.c_int32),
+        ("memory_budget", ctypes.c_int64),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class LoadgenOptions(ctypes.Structure):
+    _fields_ = [
+        ("qps", ctypes.c_double),
+        ("duration_s", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("n_games", ctypes.c_int32),
+        ("table_id", ctypes.c_int32),
+        ("pool", ctypes.POINTER(ctypes.c_uint64)),
+        ("pool_size", ctypes.c_size_t),
+    ]
+
+
+class LoadgenReport(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("qps_offered", "qps_achieved", "p50_ms", "p90_ms", "p99_ms",
+                                               "p999_ms", "max_ms", "mean_ms", "submit_seconds")] + \
+               [(n, ctypes.c_uint64) for n in ("completed", "batches", "max_batch", "fallbacks")]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "egtb_build": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(Pattern), ctypes.POINTER(Options)]),
@@ -84,6 +113,16 @@ SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32)]),
     "egtb_table_free": (None, [ctypes.c_void_p]),
+    "egtb_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    "egtb_server_attach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "egtb_server_detach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "egtb_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u64p, ctypes.c_size_t, u64p]),
+    "egtb_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, u64p, f32p, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_size_t), ctypes.c_int32]),
+    "egtb_server_stats": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, u64p]),
+    "egtb_server_destroy": (None, [ctypes.c_void_p]),
+    "egtb_server_loadgen": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LoadgenOptions),
+                                           ctypes.POINTER(LoadgenReport)]),
     "sort_uint64": (None, [u64p, ctypes.c_size_t, ctypes.c_bool]),
     "unique_sorted_u64_inplace": (ctypes.c_size_t, [u64p, ctypes.c_size_t]),
     "egtb_set_sort_tuning": (ctypes.c_int, [u64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),

@@ -137,6 +137,65 @@ EGTB_EXPORT int egtb_lookup_batch_f64(egtb_table *t, const uint64_t *boards, siz
                                       int32_t *status, int32_t *op_used);
 EGTB_EXPORT void egtb_table_free(egtb_table *t);
 
+/* ---- serving: continuous batching over resident tables -------------------- */
+
+/* A request loop that keeps tables in HBM and answers position queries.
+ * A worker owns `slots` batch slots (own stream + pinned buffers each);
+ * whenever a slot is free and queries wait, it takes up to max_batch of
+ * them (oldest table first) and launches the lookup, so queries arriving
+ * while other batches are on the GPU are admitted at once and batches
+ * overlap on the device.  Tables are admitted against memory_budget bytes
+ * (0 = unlimited).  Replaces the reference's per-call on-disk reader for
+ * serving use (ReaderRuntime.cpp:1055-1068); no reference counterpart for
+ * the batching itself. */
+typedef struct egtb_server egtb_server;
+typedef struct egtb_server_options {
+    int32_t device;
+    int32_t slots;       /* batches in flight (default 4) */
+    uint32_t max_batch;  /* queries per batch (default 4096) */
+    int32_t reserved0;
+    int64_t memory_budget;
+    int32_t reserved[4];
+} egtb_server_options;
+
+EGTB_EXPORT int egtb_server_create(const egtb_server_options *options, egtb_server **out);
+/* Admit a table (fails with EGTB_ERR_MEMORY past the budget). */
+EGTB_EXPORT int egtb_server_attach(egtb_server *s, egtb_table *table, int32_t *table_id);
+/* Release a table once its queued and in-flight queries are done. */
+EGTB_EXPORT int egtb_server_detach(egtb_server *s, int32_t table_id);
+/* Queue n boards for table_id; tickets[i] identifies each answer. */
+EGTB_EXPORT int egtb_server_submit(egtb_server *s, int32_t table_id, const uint64_t *boards, size_t n,
+                                   uint64_t *tickets);
+/* Take up to max finished answers (4 floats + status each, as
+ * egtb_lookup_batch), waiting up to wait_us for the first. */
+EGTB_EXPORT int egtb_server_poll(egtb_server *s, size_t max, uint64_t *tickets, float *out4, int32_t *status,
+                                 size_t *n_out, int32_t wait_us);
+EGTB_EXPORT int egtb_server_stats(const egtb_server *s, uint64_t *batches, uint64_t *queries, uint64_t *max_batch);
+EGTB_EXPORT void egtb_server_destroy(egtb_server *s);
+
+/* Poisson load: queries at rate qps for duration_s seconds, each board
+ * drawn from pool[] by a seeded uniform index (n_games is the number of
+ * concurrent game sources the rate is split over, for reporting).  The
+ * generator owns the server's completion queue while it runs. */
+typedef struct egtb_loadgen_options {
+    double qps;
+    double duration_s;
+    uint64_t seed;
+    int32_t n_games;
+    int32_t table_id;
+    const uint64_t *pool;
+    size_t pool_size;
+} egtb_loadgen_options;
+
+typedef struct egtb_loadgen_report {
+    double qps_offered, qps_achieved;
+    double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms;
+    double submit_seconds;
+    uint64_t completed, batches, max_batch, fallbacks;
+} egtb_loadgen_report;
+
+EGTB_EXPORT int egtb_server_loadgen(egtb_server *s, const egtb_loadgen_options *options, egtb_loadgen_report *report);
+
 /* ---- dedup primitives: drop-in for the reference's bookgen_native.so ----- */
 
 /* Ascending (or descending) sort of a host array on the GPU.  Same name and

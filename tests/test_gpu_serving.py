@@ -1,0 +1,77 @@
+"""Serving: answers are identical alone (direct batched lookup) and inside
+a concurrent mix under load; admission by memory; load generator sanity."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def table(pkg):
+    res = pkg.build_in_memory("442", 8)
+    t = pkg.Table.from_result(res, "442", 8)
+    yield res, t
+    t.close()
+    res.close()
+
+
+def test_alone_equals_mixed(pkg, table):
+    from importlib import import_module
+
+    serving = import_module("2048endgametablebase_amd.serving")
+    res, t = table
+    pool = serving.make_query_pool(res, "442", n=60000, seed=3)
+    probe = pool[:5000]
+    alone, st_alone = t.lookup(probe)
+    with serving.Server(slots=4, max_batch=1024) as srv:
+        tid = srv.attach(t)
+        rng = np.random.default_rng(9)
+        tickets = []
+        # interleave the probe queries with other traffic, in uneven chunks
+        others = pool[5000:]
+        pi = oi = 0
+        while pi < probe.size:
+            k = int(rng.integers(1, 300))
+            tickets.append(srv.submit(tid, probe[pi:pi + k]))
+            pi += k
+            m = int(rng.integers(0, 2000))
+            srv.submit(tid, others[oi:oi + m])
+            oi = (oi + m) % (others.size - 2000)
+        tickets = np.concatenate(tickets)
+        got = srv.collect(np.arange(1, int(tickets.max()) + 1))
+        st = srv.stats()
+    assert st["batches"] > 4 and st["max_batch"] > 1
+    for i, tk in enumerate(tickets):
+        out, s = got[int(tk)]
+        assert s == st_alone[i]
+        assert np.array_equal(np.nan_to_num(out, nan=-1.0), np.nan_to_num(alone[i], nan=-1.0)), i
+    assert (st_alone == 1).mean() > 0.1  # the off-table fifth took the fallback
+
+
+def test_admission_by_memory(pkg, table):
+    from importlib import import_module
+
+    serving = import_module("2048endgametablebase_amd.serving")
+    _, t = table
+    _, _, nbytes = t.info()
+    with serving.Server(memory_budget=int(nbytes * 1.5)) as srv:
+        tid = srv.attach(t)
+        with pytest.raises(pkg.native.EgtbError, match="budget"):
+            srv.attach(t)
+        srv.detach(tid)
+        tid2 = srv.attach(t)
+        assert tid2 != tid
+
+
+def test_loadgen_reports_latency(pkg, table):
+    from importlib import import_module
+
+    serving = import_module("2048endgametablebase_amd.serving")
+    res, t = table
+    pool = serving.make_query_pool(res, "442", n=20000, seed=1)
+    with serving.Server(slots=4, max_batch=4096) as srv:
+        tid = srv.attach(t)
+        rep = srv.loadgen(tid, pool, qps=20000, duration_s=0.5, seed=0)
+    assert rep["completed"] == 10000
+    assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+    assert rep["fallbacks"] > 0
