@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="copy 1 GiB with torch first (a kernel of known bytes for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -129,6 +131,13 @@ def main():
 
     def one_build():
         return pkg.build_in_memory(args.pattern, args.target, success_rate_dtype="float32")
+
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         one_build().close()
