@@ -56,6 +56,29 @@ class Options(ctypes.Structure):
     ]
 
 
+class BatchJob(ctypes.Structure):
+    _fields_ = [
+        ("seeds", u64p),
+        ("n_seeds", ctypes.c_size_t),
+        ("pattern", Pattern),
+        ("target", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("docheck_step", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("spawn_rate4", ctypes.c_double),
+    ]
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("radix_bits", ctypes.c_int32),
+        ("sort_schedule", ctypes.c_int32),
+        ("keep_raw", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
 class ServerOptions(ctypes.Structure):
     _fields_ = [
         ("device", ctypes.c_int32),
@@ -101,6 +124,14 @@ SIGNATURES = {
     "egtb_result_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "egtb_result_free": (None, [ctypes.c_void_p]),
+    "egtb_batch_build": (ctypes.c_int, [ctypes.POINTER(BatchJob), ctypes.c_int, ctypes.POINTER(BatchOptions),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "egtb_batch_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "egtb_batch_layer_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u64p, u64p]),
+    "egtb_batch_book": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, u64p, f32p, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+    "egtb_batch_free": (None, [ctypes.c_void_p]),
     "egtb_table_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(Pattern), ctypes.c_int,
                                        ctypes.c_int64, ctypes.c_char_p, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_void_p)]),

@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--target", type=int, default=9)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy 1 GiB with torch first (a kernel of known bytes for counter scaling)")
@@ -179,6 +180,14 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
                     "avg_launch_ms": round(avg_ms, 5), "launches_per_step": st["launches"] // args.steps}
 
+    batched = None
+    if not args.no_batched:
+        bm = importlib.import_module("2048endgametablebase_amd.batched")
+        bm.build_batch(bm.SMALL_JOBS).close()  # warm-up
+        batched = {"workload_24_jobs": [f"{p}@{2**t}" for p, t in bm.DEFAULT_JOBS],
+                   "uneven_24_jobs": bm.compare_with_plain(bm.DEFAULT_JOBS),
+                   "small_24_jobs": bm.compare_with_plain(bm.SMALL_JOBS, reps=3)}
+
     serving = None
     if not args.no_serving:
         serving = serving_sweep(pkg, args.pattern, args.target, args.serve_seconds)
@@ -210,6 +219,7 @@ def main():
                    "kernels": {k: {"ms_per_step": round(v["ms"] / args.steps, 3),
                                    "launches_per_step": v["launches"] // args.steps} for k, v in stats.items()}},
         "roofline": roofline,
+        "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,
     }

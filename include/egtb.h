@@ -106,6 +106,47 @@ EGTB_EXPORT int egtb_result_book(const egtb_result *r, int layer, uint64_t *keys
 EGTB_EXPORT int egtb_result_timing(const egtb_result *r, double *fwd_ms, double *bwd_ms, double *total_ms);
 EGTB_EXPORT void egtb_result_free(egtb_result *r);
 
+/* ---- batched builds: many (pattern, target) jobs through one pipeline ------ */
+
+/* One job of a batched build: the arguments of egtb_build for one table.
+ * Every mask of the pattern must pin nibbles 0 and 1 (all catalogue
+ * patterns but the free* ones do); rates are float32. */
+typedef struct egtb_batch_job {
+    const uint64_t *seeds;
+    size_t n_seeds;
+    egtb_pattern pattern;
+    int32_t target;
+    int32_t steps;
+    int32_t docheck_step;
+    int32_t reserved0;
+    double spawn_rate4;
+} egtb_batch_job;
+
+typedef struct egtb_batch_options {
+    int32_t device;
+    int32_t radix_bits;     /* 0 = autotuned */
+    int32_t sort_schedule;  /* 0 = autotuned */
+    int32_t keep_raw;
+    int32_t reserved[4];
+} egtb_batch_options;
+
+typedef struct egtb_batch_result egtb_batch_result;
+
+/* Build every job: one set of kernel launches per forward step and per
+ * backward layer covers all jobs.  Each job's layers equal its egtb_build
+ * layers.  No reference counterpart (the reference builds one table per
+ * run_pattern_build call). */
+EGTB_EXPORT int egtb_batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch_options *options,
+                                 egtb_batch_result **out);
+EGTB_EXPORT int egtb_batch_timing(const egtb_batch_result *b, double *fwd_ms, double *bwd_ms, double *total_ms);
+/* Sizes of layer `layer` summed over all jobs. */
+EGTB_EXPORT int egtb_batch_layer_info(const egtb_batch_result *b, int layer, uint64_t *n_raw_all,
+                                      uint64_t *n_live_all);
+/* Solved layer `layer` of job `job`: keys (plain encoding) and float32 rates. */
+EGTB_EXPORT int egtb_batch_book(const egtb_batch_result *b, int job, int layer, uint64_t *keys, float *rates,
+                                size_t cap, size_t *n);
+EGTB_EXPORT void egtb_batch_free(egtb_batch_result *b);
+
 /* ---- solved tables resident in HBM and batched lookups ------------------- */
 
 typedef struct egtb_table egtb_table;

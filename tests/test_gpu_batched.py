@@ -1,0 +1,52 @@
+"""Batched builds: every job's solved layers equal the reference golden
+digests / the plain GPU build of that job alone."""
+import numpy as np
+import pytest
+
+from conftest import book_sha, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_matches_plain_and_golden(pkg):
+    from importlib import import_module
+
+    batched = import_module("2048endgametablebase_amd.batched")
+    jobs = [("442", 7), ("L3", 7), ("LL", 6), ("444", 6), ("t", 6), ("442", 5), ("4442", 4), ("4431", 5)]
+    goldens = {("442", 7): "442_128", ("L3", 7): "L3_128", ("LL", 6): "LL_64", ("444", 6): "444_64"}
+    with batched.build_batch(jobs) as br:
+        for j, (pat, t) in enumerate(jobs):
+            print(f"case job {j} {pat}@{2**t}", flush=True)
+            if (pat, t) in goldens:
+                g = load_golden(goldens[(pat, t)])
+                for layer in g["layers"]:
+                    keys, rates = br.book(j, layer["i"])
+                    assert book_sha(keys, rates) == layer["sha256"], (pat, t, layer["i"])
+            else:
+                res = pkg.build_in_memory(pat, t)
+                try:
+                    for i in range(res.steps):
+                        pk, pr = res.book(i)
+                        bk, brr = br.book(j, i)
+                        assert np.array_equal(pk, bk) and np.array_equal(pr.view(np.uint32), brr.view(np.uint32)), \
+                            (pat, t, i)
+                finally:
+                    res.close()
+
+
+def test_batch_rejects_unpinned_patterns(pkg):
+    from importlib import import_module
+
+    native = pkg.native
+    job = native.BatchJob()
+    seeds = np.array([0x01111111FFFFFFFF], dtype=np.uint64)
+    job.seeds = seeds.ctypes.data_as(native.u64p)
+    job.n_seeds = 1
+    job.pattern = native.Pattern(None, 0, None, 0, 1)
+    job.target, job.steps, job.docheck_step, job.spawn_rate4 = 7, 100, 50, 0.1
+    o = native.BatchOptions()
+    import ctypes
+
+    h = ctypes.c_void_p()
+    rc = native.lib().egtb_batch_build(ctypes.byref(job), 1, ctypes.byref(o), ctypes.byref(h))
+    assert rc != 0 and "pin nibbles" in native.last_error()
