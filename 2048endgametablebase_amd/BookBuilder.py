@@ -45,7 +45,8 @@ class _Args:
     """ctypes argument block kept alive for the duration of a native call."""
 
     def __init__(self, seeds, masks, shifts, symm, target, steps, docheck, pathname, is_free,
-                 spawn_rate4, dtype, write_files, keep_raw, device, radix_bits, sort_schedule=0):
+                 spawn_rate4, dtype, write_files, keep_raw, device, radix_bits, sort_schedule=0,
+                 chunk_mkeys=0, host_tables=False):
         self.seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
         self.masks = np.ascontiguousarray(np.asarray(list(masks) or [0], dtype=np.uint64))
         self.shifts = np.ascontiguousarray(np.asarray(list(shifts) or [0], dtype=np.uint8))
@@ -68,6 +69,8 @@ class _Args:
         o.device = int(device)
         o.radix_bits = int(radix_bits)
         o.sort_schedule = int(sort_schedule)
+        o.chunk_mkeys = int(chunk_mkeys)
+        o.host_tables = int(bool(host_tables))
 
 
 class BuildResult:
@@ -107,6 +110,15 @@ class BuildResult:
                                         rates.ctypes.data_as(ctypes.c_void_p), keys.size, ctypes.byref(n)))
         return keys, rates
 
+    def memory(self):
+        """(peak HBM bytes, streamed bytes, stream ms, chunked steps, chunks)."""
+        a, b, d, e = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        c = ctypes.c_double()
+        native.check(native.lib().egtb_result_memory(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                     ctypes.byref(d), ctypes.byref(e)))
+        return {"peak_device_bytes": a.value, "streamed_bytes": b.value, "stream_ms": c.value,
+                "chunked_steps": d.value, "chunks": e.value}
+
     def timing(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         native.check(native.lib().egtb_result_timing(self._h, ctypes.byref(a), ctypes.byref(b),
@@ -135,13 +147,15 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
                       steps: int, docheck_step: int, pathname: str = "", *, is_free: bool = False,
                       spawn_rate4: float = 0.1, success_rate_dtype: str = "float32",
                       write_files: bool = True, keep: bool = False, keep_raw: bool = False,
-                      device: int = 0, radix_bits: int = 0, sort_schedule: int = 0) -> Optional[BuildResult]:
+                      device: int = 0, radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
+                      host_tables: bool = False) -> Optional[BuildResult]:
     """The native build call (reference formation_core.run_pattern_build,
     native_core/src/bindings_formation.cpp:220-229) with the PatternSpec and
     RunOptions fields spelled out.  ``keep=True`` returns the solved layers
     resident in HBM as a BuildResult."""
     a = _Args(arr_init, masks, shifts, symm_mode, target, steps, docheck_step, pathname, is_free,
-              spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits, sort_schedule)
+              spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits, sort_schedule,
+              chunk_mkeys, host_tables)
     L = native.lib()
     seeds_p = a.seeds.ctypes.data_as(native.u64p)
     if keep or keep_raw:
@@ -201,11 +215,12 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
 
 def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "float32",
                     spawn_rate4: float = 0.1, keep_raw: bool = False, device: int = 0,
-                    radix_bits: int = 0, sort_schedule: int = 0) -> BuildResult:
+                    radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
+                    host_tables: bool = False) -> BuildResult:
     """Build a catalogue pattern and keep every solved layer in HBM."""
     p = build_params(pattern, target)
     return run_pattern_build(p["seeds"], p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"],
                              p["docheck_step"], "", is_free=p["is_free"], spawn_rate4=spawn_rate4,
                              success_rate_dtype=success_rate_dtype, write_files=False, keep=True,
                              keep_raw=keep_raw, device=device, radix_bits=radix_bits,
-                             sort_schedule=sort_schedule)
+                             sort_schedule=sort_schedule, chunk_mkeys=chunk_mkeys, host_tables=host_tables)

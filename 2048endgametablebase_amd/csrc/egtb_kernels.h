@@ -54,7 +54,8 @@ void launch_build_lut(hipStream_t s, uint16_t *lut);
 void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                    const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
-                   uint64_t *or_and4);
+                   uint64_t *or_and4, uint64_t offs_base = 0);
+void launch_gather_u64(hipStream_t s, const uint64_t *src, const uint64_t *idx, uint32_t m, uint64_t *out);
 
 uint64_t scan_partials_needed(uint64_t n);
 void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n_bound, const uint64_t *n_ptr, uint64_t *out,

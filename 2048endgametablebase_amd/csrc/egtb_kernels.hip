@@ -115,7 +115,7 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                                                  const uint64_t *__restrict__ n_ptr,
                                                  const uint16_t *__restrict__ glut, PatternDev p,
                                                  int do_check, uint64_t *__restrict__ counts,
-                                                 const uint64_t *__restrict__ offs,
+                                                 const uint64_t *__restrict__ offs, uint64_t offs_base,
                                                  uint64_t *__restrict__ out2,
                                                  uint64_t *__restrict__ out4,
                                                  uint64_t *__restrict__ or_and2,
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
         uint32_t c2 = 0, c4 = 0;
         uint64_t w2 = 0, w4 = 0;
         if (WRITE) {
-            const uint64_t o = offs[i];
+            const uint64_t o = offs[i] - offs_base;  // both halves stay >= 0
             w2 = o & 0xFFFFFFFFull;
             w4 = o >> 32;
         }
@@ -955,6 +955,13 @@ __global__ void k_pack_records(const uint64_t *__restrict__ keys, const T *__res
     if constexpr (W == 4) rec[3] = (uint32_t)(rb >> 32);
 }
 
+// out[k] = src[idx[k]] for k < m (offsets at chunk boundaries).
+__global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *__restrict__ idx, uint32_t m,
+                             uint64_t *__restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) out[k] = src[idx[k]];
+}
+
 // Packed records -> SoA (the reader's table load).
 __global__ void k_unpack_records(const uint32_t *__restrict__ recs, uint64_t n, int words, uint64_t *__restrict__ keys,
                                  uint32_t *__restrict__ rates) {
@@ -989,17 +996,17 @@ void launch_build_lut(hipStream_t s, uint16_t *lut) {
 void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                    const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
-                   uint64_t *or_and4) {
+                   uint64_t *or_and4, uint64_t offs_base) {
     if (n_bound == 0) return;
     const uint32_t want = blocks_for(n_bound, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     const size_t lds = kLutEntries * sizeof(uint16_t);
     if (write)
         hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
-                           do_check, counts, offs, out2, out4, or_and2, or_and4);
+                           do_check, counts, offs, offs_base, out2, out4, or_and2, or_and4);
     else
         hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
-                           do_check, counts, offs, out2, out4, or_and2, or_and4);
+                           do_check, counts, offs, (uint64_t)0, out2, out4, or_and2, or_and4);
 }
 
 template <typename TI, typename TO>
@@ -1166,6 +1173,10 @@ template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out) {
     if (n == 0) return;
     hipLaunchKernelGGL((k_pack_records<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, rates, n, out);
+}
+
+void launch_gather_u64(hipStream_t s, const uint64_t *src, const uint64_t *idx, uint32_t m, uint64_t *out) {
+    if (m) hipLaunchKernelGGL(k_gather_u64, dim3((m + 255) / 256), dim3(256), 0, s, src, idx, m, out);
 }
 
 void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int rate_size, uint64_t *keys,

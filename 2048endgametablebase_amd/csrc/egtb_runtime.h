@@ -161,6 +161,14 @@ struct Result {
     std::vector<Book> books;
     std::vector<LayerInfo> info;
     double fwd_ms = 0, bwd_ms = 0, total_ms = 0;
+    // host-tables mode: books live in per-layer device buffers until they are
+    // no longer needed, then in host memory
+    bool host_tables = false;
+    std::vector<DBuf> book_bufs;  // 3 per layer: keys, rates, index
+    std::vector<std::vector<uint64_t>> host_keys;
+    std::vector<std::vector<char>> host_rates;
+    uint64_t streamed_bytes = 0, chunked_steps = 0, chunks = 0, peak_device_bytes = 0;
+    double stream_ms = 0;
     LayerDev *meta_dev() const { return (LayerDev *)meta.p; }
 };
 

@@ -17,6 +17,9 @@
 #include <string>
 #include <sys/stat.h>
 #include <unistd.h>
+#include <condition_variable>
+#include <deque>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -486,8 +489,14 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     Book &bk = res.books[layer];
     LayerDev *md = res.meta_dev() + layer;
     bk.n_dev = &md->n_live;
-    bk.keys = (uint64_t *)res.book_arena.alloc(bound * 8);
-    bk.rates = res.book_arena.alloc(bound * sizeof(T));
+    auto carve = [&](size_t bytes, int which) -> void * {
+        if (!res.host_tables) return res.book_arena.alloc(bytes);
+        DBuf &d = res.book_bufs[(size_t)layer * 3 + which];
+        d.alloc(bytes + 256, s);
+        return d.p;
+    };
+    bk.keys = (uint64_t *)carve(bound * 8, 0);
+    bk.rates = carve(bound * sizeof(T), 1);
     if (bound) {
         const uint64_t tiles = compact_tiles(bound);
         w.tile_counts.ensure(tiles * 8, s);
@@ -502,10 +511,117 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     }
     if (index) {
         bk.bits = index_bits_for(bound);
-        bk.idx = (uint32_t *)res.book_arena.alloc(((1ull << bk.bits) + 1) * 4);
+        bk.idx = (uint32_t *)carve(((1ull << bk.bits) + 1) * 4, 2);
         launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
     }
 }
+
+// Host-tables mode: a copy thread moves each finished book (keys + rates)
+// to host memory through two pinned staging buffers on its own stream, after
+// the event that marks the book's last use, then frees its device buffers.
+class BookStreamer {
+  public:
+    BookStreamer(Result &res, int rate_size) : res_(res), rate_size_(rate_size) {
+        HIPCHK(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking));
+        for (int k = 0; k < 2; ++k) HIPCHK(hipHostMalloc(&stage_[k], kStage, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&n_host_, 8, hipHostMallocDefault));
+        th_ = std::thread([this] { run(); });
+    }
+    ~BookStreamer() { finish(); }
+
+    // book `layer`'s device buffers are read by work recorded before `ev`
+    void retire(int layer, hipStream_t s) {
+        hipEvent_t ev;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(ev, s));
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back({layer, ev});
+        }
+        cv_.notify_one();
+    }
+
+    void finish() {
+        if (!th_.joinable()) return;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_one();
+        th_.join();
+        for (int k = 0; k < 2; ++k) (void)hipHostFree(stage_[k]);
+        (void)hipHostFree(n_host_);
+        (void)hipStreamDestroy(cs_);
+        if (!err_.empty()) throw Error(EGTB_ERR_HIP, "streaming a layer to the host failed: " + err_);
+    }
+
+  private:
+    static constexpr size_t kStage = (size_t)64 << 20;
+    struct Job {
+        int layer;
+        hipEvent_t ev;
+    };
+
+    void copy_out(const void *src, size_t bytes, char *dst) {
+        size_t off = 0;
+        int k = 0;
+        while (off < bytes) {
+            const size_t m = std::min(kStage, bytes - off);
+            HIPCHK(hipMemcpyAsync(stage_[k], (const char *)src + off, m, hipMemcpyDeviceToHost, cs_));
+            HIPCHK(hipStreamSynchronize(cs_));
+            std::memcpy(dst + off, stage_[k], m);
+            off += m;
+            k ^= 1;
+        }
+    }
+
+    void run() {
+        (void)hipSetDevice(res_.dev);
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                j = q_.front();
+                q_.pop_front();
+            }
+            try {
+                const auto t0 = std::chrono::steady_clock::now();
+                HIPCHK(hipStreamWaitEvent(cs_, j.ev, 0));
+                Book &bk = res_.books[j.layer];
+                HIPCHK(hipMemcpyAsync(n_host_, bk.n_dev, 8, hipMemcpyDeviceToHost, cs_));
+                HIPCHK(hipStreamSynchronize(cs_));
+                const uint64_t n = *n_host_;
+                res_.host_keys[j.layer].resize(n);
+                res_.host_rates[j.layer].resize(n * (size_t)rate_size_);
+                copy_out(bk.keys, n * 8, (char *)res_.host_keys[j.layer].data());
+                copy_out(bk.rates, n * (size_t)rate_size_, res_.host_rates[j.layer].data());
+                res_.streamed_bytes += n * (8 + (uint64_t)rate_size_);
+                for (int w = 0; w < 3; ++w) res_.book_bufs[(size_t)j.layer * 3 + w].release();
+                bk.keys = nullptr;
+                bk.rates = nullptr;
+                bk.idx = nullptr;
+                res_.stream_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            } catch (const std::exception &e) {
+                err_ = e.what();
+            }
+            (void)hipEventDestroy(j.ev);
+        }
+    }
+
+    Result &res_;
+    int rate_size_;
+    hipStream_t cs_ = nullptr;
+    void *stage_[2] = {nullptr, nullptr};
+    uint64_t *n_host_ = nullptr;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Job> q_;
+    bool stop_ = false;
+    std::string err_;
+};
 
 template <typename T>
 static void write_book_file(hipStream_t s, const std::string &path, const Book &bk, uint64_t n, HostStage &stage,
@@ -604,45 +720,177 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                         (uint64_t *)w.partial.p, &M[step].tot);
     };
 
+    // ---- chunked steps: runs (sorted unique pieces) of the layer being
+    // assembled and of the next one; used once a step's candidates exceed
+    // the budget, and from then on for the rest of the forward pass
+    bool chunked = false;
+    std::vector<DBuf> runs_cur, runs_next;
+    std::vector<uint64_t> runs_cur_n, runs_next_n;
+    DBuf chunk2, chunk4, run_tmp, cnt_dev, gidx, gval;
+    cnt_dev.alloc(64, s);
+    auto budget_keys = [&]() -> uint64_t {
+        if (o.chunk_mkeys > 0) return (uint64_t)o.chunk_mkeys << 20;
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        // a chunk's two streams, the sort scratch and the unique output
+        return std::max<uint64_t>(1ull << 20, (uint64_t)(fr / 8 / 6));
+    };
+    // sort + unique keys[0..n) into an exact-size run (synchronises)
+    auto to_run = [&](uint64_t *keys, uint64_t n, uint64_t or_all, uint64_t and_all, std::vector<DBuf> &runs,
+                      std::vector<uint64_t> &runs_n) {
+        if (n == 0) return;
+        run_tmp.ensure(n * 8, s);
+        uint64_t *sorted;
+        {
+            Timer t(s, "sort", 16.0 * (double)n);
+            sorted = radix_sort(s, keys, (uint64_t *)run_tmp.p, n, or_all, and_all, o.radix_bits, w,
+                                o.sort_schedule);
+        }
+        uint64_t *u = sorted == keys ? (uint64_t *)run_tmp.p : keys;
+        {
+            Timer t(s, "unique", 16.0 * (double)n);
+            unique_into(s, sorted, n, u, (uint64_t *)cnt_dev.p, w, o.sort_schedule);
+        }
+        uint64_t m = 0;
+        HIPCHK(hipMemcpyAsync(&m, cnt_dev.p, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        DBuf r;
+        r.alloc(m * 8, s);
+        if (m) HIPCHK(hipMemcpyAsync(r.p, u, m * 8, hipMemcpyDeviceToDevice, s));
+        runs.push_back(std::move(r));
+        runs_n.push_back(m);
+    };
+    // layer `L` = sort + unique of the concatenation of its runs
+    auto merge_runs = [&](int L, std::vector<DBuf> &runs, std::vector<uint64_t> &runs_n) -> uint64_t {
+        uint64_t tot = 0;
+        for (uint64_t v : runs_n) tot += v;
+        DBuf cat;
+        cat.alloc(tot * 8 + 8, s);
+        uint64_t off = 0;
+        for (size_t k = 0; k < runs.size(); ++k) {
+            if (runs_n[k])
+                HIPCHK(hipMemcpyAsync((uint64_t *)cat.p + off, runs[k].p, runs_n[k] * 8, hipMemcpyDeviceToDevice, s));
+            off += runs_n[k];
+        }
+        runs.clear();
+        runs_n.clear();
+        run_tmp.ensure(tot * 8, s);
+        uint64_t *sorted;
+        {
+            Timer t(s, "sort", 16.0 * (double)tot);
+            sorted = radix_sort(s, (uint64_t *)cat.p, (uint64_t *)run_tmp.p, tot, H[L].or_all, H[L].and_all,
+                                o.radix_bits, w, o.sort_schedule);
+        }
+        res.raw[L] = (uint64_t *)res.raw_arena.alloc(tot * 8);
+        {
+            Timer t(s, "unique", 16.0 * (double)tot);
+            unique_into(s, sorted, tot, res.raw[L], &M[L].n_raw, w, o.sort_schedule);
+        }
+        return tot;
+    };
+
     uint64_t c4_prev = 0;
     if (steps >= 3) launch_count(1, n_seeds);
     for (int i = 1; i <= steps - 2; ++i) {
         progress_set((uint32_t)i, prog_total);
-        read_meta(s, res, H, i - 1, i);
+        read_meta(s, res, H, i - 1, i + 1);
         const uint64_t n_par = H[i - 1].n_raw;
         if (i > 1) res.raw_arena.trim_last(res.raw[i - 1], n_par * 8);
         res.info[i - 1].n_raw = n_par;
+        if (n_par * 64ull >= (1ull << 32))
+            throw Error(EGTB_ERR_ARG, "layer " + std::to_string(i - 1) + " has too many states for one step");
         const uint64_t c2 = H[i].tot & 0xFFFFFFFFull, c4 = H[i].tot >> 32;
         res.info[i].c2 = c2;
         res.info[i].c4 = c4;
         const uint64_t n_c = c4_prev + c2;
-        DBuf &cur = cand[i & 1], &nxt = cand[(i + 1) & 1];
-        if (cur.bytes < n_c * 8) {  // grow, keeping the spawn-4 prefix
-            DBuf g;
-            g.alloc(n_c * 8 + n_c * 4 + 256, s);
-            if (c4_prev) HIPCHK(hipMemcpyAsync(g.p, cur.p, c4_prev * 8, hipMemcpyDeviceToDevice, s));
-            cur = std::move(g);
+        const int do_check = i > o.docheck_step ? 1 : 0;
+        const uint64_t budget = budget_keys();
+        if (!chunked && (n_c > budget || c4 > budget)) {
+            // switch to runs: the spawn-4 prefix already in `cur` becomes a run
+            chunked = true;
+            if (c4_prev)
+                to_run((uint64_t *)cand[i & 1].p, c4_prev, H[i].or_all, H[i].and_all, runs_cur, runs_cur_n);
+            cand[0].release();
+            cand[1].release();
+            sort_tmp.release();
         }
-        nxt.ensure(c4 * 8, s);
-        sort_tmp.ensure(n_c * 8, s);
-        if (n_par && (c2 || c4)) {
-            Timer t(s, "expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
-            launch_expand(s, true, C.shape, res.raw[i - 1], n_par, nullptr, C.lut, p, i > o.docheck_step ? 1 : 0,
-                          nullptr, (const uint64_t *)offs.p, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p,
-                          nullptr, nullptr);
+        uint64_t bound_next;
+        if (!chunked) {
+            DBuf &cur = cand[i & 1], &nxt = cand[(i + 1) & 1];
+            if (cur.bytes < n_c * 8) {  // grow, keeping the spawn-4 prefix
+                DBuf g;
+                g.alloc(n_c * 8 + n_c * 4 + 256, s);
+                if (c4_prev) HIPCHK(hipMemcpyAsync(g.p, cur.p, c4_prev * 8, hipMemcpyDeviceToDevice, s));
+                cur = std::move(g);
+            }
+            nxt.ensure(c4 * 8, s);
+            sort_tmp.ensure(n_c * 8, s);
+            if (n_par && (c2 || c4)) {
+                Timer t(s, "expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
+                launch_expand(s, true, C.shape, res.raw[i - 1], n_par, nullptr, C.lut, p, do_check, nullptr,
+                              (const uint64_t *)offs.p, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, nullptr,
+                              nullptr);
+            }
+            uint64_t *sorted;
+            {
+                Timer t(s, "sort", 16.0 * (double)n_c);
+                sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all,
+                                    o.radix_bits, w, o.sort_schedule);
+            }
+            res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
+            {
+                Timer t(s, "unique", 16.0 * (double)n_c);
+                unique_into(s, sorted, n_c, res.raw[i], &M[i].n_raw, w, o.sort_schedule);
+            }
+            bound_next = n_c;
+        } else {
+            // parent ranges whose candidates fit the budget, from the prefix
+            res.chunked_steps++;
+            uint32_t K = (uint32_t)std::max<uint64_t>(1, (std::max(c2, c4) + budget - 1) / budget);
+            std::vector<uint64_t> bnd, ofs;
+            for (;;) {
+                bnd.assign(K + 1, 0);
+                for (uint32_t k = 0; k <= K; ++k) bnd[k] = n_par * k / K;
+                ofs.assign(K + 1, 0);
+                gidx.ensure((K + 1) * 8, s);
+                gval.ensure((K + 1) * 8, s);
+                HIPCHK(hipMemcpyAsync(gidx.p, bnd.data(), K * 8, hipMemcpyHostToDevice, s));
+                launch_gather_u64(s, (const uint64_t *)offs.p, (const uint64_t *)gidx.p, K, (uint64_t *)gval.p);
+                HIPCHK(hipMemcpyAsync(ofs.data(), gval.p, K * 8, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                ofs[K] = H[i].tot;
+                bool fits = true;
+                for (uint32_t k = 0; k < K; ++k) {
+                    const uint64_t d = ofs[k + 1] - ofs[k];
+                    fits &= (d & 0xFFFFFFFFull) <= budget && (d >> 32) <= budget;
+                }
+                if (fits || K >= n_par) break;
+                K *= 2;
+            }
+            for (uint32_t k = 0; k < K; ++k) {
+                const uint64_t a = bnd[k], b = bnd[k + 1];
+                const uint64_t d = ofs[k + 1] - ofs[k];
+                const uint64_t k2 = d & 0xFFFFFFFFull, k4 = d >> 32;
+                if (a == b || (k2 == 0 && k4 == 0)) continue;
+                res.chunks++;
+                chunk2.ensure(k2 * 8 + 8, s);
+                chunk4.ensure(k4 * 8 + 8, s);
+                {
+                    Timer t(s, "expand", 8.0 * (double)(b - a) + 8.0 * (double)(k2 + k4));
+                    launch_expand(s, true, C.shape, res.raw[i - 1] + a, b - a, nullptr, C.lut, p, do_check, nullptr,
+                                  (const uint64_t *)offs.p + a, (uint64_t *)chunk2.p, (uint64_t *)chunk4.p, nullptr,
+                                  nullptr, ofs[k]);
+                }
+                to_run((uint64_t *)chunk2.p, k2, H[i].or_all, H[i].and_all, runs_cur, runs_cur_n);
+                to_run((uint64_t *)chunk4.p, k4, H[i + 1].or_all, H[i + 1].and_all, runs_next, runs_next_n);
+            }
+            bound_next = merge_runs(i, runs_cur, runs_cur_n);
+            runs_cur = std::move(runs_next);
+            runs_cur_n = std::move(runs_next_n);
+            runs_next.clear();
+            runs_next_n.clear();
         }
-        uint64_t *sorted;
-        {
-            Timer t(s, "sort", 16.0 * (double)n_c);
-            sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all,
-                                o.radix_bits, w, o.sort_schedule);
-        }
-        res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-        {
-            Timer t(s, "unique", 16.0 * (double)n_c);
-            unique_into(s, sorted, n_c, res.raw[i], &M[i].n_raw, w, o.sort_schedule);
-        }
-        if (i + 1 <= steps - 2) launch_count(i + 1, n_c);
+        if (i + 1 <= steps - 2) launch_count(i + 1, bound_next);
         HIPCHK(hipGetLastError());
         c4_prev = c4;
     }
@@ -652,21 +900,28 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         read_meta(s, res, H, L - 1, L);
         res.raw_arena.trim_last(res.raw[L - 1], H[L - 1].n_raw * 8);
         res.info[L - 1].n_raw = H[L - 1].n_raw;
-        const uint64_t n_c = c4_prev;
-        DBuf &cur = cand[L & 1];
-        sort_tmp.ensure(n_c * 8, s);
-        uint64_t *sorted;
-        {
-            Timer t(s, "sort", 16.0 * (double)n_c);
-            sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all,
-                                o.radix_bits, w, o.sort_schedule);
+        if (!chunked) {
+            const uint64_t n_c = c4_prev;
+            DBuf &cur = cand[L & 1];
+            sort_tmp.ensure(n_c * 8, s);
+            uint64_t *sorted;
+            {
+                Timer t(s, "sort", 16.0 * (double)n_c);
+                sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all,
+                                    o.radix_bits, w, o.sort_schedule);
+            }
+            res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
+            unique_into(s, sorted, n_c, res.raw[L], &M[L].n_raw, w, o.sort_schedule);
+        } else {
+            merge_runs(L, runs_cur, runs_cur_n);
         }
-        res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-        unique_into(s, sorted, n_c, res.raw[L], &M[L].n_raw, w, o.sort_schedule);
         read_meta(s, res, H, L, L);
         res.raw_arena.trim_last(res.raw[L], H[L].n_raw * 8);
         res.info[L].n_raw = H[L].n_raw;
     }
+    chunk2.release();
+    chunk4.release();
+    run_tmp.release();
     cand[0].release();
     cand[1].release();
     sort_tmp.release();
@@ -675,6 +930,15 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     const auto t_fwd = std::chrono::steady_clock::now();
 
     // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
+    res.host_tables = o.host_tables != 0;
+    std::unique_ptr<BookStreamer> streamer;
+    if (res.host_tables) {
+        if (files) throw Error(EGTB_ERR_ARG, "host_tables and write_files cannot be combined");
+        res.book_bufs.resize((size_t)steps * 3);
+        res.host_keys.assign(steps, {});
+        res.host_rates.assign(steps, {});
+        streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T));
+    }
     uint64_t max_n = 0;
     for (int i = 0; i < steps; ++i) max_n = std::max(max_n, res.info[i].n_raw);
     rates.ensure(max_n * sizeof(T) + 8, s);
@@ -708,6 +972,12 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         }
         make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0);
         HIPCHK(hipGetLastError());
+        if (streamer) {
+            // book i+2 was last read by this layer's solve
+            streamer->retire(i + 2, s);
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess) res.peak_device_bytes = std::max<uint64_t>(res.peak_device_bytes, tot - fr);
+        }
         if (files) {
             read_meta(s, res, H, i, i);
             const auto l1 = std::chrono::steady_clock::now();
@@ -725,6 +995,11 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             }
             write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], H[i].n_live, stage, packed);
         }
+    }
+    if (streamer) {
+        streamer->retire(1, s);
+        streamer->retire(0, s);
+        streamer->finish();
     }
     if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     read_meta(s, res, H, 0, steps - 1);
@@ -864,9 +1139,25 @@ int egtb_result_book(const egtb_result *r, int layer, uint64_t *keys, void *rate
         if (n) *n = b.n;
         if (!keys && !rates) return;
         if (cap < b.n) throw Error(EGTB_ERR_ARG, "buffer too small");
+        if (r->host_tables) {
+            if (b.n && keys) std::memcpy(keys, r->host_keys[layer].data(), b.n * 8);
+            if (b.n && rates) std::memcpy(rates, r->host_rates[layer].data(), b.n * r->rate_size);
+            return;
+        }
         if (b.n && keys) HIPCHK(hipMemcpy(keys, b.keys, b.n * 8, hipMemcpyDeviceToHost));
         if (b.n && rates) HIPCHK(hipMemcpy(rates, b.rates, b.n * r->rate_size, hipMemcpyDeviceToHost));
     });
+}
+
+int egtb_result_memory(const egtb_result *r, uint64_t *peak_device_bytes, uint64_t *streamed_bytes,
+                       double *stream_ms, uint64_t *chunked_steps, uint64_t *chunks) {
+    if (!r) return EGTB_ERR_ARG;
+    if (peak_device_bytes) *peak_device_bytes = r->peak_device_bytes;
+    if (streamed_bytes) *streamed_bytes = r->streamed_bytes;
+    if (stream_ms) *stream_ms = r->stream_ms;
+    if (chunked_steps) *chunked_steps = r->chunked_steps;
+    if (chunks) *chunks = r->chunks;
+    return EGTB_OK;
 }
 
 int egtb_result_timing(const egtb_result *r, double *fwd_ms, double *bwd_ms, double *total_ms) {

@@ -52,7 +52,9 @@ class Options(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("radix_bits", ctypes.c_int32),
         ("sort_schedule", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("chunk_mkeys", ctypes.c_int32),
+        ("host_tables", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -123,6 +125,7 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_size_t)]),
     "egtb_result_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "egtb_result_memory": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, ctypes.POINTER(ctypes.c_double), u64p, u64p]),
     "egtb_result_free": (None, [ctypes.c_void_p]),
     "egtb_batch_build": (ctypes.c_int, [ctypes.POINTER(BatchJob), ctypes.c_int, ctypes.POINTER(BatchOptions),
                                         ctypes.POINTER(ctypes.c_void_p)]),

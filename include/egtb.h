@@ -61,7 +61,9 @@ typedef struct egtb_options {
     int32_t device;                  /* HIP device ordinal */
     int32_t radix_bits;              /* 0 = autotuned; else 4, 6, 8 or 11 */
     int32_t sort_schedule;           /* 0 = autotuned, 1 = per-digit 3 kernels, 2 = single-pass look-back */
-    int32_t reserved[6];
+    int32_t chunk_mkeys;             /* candidate budget per chunk in Mi keys; 0 = from free HBM */
+    int32_t host_tables;             /* 1: stream finished layers to host memory, free them in HBM */
+    int32_t reserved[4];
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file
@@ -104,6 +106,11 @@ EGTB_EXPORT int egtb_result_book(const egtb_result *r, int layer, uint64_t *keys
                      size_t *n);
 /* Timing of the last build: forward, backward and total milliseconds. */
 EGTB_EXPORT int egtb_result_timing(const egtb_result *r, double *fwd_ms, double *bwd_ms, double *total_ms);
+/* Large-build accounting: peak HBM in use during the backward pass (host
+ * tables mode), bytes and time of the layers streamed to the host, forward
+ * steps that ran chunked and their chunk count. */
+EGTB_EXPORT int egtb_result_memory(const egtb_result *r, uint64_t *peak_device_bytes, uint64_t *streamed_bytes,
+                                   double *stream_ms, uint64_t *chunked_steps, uint64_t *chunks);
 EGTB_EXPORT void egtb_result_free(egtb_result *r);
 
 /* ---- batched builds: many (pattern, target) jobs through one pipeline ------ */
