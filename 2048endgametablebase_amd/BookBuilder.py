@@ -46,7 +46,7 @@ class _Args:
 
     def __init__(self, seeds, masks, shifts, symm, target, steps, docheck, pathname, is_free,
                  spawn_rate4, dtype, write_files, keep_raw, device, radix_bits, sort_schedule=0,
-                 chunk_mkeys=0, host_tables=False):
+                 chunk_mkeys=0, host_tables=False, write_raw_files=False, deletion_threshold=0.0):
         self.seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
         self.masks = np.ascontiguousarray(np.asarray(list(masks) or [0], dtype=np.uint64))
         self.shifts = np.ascontiguousarray(np.asarray(list(shifts) or [0], dtype=np.uint8))
@@ -71,6 +71,8 @@ class _Args:
         o.sort_schedule = int(sort_schedule)
         o.chunk_mkeys = int(chunk_mkeys)
         o.host_tables = int(bool(host_tables))
+        o.write_raw_files = int(write_raw_files)
+        o.deletion_threshold = float(deletion_threshold)
 
 
 class BuildResult:
@@ -148,14 +150,17 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
                       spawn_rate4: float = 0.1, success_rate_dtype: str = "float32",
                       write_files: bool = True, keep: bool = False, keep_raw: bool = False,
                       device: int = 0, radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
-                      host_tables: bool = False) -> Optional[BuildResult]:
+                      host_tables: bool = False, write_raw_files: int = 0,
+                      deletion_threshold: float = 0.0) -> Optional[BuildResult]:
     """The native build call (reference formation_core.run_pattern_build,
     native_core/src/bindings_formation.cpp:220-229) with the PatternSpec and
     RunOptions fields spelled out.  ``keep=True`` returns the solved layers
-    resident in HBM as a BuildResult."""
+    resident in HBM as a BuildResult.  With files, an interrupted build in
+    the same folder is resumed: layers whose .book exists are not solved
+    again (BookSolver.cpp:476-511)."""
     a = _Args(arr_init, masks, shifts, symm_mode, target, steps, docheck_step, pathname, is_free,
               spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits, sort_schedule,
-              chunk_mkeys, host_tables)
+              chunk_mkeys, host_tables, write_raw_files, deletion_threshold)
     L = native.lib()
     seeds_p = a.seeds.ctypes.data_as(native.u64p)
     if keep or keep_raw:
@@ -209,7 +214,9 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
     _run_with_single_resume_retry(lambda: run_pattern_build(
         p["seeds"], p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"], p["docheck_step"],
         pathname, is_free=p["is_free"], spawn_rate4=float(cfg["4_spawn_rate"]), success_rate_dtype=dtype,
-        write_files=True, device=int(cfg.get("device", 0))))
+        write_files=True, device=int(cfg.get("device", 0)),
+        write_raw_files=bool(cfg.get("write_raw_files", False)),
+        deletion_threshold=float(cfg.get("deletion_threshold", 0.0))))
     return True
 
 

@@ -116,7 +116,7 @@ uint64_t compact_tiles(uint64_t n);
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
                     uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
-                    unsigned long long *max_bits, uint64_t *count_out);
+                    unsigned long long *max_bits, uint64_t *count_out, T threshold = T(0));
 
 template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);

@@ -877,7 +877,7 @@ constexpr int kCompItems = 8;
 constexpr int kCompTile = kCompThreads * kCompItems;
 
 template <typename T>
-__global__ __launch_bounds__(kCompThreads) void k_compact_count(const T *__restrict__ rates, uint64_t n,
+__global__ __launch_bounds__(kCompThreads) void k_compact_count(const T *__restrict__ rates, uint64_t n, T thr,
                                                                 uint64_t *__restrict__ counts) {
     __shared__ uint64_t scratch[kCompThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kCompTile;
@@ -885,7 +885,7 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_count(const T *__restr
 #pragma unroll
     for (int j = 0; j < kCompItems; ++j) {
         const uint64_t idx = base + (uint64_t)j * kCompThreads + threadIdx.x;
-        if (idx < n) c += rates[idx] > (T)0 ? 1u : 0u;
+        if (idx < n) c += rates[idx] > thr ? 1u : 0u;
     }
     c = wave_reduce_sum(c);
     if (lane_id() == 0) scratch[threadIdx.x >> 6] = c;
@@ -905,7 +905,7 @@ __device__ __forceinline__ uint64_t rate_bits(T v) {
 
 template <typename T>
 __global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *__restrict__ keys,
-                                                                const T *__restrict__ rates, uint64_t n,
+                                                                const T *__restrict__ rates, uint64_t n, T thr,
                                                                 const uint64_t *__restrict__ offsets,
                                                                 uint64_t *__restrict__ out_keys,
                                                                 T *__restrict__ out_rates,
@@ -920,7 +920,7 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *
     for (int j = 0; j < kCompItems; ++j) {
         const uint64_t idx = base + j;
         r[j] = idx < n ? rates[idx] : (T)0;
-        cnt += r[j] > (T)0;
+        cnt += r[j] > thr;
         mx = mx < r[j] ? r[j] : mx;
     }
     uint64_t total;
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *
     if (count_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count_out = offsets[blockIdx.x] + total;
 #pragma unroll
     for (int j = 0; j < kCompItems; ++j) {
-        if (r[j] > (T)0) {
+        if (r[j] > thr) {
             out_keys[pos] = keys[base + j];
             out_rates[pos] = r[j];
             ++pos;
@@ -1157,16 +1157,17 @@ uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
                     uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
-                    unsigned long long *max_bits, uint64_t *count_out) {
+                    unsigned long long *max_bits, uint64_t *count_out, T threshold) {
     if (n == 0) {
         if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
         return;
     }
     const uint32_t tiles = (uint32_t)compact_tiles(n);
-    hipLaunchKernelGGL((k_compact_count<T>), dim3(tiles), dim3(kCompThreads), 0, s, rates, n, tile_counts);
+    hipLaunchKernelGGL((k_compact_count<T>), dim3(tiles), dim3(kCompThreads), 0, s, rates, n, threshold,
+                       tile_counts);
     launch_scan_u64(s, tile_counts, tiles, nullptr, tile_offs, partial, tile_offs + tiles);
-    hipLaunchKernelGGL((k_compact_write<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, tile_offs,
-                       out_keys, out_rates, max_bits, count_out);
+    hipLaunchKernelGGL((k_compact_write<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, threshold,
+                       tile_offs, out_keys, out_rates, max_bits, count_out);
 }
 
 template <typename T>
@@ -1193,7 +1194,7 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *,           \
                                     uint64_t *, uint64_t *, uint64_t *, T *, unsigned long long *,            \
-                                    uint64_t *);                                                              \
+                                    uint64_t *, T);                                                           \
     template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
 
 EGTB_INSTANTIATE(float)

@@ -103,15 +103,19 @@ class Timer {
 
 // Bump allocator over large device chunks: layers are carved out in build
 // order, the newest allocation can be trimmed once its true size is known,
-// and everything is released at once.
+// and everything is released at once. With a stream set, chunks come from
+// the stream-ordered pool and the chunks above a consumed allocation can be
+// handed back while the stream still runs.
 class Arena {
   public:
     explicit Arena(size_t min_chunk = (size_t)1 << 30) : min_chunk_(min_chunk) {}
     Arena(const Arena &) = delete;
     Arena &operator=(const Arena &) = delete;
     ~Arena() { release(); }
+    void set_stream(hipStream_t s) { stream_ = s; }
     void *alloc(size_t bytes);
     void trim_last(void *p, size_t bytes);  // shrink the newest allocation
+    void release_after(const void *p);      // free every chunk newer than the one holding p
     void release();
     size_t used() const;
 
@@ -120,8 +124,10 @@ class Arena {
         char *p;
         size_t cap, used;
     };
+    void free_chunk(Chunk &c);
     std::vector<Chunk> chunks_;
     size_t min_chunk_;
+    hipStream_t stream_ = nullptr;
 };
 
 // Per-layer device record: sizes, key span and stats of layer i.
@@ -149,6 +155,25 @@ struct LayerInfo {
     uint64_t n_raw = 0, n_live = 0, c2 = 0, c4 = 0;
 };
 
+// Page-aligned host memory mapped on first write (no zero-fill pass), with
+// transparent huge pages requested so that filling it costs few page faults.
+struct HostArray {
+    char *p = nullptr;
+    size_t bytes = 0;
+    HostArray() = default;
+    HostArray(const HostArray &) = delete;
+    HostArray &operator=(const HostArray &) = delete;
+    HostArray(HostArray &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+    HostArray &operator=(HostArray &&o) noexcept {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        return *this;
+    }
+    ~HostArray() { reset(); }
+    void allocate(size_t n);
+    void reset();
+};
+
 struct Result {
     int steps = 0;
     int rate_size = 4;
@@ -165,8 +190,7 @@ struct Result {
     // no longer needed, then in host memory
     bool host_tables = false;
     std::vector<DBuf> book_bufs;  // 3 per layer: keys, rates, index
-    std::vector<std::vector<uint64_t>> host_keys;
-    std::vector<std::vector<char>> host_rates;
+    std::vector<HostArray> host_keys, host_rates;
     uint64_t streamed_bytes = 0, chunked_steps = 0, chunks = 0, peak_device_bytes = 0;
     double stream_ms = 0;
     LayerDev *meta_dev() const { return (LayerDev *)meta.p; }

@@ -15,6 +15,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 #include <condition_variable>
@@ -429,7 +430,7 @@ void *Arena::alloc(size_t bytes) {
     if (chunks_.empty() || chunks_.back().cap - chunks_.back().used < bytes) {
         size_t cap = bytes > min_chunk_ ? bytes : min_chunk_;
         void *p = nullptr;
-        hipError_t e = hipMalloc(&p, cap);
+        hipError_t e = stream_ ? hipMallocAsync(&p, cap, stream_) : hipMalloc(&p, cap);
         if (e != hipSuccess)
             throw Error(EGTB_ERR_MEMORY, "device allocation of " + std::to_string(cap) + " bytes failed: " +
                                              hipGetErrorString(e));
@@ -450,8 +451,28 @@ void Arena::trim_last(void *p, size_t bytes) {
     if (end < c.used) c.used = end;
 }
 
+void Arena::free_chunk(Chunk &c) {
+    if (stream_)
+        (void)hipFreeAsync(c.p, stream_);
+    else
+        (void)hipFree(c.p);
+}
+
+void Arena::release_after(const void *p) {
+    for (size_t k = 0; k < chunks_.size(); ++k) {
+        const Chunk &c = chunks_[k];
+        if ((const char *)p >= c.p && (const char *)p < c.p + c.cap) {
+            while (chunks_.size() > k + 1) {
+                free_chunk(chunks_.back());
+                chunks_.pop_back();
+            }
+            return;
+        }
+    }
+}
+
 void Arena::release() {
-    for (auto &c : chunks_) (void)hipFree(c.p);
+    for (auto &c : chunks_) free_chunk(c);
     chunks_.clear();
 }
 
@@ -516,18 +537,103 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     }
 }
 
+void HostArray::allocate(size_t n) {
+    reset();
+    if (!n) return;
+    void *q = mmap(nullptr, n, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+    if (q == MAP_FAILED) throw Error(EGTB_ERR_MEMORY, "host memory for a streamed layer (" + std::to_string(n) + " B)");
+    (void)madvise(q, n, MADV_HUGEPAGE);
+    p = (char *)q;
+    bytes = n;
+}
+
+void HostArray::reset() {
+    if (p) (void)munmap(p, bytes);
+    p = nullptr;
+    bytes = 0;
+}
+
+// memcpy split across a fixed set of worker threads plus the caller.
+class CopyPool {
+  public:
+    explicit CopyPool(int workers) {
+        for (int w = 0; w < workers; ++w) th_.emplace_back([this, w] { loop(w + 1); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void copy(char *dst, const char *src, size_t bytes) {
+        const int parts = (int)th_.size() + 1;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = dst, src_ = src, bytes_ = bytes, left_ = parts - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        part(0, parts);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return left_ == 0; });
+    }
+
+  private:
+    void part(int k, int parts) {
+        const size_t per = ((bytes_ + parts - 1) / parts + 4095) & ~(size_t)4095;
+        const size_t a = std::min(bytes_, per * (size_t)k), b = std::min(bytes_, a + per);
+        if (b > a) std::memcpy(dst_ + a, src_ + a, b - a);
+    }
+    void loop(int k) {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            part(k, (int)th_.size() + 1);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--left_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t bytes_ = 0;
+    int left_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
 // Host-tables mode: a copy thread moves each finished book (keys + rates)
-// to host memory through two pinned staging buffers on its own stream, after
-// the event that marks the book's last use, then frees its device buffers.
+// to host memory after the event that marks the book's last use, then frees
+// its device buffers. Device-to-host copies run on the thread's own stream
+// into a ring of pinned staging slots; the host drains slot k (a parallel
+// memcpy into the destination) while the copies into the later slots are in
+// flight.
 class BookStreamer {
   public:
-    BookStreamer(Result &res, int rate_size) : res_(res), rate_size_(rate_size) {
+    BookStreamer(Result &res, int rate_size) : res_(res), rate_size_(rate_size), pool_(kCopyWorkers) {
         HIPCHK(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking));
-        for (int k = 0; k < 2; ++k) HIPCHK(hipHostMalloc(&stage_[k], kStage, hipHostMallocDefault));
+        for (int k = 0; k < kSlots; ++k) {
+            HIPCHK(hipHostMalloc(&stage_[k], kStage, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&slot_ev_[k], hipEventDisableTiming));
+        }
         HIPCHK(hipHostMalloc((void **)&n_host_, 8, hipHostMallocDefault));
         th_ = std::thread([this] { run(); });
     }
-    ~BookStreamer() { finish(); }
+    ~BookStreamer() {
+        try {
+            finish();
+        } catch (...) {
+        }
+    }
 
     // book `layer`'s device buffers are read by work recorded before `ev`
     void retire(int layer, hipStream_t s) {
@@ -549,30 +655,49 @@ class BookStreamer {
         }
         cv_.notify_one();
         th_.join();
-        for (int k = 0; k < 2; ++k) (void)hipHostFree(stage_[k]);
+        for (int k = 0; k < kSlots; ++k) {
+            (void)hipHostFree(stage_[k]);
+            (void)hipEventDestroy(slot_ev_[k]);
+        }
         (void)hipHostFree(n_host_);
         (void)hipStreamDestroy(cs_);
         if (!err_.empty()) throw Error(EGTB_ERR_HIP, "streaming a layer to the host failed: " + err_);
     }
 
   private:
-    static constexpr size_t kStage = (size_t)64 << 20;
+    static constexpr size_t kStage = (size_t)32 << 20;
+    static constexpr int kSlots = 4;
+    static constexpr int kCopyWorkers = 7;
     struct Job {
         int layer;
         hipEvent_t ev;
     };
+    struct Piece {
+        const char *src;
+        char *dst;
+        size_t bytes;
+    };
 
-    void copy_out(const void *src, size_t bytes, char *dst) {
-        size_t off = 0;
-        int k = 0;
-        while (off < bytes) {
-            const size_t m = std::min(kStage, bytes - off);
-            HIPCHK(hipMemcpyAsync(stage_[k], (const char *)src + off, m, hipMemcpyDeviceToHost, cs_));
-            HIPCHK(hipStreamSynchronize(cs_));
-            std::memcpy(dst + off, stage_[k], m);
-            off += m;
-            k ^= 1;
+    void copy_out(const std::vector<Piece> &pieces) {
+        size_t issued = 0, done = 0;
+        while (done < pieces.size()) {
+            while (issued < pieces.size() && issued - done < (size_t)kSlots) {
+                const int k = (int)(issued % kSlots);
+                HIPCHK(hipMemcpyAsync(stage_[k], pieces[issued].src, pieces[issued].bytes, hipMemcpyDeviceToHost,
+                                      cs_));
+                HIPCHK(hipEventRecord(slot_ev_[k], cs_));
+                ++issued;
+            }
+            const int k = (int)(done % kSlots);
+            HIPCHK(hipEventSynchronize(slot_ev_[k]));
+            pool_.copy(pieces[done].dst, (const char *)stage_[k], pieces[done].bytes);
+            ++done;
         }
+    }
+
+    static void split(std::vector<Piece> &out, const void *src, char *dst, size_t bytes) {
+        for (size_t off = 0; off < bytes; off += kStage)
+            out.push_back({(const char *)src + off, dst + off, std::min(kStage, bytes - off)});
     }
 
     void run() {
@@ -593,18 +718,21 @@ class BookStreamer {
                 HIPCHK(hipMemcpyAsync(n_host_, bk.n_dev, 8, hipMemcpyDeviceToHost, cs_));
                 HIPCHK(hipStreamSynchronize(cs_));
                 const uint64_t n = *n_host_;
-                res_.host_keys[j.layer].resize(n);
-                res_.host_rates[j.layer].resize(n * (size_t)rate_size_);
-                copy_out(bk.keys, n * 8, (char *)res_.host_keys[j.layer].data());
-                copy_out(bk.rates, n * (size_t)rate_size_, res_.host_rates[j.layer].data());
-                res_.streamed_bytes += n * (8 + (uint64_t)rate_size_);
+                const size_t kb = n * 8, rb = n * (size_t)rate_size_;
+                res_.host_keys[j.layer].allocate(kb);
+                res_.host_rates[j.layer].allocate(rb);
+                std::vector<Piece> pieces;
+                split(pieces, bk.keys, res_.host_keys[j.layer].p, kb);
+                split(pieces, bk.rates, res_.host_rates[j.layer].p, rb);
+                copy_out(pieces);
+                res_.streamed_bytes += kb + rb;
                 for (int w = 0; w < 3; ++w) res_.book_bufs[(size_t)j.layer * 3 + w].release();
                 bk.keys = nullptr;
                 bk.rates = nullptr;
                 bk.idx = nullptr;
                 res_.stream_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             } catch (const std::exception &e) {
-                err_ = e.what();
+                if (err_.empty()) err_ = e.what();
             }
             (void)hipEventDestroy(j.ev);
         }
@@ -612,8 +740,10 @@ class BookStreamer {
 
     Result &res_;
     int rate_size_;
+    CopyPool pool_;
     hipStream_t cs_ = nullptr;
-    void *stage_[2] = {nullptr, nullptr};
+    void *stage_[kSlots] = {};
+    hipEvent_t slot_ev_[kSlots] = {};
     uint64_t *n_host_ = nullptr;
     std::thread th_;
     std::mutex mu_;
@@ -622,6 +752,184 @@ class BookStreamer {
     bool stop_ = false;
     std::string err_;
 };
+
+// Raw layer files <pathname><i> (N_i x u64, little endian; reference
+// write_binary_vector_direct, FileIOUtils.h:444-451) written by a thread of
+// their own while the forward pass goes on: device-to-host copies on a
+// private stream into pinned slots, fwrite of one slot while the next fills.
+// The layers are final (their step has synchronised) when they are queued.
+class RawFileWriter {
+  public:
+    explicit RawFileWriter(int dev) : dev_(dev) {
+        HIPCHK(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking));
+        for (int k = 0; k < kSlots; ++k) {
+            HIPCHK(hipHostMalloc(&slot_[k], kStage, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&ev_[k], hipEventDisableTiming));
+        }
+        th_ = std::thread([this] { run(); });
+    }
+    ~RawFileWriter() {
+        try {
+            finish();
+        } catch (...) {
+        }
+    }
+    void write(const std::string &path, const uint64_t *keys, uint64_t n) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            q_.push_back({path, keys, n});
+        }
+        cv_.notify_one();
+    }
+    // waits until every queued file is on disk
+    void finish() {
+        if (!th_.joinable()) return;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_one();
+        th_.join();
+        for (int k = 0; k < kSlots; ++k) {
+            (void)hipHostFree(slot_[k]);
+            (void)hipEventDestroy(ev_[k]);
+        }
+        (void)hipStreamDestroy(cs_);
+        if (!err_.empty()) throw Error(EGTB_ERR_IO, err_);
+    }
+
+  private:
+    static constexpr size_t kStage = (size_t)16 << 20;
+    static constexpr int kSlots = 3;
+    struct Job {
+        std::string path;
+        const uint64_t *keys;
+        uint64_t n;
+    };
+    void write_one(const Job &j) {
+        const std::string tmp = j.path + ".tmp";
+        FILE *f = std::fopen(tmp.c_str(), "wb");
+        if (!f) throw Error(EGTB_ERR_IO, "cannot open " + tmp + " for writing");
+        const size_t bytes = j.n * 8, pieces = (bytes + kStage - 1) / kStage;
+        bool ok = true;
+        size_t issued = 0, done = 0;
+        try {
+            while (done < pieces) {
+                while (issued < pieces && issued - done < (size_t)kSlots) {
+                    const int k = (int)(issued % kSlots);
+                    const size_t off = issued * kStage, m = std::min(kStage, bytes - off);
+                    HIPCHK(hipMemcpyAsync(slot_[k], (const char *)j.keys + off, m, hipMemcpyDeviceToHost, cs_));
+                    HIPCHK(hipEventRecord(ev_[k], cs_));
+                    ++issued;
+                }
+                const int k = (int)(done % kSlots);
+                const size_t off = done * kStage, m = std::min(kStage, bytes - off);
+                HIPCHK(hipEventSynchronize(ev_[k]));
+                ok = ok && std::fwrite(slot_[k], 1, m, f) == m;
+                ++done;
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(cs_);
+            std::fclose(f);
+            std::remove(tmp.c_str());
+            throw;
+        }
+        ok = ok && std::fflush(f) == 0;
+        std::fclose(f);
+        if (!ok) {
+            std::remove(tmp.c_str());
+            throw Error(EGTB_ERR_IO, "short write to " + tmp);
+        }
+        if (std::rename(tmp.c_str(), j.path.c_str()) != 0) throw Error(EGTB_ERR_IO, "cannot rename " + tmp);
+    }
+    void run() {
+        (void)hipSetDevice(dev_);
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                j = std::move(q_.front());
+                q_.pop_front();
+            }
+            if (!err_.empty()) continue;  // after a failure the remaining files are dropped
+            try {
+                write_one(j);
+            } catch (const std::exception &e) {
+                err_ = e.what();
+            }
+        }
+    }
+    int dev_;
+    hipStream_t cs_ = nullptr;
+    void *slot_[kSlots] = {};
+    hipEvent_t ev_[kSlots] = {};
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Job> q_;
+    bool stop_ = false;
+    std::string err_;
+};
+
+// Restart-file checks (reference is_valid_restart_file /
+// remove_invalid_restart_file, BookGenerator.cpp:71-99): a file counts when
+// it is a non-empty regular file whose size is a multiple of the record
+// size; one that does not is removed.
+static bool valid_restart_file(const std::string &path, uint64_t align) {
+    struct stat st;
+    if (::stat(path.c_str(), &st) != 0 || !S_ISREG(st.st_mode) || st.st_size <= 0) return false;
+    return align == 0 || (uint64_t)st.st_size % align == 0;
+}
+
+static void drop_invalid_restart_file(const std::string &path, uint64_t align) {
+    struct stat st;
+    if (::stat(path.c_str(), &st) != 0 || valid_restart_file(path, align)) return;
+    if (std::remove(path.c_str()) != 0) throw Error(EGTB_ERR_IO, "failed to remove invalid restart file: " + path);
+}
+
+static void remove_if_exists(const std::string &path) {
+    if (file_exists(path) && std::remove(path.c_str()) != 0) throw Error(EGTB_ERR_IO, "cannot remove " + path);
+}
+
+// Resume: load a solved layer from its .book file into a device book
+// (reference read_split_layer_file on restart, BookSolver.cpp:500-507).
+template <typename T>
+static void load_book_file(hipStream_t s, Result &res, int layer, const std::string &path, HostStage &stage,
+                           DBuf &packed, LayerDev *H) {
+    constexpr size_t rec = 8 + sizeof(T);
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) throw Error(EGTB_ERR_IO, "cannot open " + path);
+    std::fseek(f, 0, SEEK_END);
+    const long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (sz < 0 || (size_t)sz % rec) {
+        std::fclose(f);
+        throw Error(EGTB_ERR_IO, path + ": size is not a multiple of the record size");
+    }
+    const uint64_t n = (uint64_t)sz / rec;
+    stage.ensure((size_t)sz + 16);
+    const size_t got = n ? std::fread(stage.p, 1, (size_t)sz, f) : 0;
+    std::fclose(f);
+    if (got != (size_t)sz) throw Error(EGTB_ERR_IO, "short read of " + path);
+    Book &bk = res.books[layer];
+    LayerDev *md = res.meta_dev() + layer;
+    bk.n_dev = &md->n_live;
+    bk.keys = (uint64_t *)res.book_arena.alloc(n * 8 + 8);
+    bk.rates = res.book_arena.alloc(n * sizeof(T) + 8);
+    bk.bits = index_bits_for(n);
+    bk.idx = (uint32_t *)res.book_arena.alloc(((1ull << bk.bits) + 1) * 4);
+    H[layer].n_live = n;
+    HIPCHK(hipMemcpyAsync(&md->n_live, &H[layer].n_live, 8, hipMemcpyHostToDevice, s));
+    if (n) {
+        packed.ensure((size_t)sz, s);
+        HIPCHK(hipMemcpyAsync(packed.p, stage.p, (size_t)sz, hipMemcpyHostToDevice, s));
+        launch_unpack_records(s, (const uint32_t *)packed.p, n, (int)sizeof(T), bk.keys, bk.rates);
+    }
+    launch_bucket_index(s, bk.keys, n, bk.n_dev, bk.bits, bk.idx);
+    HIPCHK(hipStreamSynchronize(s));  // the staging buffers are reused
+}
 
 template <typename T>
 static void write_book_file(hipStream_t s, const std::string &path, const Book &bk, uint64_t n, HostStage &stage,
@@ -682,6 +990,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     res.raw.assign(steps, nullptr);
     res.books.assign(steps, Book{});
     res.info.assign(steps, LayerInfo{});
+    if (o.host_tables) res.raw_arena.set_stream(s);
     SortWork w;
     HostStage stage, meta_host_buf;
     DBuf cand[2], sort_tmp, counts, offs, rates, packed;
@@ -698,6 +1007,23 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     H[0].n_raw = n_seeds;
     res.meta.alloc((size_t)(steps + 1) * sizeof(LayerDev), s);
     HIPCHK(hipMemcpyAsync(res.meta.p, H, (size_t)(steps + 1) * sizeof(LayerDev), hipMemcpyHostToDevice, s));
+
+    // resume (reference handle_restart / handle_restart_recalculate): drop
+    // invalid leftovers, note which layers are already solved on disk
+    std::vector<char> have_book(steps, 0);
+    std::unique_ptr<RawFileWriter> raw_writer;
+    if (files) {
+        for (int i = 0; i < steps; ++i) {
+            const std::string bp = prefix + std::to_string(i) + ".book";
+            drop_invalid_restart_file(bp, 8 + sizeof(T));
+            drop_invalid_restart_file(prefix + std::to_string(i), 8);
+            have_book[i] = valid_restart_file(bp, 8 + sizeof(T)) ? 1 : 0;
+        }
+        if (o.write_raw_files) raw_writer = std::make_unique<RawFileWriter>(o.device);
+    }
+    auto queue_raw = [&](int L, uint64_t n) {
+        if (raw_writer && !have_book[L]) raw_writer->write(prefix + std::to_string(L), res.raw[L], n);
+    };
     LayerDev *M = res.meta_dev();
 
     // ---- forward (reference generate_process, BookGenerator.cpp:963-1121)
@@ -797,6 +1123,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         const uint64_t n_par = H[i - 1].n_raw;
         if (i > 1) res.raw_arena.trim_last(res.raw[i - 1], n_par * 8);
         res.info[i - 1].n_raw = n_par;
+        queue_raw(i - 1, n_par);
         if (n_par * 64ull >= (1ull << 32))
             throw Error(EGTB_ERR_ARG, "layer " + std::to_string(i - 1) + " has too many states for one step");
         const uint64_t c2 = H[i].tot & 0xFFFFFFFFull, c4 = H[i].tot >> 32;
@@ -900,6 +1227,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         read_meta(s, res, H, L - 1, L);
         res.raw_arena.trim_last(res.raw[L - 1], H[L - 1].n_raw * 8);
         res.info[L - 1].n_raw = H[L - 1].n_raw;
+        queue_raw(L - 1, H[L - 1].n_raw);
         if (!chunked) {
             const uint64_t n_c = c4_prev;
             DBuf &cur = cand[L & 1];
@@ -935,8 +1263,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     if (res.host_tables) {
         if (files) throw Error(EGTB_ERR_ARG, "host_tables and write_files cannot be combined");
         res.book_bufs.resize((size_t)steps * 3);
-        res.host_keys.assign(steps, {});
-        res.host_rates.assign(steps, {});
+        res.host_keys.clear();
+        res.host_rates.clear();
+        res.host_keys.resize(steps);
+        res.host_rates.resize(steps);
         streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T));
     }
     uint64_t max_n = 0;
@@ -947,10 +1277,14 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         launch_final_rates<T>(s, res.raw[i], n, p, max_scale, (T *)rates.p);
         make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, true);
     }
+    if (raw_writer) raw_writer->finish();
     if (files) {
         read_meta(s, res, H, steps - 2, steps - 1);
         for (int i = steps - 2; i <= steps - 1; ++i)
-            write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], H[i].n_live, stage, packed);
+            if (!have_book[i])
+                write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], H[i].n_live, stage,
+                                   packed);
+        if (o.write_raw_files != 2) remove_if_exists(prefix + std::to_string(steps - 2));
         if (!file_exists(prefix + "stats.txt")) {
             FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
             if (!f) throw Error(EGTB_ERR_IO, "cannot open " + prefix + "stats.txt");
@@ -960,8 +1294,17 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     }
 
     // ---- backward (reference recalculate_process_impl, BookSolver.cpp:513-605)
+    bool started = false;
+    DBuf thr_keys, thr_rates, thr_cnt;
     for (int i = steps - 3; i >= 0; --i) {
         progress_set(prog_total - (uint32_t)i - 2u, prog_total);
+        if (files && !started && have_book[i]) continue;  // solved by an earlier run
+        if (!started) {
+            started = true;
+            if (files && i != steps - 3)
+                for (int j = i + 1; j <= i + 2; ++j)
+                    load_book_file<T>(s, res, j, prefix + std::to_string(j) + ".book", stage, packed, H);
+        }
         const auto l0 = std::chrono::steady_clock::now();
         const uint64_t n = res.info[i].n_raw;
         const Book &b1 = res.books[i + 1], &b2 = res.books[i + 2];
@@ -973,8 +1316,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0);
         HIPCHK(hipGetLastError());
         if (streamer) {
-            // book i+2 was last read by this layer's solve
+            // book i+2 was last read by this layer's solve; raw layers above
+            // i are consumed
             streamer->retire(i + 2, s);
+            if (!o.keep_raw) res.raw_arena.release_after(res.raw[i]);
             size_t fr = 0, tot = 0;
             if (hipMemGetInfo(&fr, &tot) == hipSuccess) res.peak_device_bytes = std::max<uint64_t>(res.peak_device_bytes, tot - fr);
         }
@@ -985,8 +1330,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 const double dt = std::chrono::duration<double>(l1 - l0).count();
                 char row[256];
                 std::snprintf(row, sizeof row, "%d,%llu,%g,%g mbps,%g,%s\n", i, (unsigned long long)n,
-                              rate_unit<T>(H[i].maxbits), round_to_2((double)n / std::max(dt, 0.01) / 1e6), 0.0,
-                              now_string().c_str());
+                              rate_unit<T>(H[i].maxbits), round_to_2((double)n / std::max(dt, 0.01) / 1e6),
+                              o.deletion_threshold, now_string().c_str());
                 FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
                 if (f) {
                     std::fputs(row, f);
@@ -994,12 +1339,44 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 }
             }
             write_book_file<T>(s, prefix + std::to_string(i) + ".book", res.books[i], H[i].n_live, stage, packed);
+            if (o.write_raw_files != 2) remove_if_exists(prefix + std::to_string(i));
+            if (o.deletion_threshold > 0.0) {
+                // book i+2 has had its last reader: its file keeps rates > threshold
+                // (reference BookSolver.cpp:584-588)
+                const int j = i + 2;
+                const uint64_t nj = H[j].n_live;
+                const T thr = static_cast<T>(o.deletion_threshold * (double)max_scale);
+                thr_keys.ensure(nj * 8 + 8, s);
+                thr_rates.ensure(nj * sizeof(T) + 8, s);
+                thr_cnt.ensure(16, s);
+                const uint64_t tiles = compact_tiles(nj ? nj : 1);
+                w.tile_counts.ensure(tiles * 8, s);
+                w.tile_offs.ensure((tiles + 1) * 8, s);
+                w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
+                launch_compact<T>(s, res.books[j].keys, (const T *)res.books[j].rates, nj,
+                                  (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p, (uint64_t *)w.partial.p,
+                                  (uint64_t *)thr_keys.p, (T *)thr_rates.p, (unsigned long long *)thr_cnt.p + 1,
+                                  (uint64_t *)thr_cnt.p, thr);
+                uint64_t kept = 0;
+                HIPCHK(hipMemcpyAsync(&H[steps].n_live, thr_cnt.p, 8, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                kept = H[steps].n_live;
+                Book tb;
+                tb.keys = (uint64_t *)thr_keys.p;
+                tb.rates = thr_rates.p;
+                write_book_file<T>(s, prefix + std::to_string(j) + ".book", tb, kept, stage, packed);
+            }
         }
     }
     if (streamer) {
         streamer->retire(1, s);
         streamer->retire(0, s);
         streamer->finish();
+        // hand the pooled chunks of this build back to the device
+        hipMemPool_t pool;
+        if (!o.keep_raw) res.raw_arena.release();
+        HIPCHK(hipStreamSynchronize(s));
+        if (hipDeviceGetDefaultMemPool(&pool, res.dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
     }
     if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     read_meta(s, res, H, 0, steps - 1);
@@ -1140,8 +1517,8 @@ int egtb_result_book(const egtb_result *r, int layer, uint64_t *keys, void *rate
         if (!keys && !rates) return;
         if (cap < b.n) throw Error(EGTB_ERR_ARG, "buffer too small");
         if (r->host_tables) {
-            if (b.n && keys) std::memcpy(keys, r->host_keys[layer].data(), b.n * 8);
-            if (b.n && rates) std::memcpy(rates, r->host_rates[layer].data(), b.n * r->rate_size);
+            if (b.n && keys) std::memcpy(keys, r->host_keys[layer].p, b.n * 8);
+            if (b.n && rates) std::memcpy(rates, r->host_rates[layer].p, b.n * r->rate_size);
             return;
         }
         if (b.n && keys) HIPCHK(hipMemcpy(keys, b.keys, b.n * 8, hipMemcpyDeviceToHost));

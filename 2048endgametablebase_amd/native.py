@@ -54,7 +54,9 @@ class Options(ctypes.Structure):
         ("sort_schedule", ctypes.c_int32),
         ("chunk_mkeys", ctypes.c_int32),
         ("host_tables", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("write_raw_files", ctypes.c_int32),
+        ("deletion_threshold", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -63,11 +63,22 @@ typedef struct egtb_options {
     int32_t sort_schedule;           /* 0 = autotuned, 1 = per-digit 3 kernels, 2 = single-pass look-back */
     int32_t chunk_mkeys;             /* candidate budget per chunk in Mi keys; 0 = from free HBM */
     int32_t host_tables;             /* 1: stream finished layers to host memory, free them in HBM */
-    int32_t reserved[4];
+    int32_t write_raw_files;         /* 1: also write raw layers <pathname><i> (N_i x u64 LE) as the
+                                        forward pass finishes them; each is removed once solved;
+                                        2: write them and keep them */
+    double deletion_threshold;       /* RunOptions::deletion_threshold: after layer i is solved, book
+                                        i+2 is rewritten with rates > threshold only (0 = off) */
+    int32_t reserved[2];
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file
  * set (<pathname><i>.book for every layer, <pathname>stats.txt).
+ * Resumes like the reference (BookGenerator.cpp:384-430,
+ * BookSolver.cpp:476-511): empty or misaligned .book files are removed,
+ * solved layers whose .book exists are skipped from the top down, the first
+ * layer solved loads books i+1 and i+2 from their files, and leftover raw
+ * layer files are removed as their layers are solved.  The forward pass is
+ * recomputed on the device rather than read back from raw files.
  * Replaces formation_core.run_pattern_build
  * (reference native_core/src/bindings_formation.cpp:220-229 ->
  * run_pattern_build_cpp, BookSolver.cpp:823-832). */

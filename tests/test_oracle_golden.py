@@ -5,9 +5,10 @@ import pytest
 
 from conftest import book_sha, golden_params, golden_tags, load_golden
 
-# cases the oracle finishes in seconds on the CPU
-CPU_CASES = [t for t in ["442_128", "LL_128", "444_128", "4431_128", "4432_128", "4441_128", "free8_128"]
-             if t in golden_tags()]
+# cases the oracle finishes in seconds on the CPU (2e7-2.5e7 states); the
+# larger goldens and the other symmetry modes are checked against the HIP
+# build, which the oracle is compared with at targets 64-128 on the GPU
+CPU_CASES = [t for t in ["442_128", "t_64"] if t in golden_tags()]
 
 
 @pytest.mark.parametrize("tag", CPU_CASES)
