@@ -10,6 +10,7 @@ failure.
 from __future__ import annotations
 
 import ctypes
+from itertools import combinations
 from typing import Iterable, Optional, Sequence
 
 import numpy as np
@@ -200,6 +201,70 @@ def _run_with_single_resume_retry(build_fn) -> None:
             retried = True
 
 
+def board_moves(boards, symm_mode: int = 0, device: int = 0):
+    """(moves, canonical) of many boards on the device: moves is n x 4 in
+    move_all_dir order (left, right, up, down), canonical the boards'
+    canonical forms under ``symm_mode`` (egtb_board_moves)."""
+    b = np.ascontiguousarray(np.asarray(boards, dtype=np.uint64).ravel())
+    out4 = np.empty((b.size, 4), dtype=np.uint64)
+    canon = np.empty(b.size, dtype=np.uint64)
+    native.check(native.lib().egtb_board_moves(b.ctypes.data_as(native.u64p), b.size, int(symm_mode),
+                                               out4.ctypes.data_as(native.u64p),
+                                               canon.ctypes.data_as(native.u64p), int(device)))
+    return out4, canon
+
+
+def generate_free_inits(t32ks: int, t2s: int, device: int = 0) -> np.ndarray:
+    """Layer 0 of a free* pattern (engine_core/BookBuilder.py:146-181):
+    every board with ``t32ks`` 32768-tiles and ``t2s`` 2-tiles (at most 1e6,
+    enumerated in the same combination order), deduplicated, then each
+    board's four moves (an impossible move keeps the board) that are their
+    own full-symmetry canonical form, stopping after the board that brings
+    the count within 3 of the number of boards; sorted and unique."""
+    max_estimated = 1_000_000
+    generated = []
+    for pos32 in combinations(range(16), t32ks):
+        base = 0
+        for pos in pos32:
+            base |= 15 << (4 * pos)
+        remain = sorted(set(range(16)) - set(pos32))
+        for pos2 in combinations(remain, t2s):
+            if len(generated) >= max_estimated:
+                break
+            board = base
+            for pos in pos2:
+                board |= 1 << (4 * pos)
+            generated.append(board)
+    gen = np.unique(np.asarray(generated, dtype=np.uint64))
+    if gen.size == 0:
+        return gen
+    moved, _ = board_moves(gen, 0, device)
+    _, canon = board_moves(moved.ravel(), 1, device)
+    keep = (moved.ravel() == canon).reshape(moved.shape)
+    per_board = keep.sum(axis=1)
+    # the reference's capacity stop: after the first board whose running
+    # count reaches len(gen) - 3
+    running = np.cumsum(per_board)
+    stop = np.flatnonzero(running >= gen.size - 3)
+    last = int(stop[0]) if stop.size else gen.size - 1
+    return np.unique(moved[: last + 1][keep[: last + 1]])
+
+
+def resolve_seeds(pattern: str, p: dict, device: int = 0) -> np.ndarray:
+    """Layer 0 as start_build computes it (engine_core/BookBuilder.py:
+    272-281): free* patterns whose small tiles fit the free cells as 2s get
+    generate_free_inits; every other pattern keeps its catalogue seeds."""
+    seeds = np.asarray(p["seeds"], dtype=np.uint64)
+    if not pattern.startswith("free"):
+        return seeds
+    seed0 = int(seeds[0])
+    num_32k = sum(1 for k in range(16) if (seed0 >> (4 * k)) & 0xF == 0xF)
+    extra_tile_sum = int(p["tile_sum"]) - 32768 * num_32k
+    if extra_tile_sum <= (15 - num_32k) * 2:
+        return generate_free_inits(num_32k, extra_tile_sum // 2, device)
+    return seeds
+
+
 def start_build(pattern: str, target: int, pathname: str) -> bool:
     """Build ``pattern`` at tile 2**target into files prefixed ``pathname``
     (engine_core/BookBuilder.py:264-317, classic path)."""
@@ -211,8 +276,9 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
     p = build_params(pattern, target)
     save_config_to_txt(pathname + "config.txt")
     dtype = str(cfg.get("success_rate_dtype", "uint32"))
+    seeds = resolve_seeds(pattern, p, int(cfg.get("device", 0)))
     _run_with_single_resume_retry(lambda: run_pattern_build(
-        p["seeds"], p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"], p["docheck_step"],
+        seeds, p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"], p["docheck_step"],
         pathname, is_free=p["is_free"], spawn_rate4=float(cfg["4_spawn_rate"]), success_rate_dtype=dtype,
         write_files=True, device=int(cfg.get("device", 0)),
         write_raw_files=bool(cfg.get("write_raw_files", False)),
@@ -224,9 +290,10 @@ def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "flo
                     spawn_rate4: float = 0.1, keep_raw: bool = False, device: int = 0,
                     radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
                     host_tables: bool = False) -> BuildResult:
-    """Build a catalogue pattern and keep every solved layer in HBM."""
+    """Build a catalogue pattern (layer 0 as start_build makes it) and keep
+    every solved layer in HBM."""
     p = build_params(pattern, target)
-    return run_pattern_build(p["seeds"], p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"],
+    return run_pattern_build(resolve_seeds(pattern, p, device), p["masks"], p["shifts"], p["symm_mode"], p["target"], p["steps"],
                              p["docheck_step"], "", is_free=p["is_free"], spawn_rate4=spawn_rate4,
                              success_rate_dtype=success_rate_dtype, write_files=False, keep=True,
                              keep_raw=keep_raw, device=device, radix_bits=radix_bits,

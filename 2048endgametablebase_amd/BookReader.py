@@ -30,6 +30,8 @@ from .config import SingletonConfig
 from .patterns import pattern_catalog
 
 _ORDERED_KEYS = ("down", "right", "left", "up")   # kOrderedResultKeys
+# readers that keep the best op's answer (ReaderRuntime.cpp:1053)
+_PREFER_MAX_PATTERNS = ("4442ff", "4442f", "4tiler")
 _ABS_NAMES = ("left", "right", "up", "down")     # lookup output order
 _DIR_NAMES = ("up", "right", "down", "left")     # kDirectionNames
 
@@ -101,7 +103,7 @@ class Table:
 
     @classmethod
     def open(cls, prefix: str, pattern: str, target: int, dtype: str, steps: Optional[int] = None,
-             spawn_rate4: float = 0.1, device: int = 0) -> "Table":
+             spawn_rate4: float = 0.1, device: int = 0, prefer_max: Optional[bool] = None) -> "Table":
         meta = pattern_catalog[pattern]
         if steps is None:
             folder, base = os.path.split(prefix)
@@ -117,7 +119,12 @@ class Table:
         native.check(native.lib().egtb_table_open(prefix.encode(), int(steps), ctypes.byref(pat), int(target),
                                                   int(meta.tile_sum), dtype.encode(), float(spawn_rate4),
                                                   int(device), ctypes.byref(h)))
-        return cls(h.value, dtype)
+        t = cls(h.value, dtype)
+        if prefer_max is None:
+            prefer_max = pattern in _PREFER_MAX_PATTERNS
+        if prefer_max:
+            t.set_prefer_max(True)
+        return t
 
     @classmethod
     def from_result(cls, result, pattern: str, target: int, spawn_rate4: float = 0.1) -> "Table":
@@ -133,6 +140,11 @@ class Table:
         t = cls(h.value, result.dtype)
         t._result = result  # keep the build alive
         return t
+
+    def set_prefer_max(self, on: bool = True):
+        """Answer from the symmetry op with the largest best rate
+        (egtb_table_set_prefer_max)."""
+        native.check(native.lib().egtb_table_set_prefer_max(self._h, int(bool(on))))
 
     @property
     def handle(self):
@@ -181,6 +193,70 @@ class Table:
             pass
 
 
+# record layout and raw-to-rate scale per dtype (FormationRuntime.h:197-200,
+# ReaderRuntime.cpp:270-277: integer rates are divided by their max scale)
+_REC = {
+    "float32": (np.dtype([("board", "<u8"), ("rate", "<f4")]), 1.0),
+    "float64": (np.dtype([("board", "<u8"), ("rate", "<f8")]), 1.0),
+    "uint32": (np.dtype([("board", "<u8"), ("rate", "<u4")]), 4e9),
+    "uint64": (np.dtype([("board", "<u8"), ("rate", "<u8")]), 1.6e18),
+}
+
+
+def _record_layout(dtype: str):
+    try:
+        return _REC[dtype]
+    except KeyError:
+        raise ValueError(f"unsupported success_rate_dtype {dtype!r}") from None
+
+
+def find_classic_value(pathname: str, filename: str, search_key: int, success_rate_dtype: str = "uint32"):
+    """One on-disk probe of a .book file (formation_core.find_classic_value,
+    bindings_formation.cpp:181-195 -> find_success_entry_in_file,
+    ReaderRuntime.cpp:279-316).  None when the file cannot be opened; the
+    key's rate when found (integer dtypes scaled to [0, 1]); 0.0 otherwise.
+    The bisection is the reference's own (inclusive bounds, lower mid), so
+    answers agree also on layer 0, which is stored in seed order."""
+    rec, scale = _record_layout(success_rate_dtype)
+    path = os.path.join(pathname, filename)
+    try:
+        size = os.path.getsize(path)
+        if not os.path.isfile(path):
+            return None
+    except OSError:
+        return None
+    n = size // rec.itemsize
+    if n == 0:
+        return 0.0
+    recs = np.memmap(path, dtype=rec, mode="r", shape=(n,))
+    key = int(search_key) & 0xFFFFFFFFFFFFFFFF
+    left, right = 0, n - 1
+    while left <= right:
+        mid = (left + right) // 2
+        board = int(recs[mid]["board"])
+        if board == key:
+            v = float(recs[mid]["rate"])
+            return v / scale if scale > 1.0 else v
+        if board < key:
+            left = mid + 1
+        else:
+            if mid == 0:
+                break
+            right = mid - 1
+    return 0.0
+
+
+def spawn_random_tile(board: int, spawn_rate4: float, rng) -> int:
+    """A 2 (or a 4 with probability spawn_rate4) on a uniformly chosen empty
+    cell (gen_new_num, CommonMover.h:59-67); a full board is returned as is."""
+    empty = [k for k in range(16) if (board >> (4 * k)) & 0xF == 0]
+    if not empty:
+        return board
+    pos = empty[int(rng.integers(len(empty)))]
+    val = 2 if rng.random() < spawn_rate4 else 1
+    return board | (val << (4 * pos))
+
+
 class BookReader:
     """engine_core/BookReader.py: move_on_dic / get_random_state."""
 
@@ -225,6 +301,35 @@ class BookReader:
             if res[1]:
                 return res
         return blank_entries(), ""
+
+    @classmethod
+    def get_random_state(cls, path_list: list, pattern_full: str, rng=None) -> np.uint64:
+        """A random stored state of one of the books 0..9 with one tile
+        spawned on it (engine_core/BookReader.py:87-99 ->
+        sample_classic_book_state, ReaderRuntime.cpp:944-990): per folder,
+        books are tried in random order, a record is drawn uniformly, and
+        the first non-zero board is returned after gen_new_num.  0 when no
+        book qualifies or the pattern is unknown."""
+        if pattern_full.split("_", 1)[0] not in pattern_catalog:
+            return np.uint64(0)
+        rng = rng if rng is not None else np.random.default_rng()
+        p4 = float(SingletonConfig().config["4_spawn_rate"])
+        for folder, dtype in path_list:
+            rec, _ = _record_layout(dtype)
+            books = list(range(10))
+            while books:
+                book_id = books.pop(int(rng.integers(len(books))))
+                path = os.path.join(folder, f"{pattern_full}_{book_id}.book")
+                if not os.path.isfile(path):
+                    continue
+                n = os.path.getsize(path) // rec.itemsize
+                if n == 0:
+                    continue
+                r = int(rng.integers(n))
+                state = int(np.memmap(path, dtype=rec, mode="r", shape=(n,))[r]["board"])
+                if state != 0:
+                    return np.uint64(spawn_random_tile(state, p4, rng))
+        return np.uint64(0)
 
     @classmethod
     def clear(cls):

@@ -9,7 +9,7 @@ package name starts with a digit).  Submodules:
 * ``native``      -- ctypes binding of the C ABI (include/egtb.h)
 """
 from . import config, native, patterns  # noqa: F401
-from .BookReader import BookReader, Table  # noqa: F401
+from .BookReader import BookReader, Table, find_classic_value  # noqa: F401
 from .BookBuilder import (  # noqa: F401
     BuildResult,
     build_in_memory,

@@ -19,6 +19,9 @@ struct TableSet {
     bool is_float = true;
     double scale = 1.0;  // raw rate / scale = probability
     double p4 = 0.1;
+    // evaluate every symmetry op and keep the one whose best rate is largest
+    // (> 0); reference prefer_max_result_ (ReaderRuntime.cpp:827-832,1053)
+    int prefer_max = 0;
     std::vector<uint64_t> n;  // records per layer
     Arena storage{(size_t)256 << 20};
     DBuf tabs;  // TableDev<T>[steps] on the device

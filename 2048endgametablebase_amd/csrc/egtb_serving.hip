@@ -89,7 +89,8 @@ template <typename T, typename OutT>
 __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ boards, uint64_t n,
                                                  const TableDev<T> *__restrict__ tabs, int steps,
                                                  int64_t nums_adjust, PatternDev p, double scale, double p4,
-                                                 const uint16_t *__restrict__ glut, OutT *__restrict__ out4,
+                                                 int prefer_max, int round9, const uint16_t *__restrict__ glut,
+                                                 OutT *__restrict__ out4,
                                                  int32_t *__restrict__ status, int32_t *__restrict__ op_out) {
     extern __shared__ uint16_t lut[];
     stage_lut(lut, glut);
@@ -102,22 +103,29 @@ __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ bo
         const int64_t nums = ((int64_t)tile_sum(b) + nums_adjust) / 2;
         if (nums >= 0 && nums < steps && tabs[nums].present) {
             const TableDev<T> &t = tabs[nums];
-            for (int op = 0; op < 8 && used < 0; ++op) {
+            double best = 0.0;
+            for (int op = 0; op < 8 && (prefer_max || used < 0); ++op) {
                 const uint64_t x = apply_op(b, op);
                 if (!is_pattern(x, p)) continue;
                 uint64_t mv[4];
                 move4(x, lut, mv);
                 bool any = false;
+                double top = -1.0;
                 OutT r[4] = {nanv, nanv, nanv, nanv};
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     if (mv[d] != x && is_pattern(mv[d], p)) {
                         const T raw = table_lookup_dev(t, canonical(mv[d], p.symm));
-                        r[c_absdir[op][d]] = (OutT)((double)raw / scale);
+                        const double val = (double)raw / scale;
+                        r[c_absdir[op][d]] = (OutT)val;
+                        // compared as the reader reports them: 4-byte rates
+                        // rounded to 1e-9 (maybe_round_value, ReaderRuntime.cpp:208-214)
+                        top = fmax(top, (round9 && fabs(val) > 1e-7) ? round(val * 1e9) / 1e9 : val);
                         any = true;
                     }
                 }
-                if (any) {
+                if (prefer_max ? (any && top > best) : any) {
+                    best = top;
 #pragma unroll
                     for (int d = 0; d < 4; ++d) v[d] = r[d];
                     used = op;
@@ -138,6 +146,23 @@ __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ bo
         if (status) status[q] = st;
         if (op_out) op_out[q] = used;
     }
+}
+
+// Moves and symmetry reduction of a batch of boards: host-side preparation
+// (free-pattern seeds) and tests use the same device code as the build.
+__global__ __launch_bounds__(256) void k_board_ops(const uint64_t *__restrict__ boards, uint64_t n,
+                                                   const uint16_t *__restrict__ lut, uint32_t symm,
+                                                   uint64_t *__restrict__ out4, uint64_t *__restrict__ canon) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t b = boards[i];
+    if (out4) {
+        uint64_t m[4];
+        move4(b, lut, m);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) out4[i * 4 + d] = m[d];
+    }
+    if (canon) canon[i] = canonical(b, symm);
 }
 
 void configure_serving_kernels() {
@@ -167,7 +192,7 @@ static void launch_lookup_t(hipStream_t s, const TableSet &ts, const DeviceCtx &
     const uint32_t grid = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((k_lookup<T, OutT>), dim3(grid ? grid : 1), dim3(threads), kLutEntries * sizeof(uint16_t), s,
                        boards, n, (const TableDev<T> *)ts.tabs.p, ts.steps, ts.nums_adjust, ts.pattern, ts.scale,
-                       ts.p4, C.lut, out4, status, op_out);
+                       ts.p4, ts.prefer_max, ts.rate_size == 4 ? 1 : 0, C.lut, out4, status, op_out);
 }
 
 template <typename OutT>
@@ -424,6 +449,34 @@ int egtb_lookup_batch_f64(egtb_table *t, const uint64_t *boards, size_t n, doubl
         HIPCHK(hipMemcpyAsync(out4, o.p, n * 32, hipMemcpyDeviceToHost, s));
         if (status) HIPCHK(hipMemcpyAsync(status, st.p, n * 4, hipMemcpyDeviceToHost, s));
         if (op_used) HIPCHK(hipMemcpyAsync(op_used, op.p, n * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int egtb_table_set_prefer_max(egtb_table *t, int32_t on) {
+    if (!t) return EGTB_ERR_ARG;
+    t->prefer_max = on ? 1 : 0;
+    return EGTB_OK;
+}
+
+int egtb_board_moves(const uint64_t *boards, size_t n, int32_t symm_mode, uint64_t *out4, uint64_t *canon,
+                     int device) {
+    return guarded_s([&] {
+        if (n && !boards) throw Error(EGTB_ERR_ARG, "boards is NULL");
+        if (symm_mode < 0 || symm_mode > 3) throw Error(EGTB_ERR_ARG, "symm_mode must be 0..3");
+        if (n == 0 || (!out4 && !canon)) return;
+        DeviceCtx &C = device_ctx(device);
+        hipStream_t s = C.stream;
+        DBuf b, o, c;
+        b.alloc(n * 8, s);
+        if (out4) o.alloc(n * 32, s);
+        if (canon) c.alloc(n * 8, s);
+        HIPCHK(hipMemcpyAsync(b.p, boards, n * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_board_ops, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const uint64_t *)b.p,
+                           (uint64_t)n, C.lut, (uint32_t)symm_mode, (uint64_t *)o.p, (uint64_t *)c.p);
+        HIPCHK(hipGetLastError());
+        if (out4) HIPCHK(hipMemcpyAsync(out4, o.p, n * 32, hipMemcpyDeviceToHost, s));
+        if (canon) HIPCHK(hipMemcpyAsync(canon, c.p, n * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     });
 }

@@ -149,6 +149,8 @@ SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32)]),
     "egtb_table_free": (None, [ctypes.c_void_p]),
+    "egtb_table_set_prefer_max": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "egtb_board_moves": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int32, u64p, u64p, ctypes.c_int]),
     "egtb_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "egtb_server_attach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "egtb_server_detach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

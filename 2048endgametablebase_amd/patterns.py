@@ -132,6 +132,35 @@ pattern_catalog: Dict[str, PatternMeta] = {
                ["0x0000100000001fff", "0x0001000000001fff"], 48),
         _entry("44", "others", ["0xffffffff"], "0xffff00000000", "identity",
                ["0x00001000ffffffff"], 48),
+        _entry("L3f", "10 space",
+               ["0xfff0fff", "0xf0fff0ff0", "0xf000f0ff00ff0", "0xf00fff0ff0", "0xf000fff0ff0", "0xff0ff00ff0",
+                "0xff00ff00ff0", "0xf000fff00ff", "0xf00fff00ff", "0xff0fff0f00", "f00f00ff00ff0",
+                "fff0fff0000", "ff00fff0f00", "f00fff0f0f"],
+               "0xffffffffffffffff", "identity", ["0x100000001fff2fff", "0x000000012fff1fff"], 48),
+        _entry("442t", "10 space",
+               ["0xffffff", "0xf0ff0fff", "0xffff0ff", "0xffff00ff", "0xf00fffff0", "0xff0ff0ff0",
+                "0xf00000ff0fff", "0xf0000fff00ff", "0xf00f00ff0ff0", "0xf00f0fff00f0"],
+               "0xff0ff000000", "identity", ["0x1000000021ffffff", "0x0000000112ffffff"], 48),
+        _entry("4432f", "12 space", ["0xf00ff"], "0xf00000", "diagonal",
+               ["0x00001000121f2fff", "0x00010000121ff2ff", "0x0001000012ff21ff"], 48),
+        _entry("4442ff", "12 space", ["0xff"], "0xff0fff0f00", "identity",
+               ["0x0000100012ff21ff", "0x000100001212ffff", "0x00ff0012102100ff"], 48),
+        _entry("4421", "others", ["0xff0fff"], "0xf000000", "identity",
+               ["0x1000000012ff2fff", "0x1000000121ff1fff"], 48),
+        _entry("4442f", "others", ["0xff"], "0xffff00", "identity",
+               ["0x00001000121f21ff", "0x0001000012112fff"], 48),
+        _entry("L1", "others", ["ff0fff0fff"], "0xf0000000000", "diagonal", ["0x11202ff2fff1fff"], 12),
+        _entry("free8", "free", [], "0xffffffffffffffff", "full", ["0x01111111ffffffff"], 36),
+        _entry("free9", "free", [], "0xffffffffffffffff", "full", ["0x011111111fffffff"], 36),
+        _entry("free10", "free", [], "0xffffffffffffffff", "full", ["0x0111111111ffffff"], 36),
+        _entry("free11", "free", [], "0xffffffffffffffff", "full", ["0x01111111111fffff"], 36),
+        _entry("free12", "free", [], "0xffffffffffffffff", "full", ["0x011111111111ffff"], 36),
+        # variant boards (min24/min33/min34 symmetry): listed so that a build
+        # request fails with a clear message rather than "unknown pattern"
+        _entry("2x4", "variant", [], "0xffffffff0000", "min24", ["0xffff00000000ffff"], 48),
+        _entry("3x3", "variant", [], "0xfff0fff0fff00000", "min33", ["0x000f000f000fffff"], 60),
+        _entry("3x4", "variant", [], "0xffffffffffff0000", "min34", ["0x000000000000ffff"], 60),
+        _entry("3x405", "variant", [], "0xffffffffffff0000", "min34", ["0x0001218912abffff"], -1900),
     )
 }
 
@@ -146,11 +175,17 @@ def steps_and_docheck(tile_sum: int, target: int, extra_steps: int) -> Tuple[int
 
 
 def build_params(pattern: str, target: int) -> Dict[str, object]:
-    """Everything a classic build of ``pattern`` at 2**target needs."""
+    """Everything a classic build of ``pattern`` at 2**target needs.  The
+    seeds are the catalogue's; start_build widens those of the free*
+    patterns with generate_free_inits (BookBuilder.py:272-281)."""
     meta = pattern_catalog[pattern]
+    if meta.category == "variant":
+        raise NotImplementedError(f"pattern {pattern} uses variant boards ({meta.canonical_mode} symmetry), "
+                                  "which this engine does not build")
     steps, docheck = steps_and_docheck(meta.tile_sum, target, meta.extra_steps)
-    # BookBuilder.py:283-285: is_free = fewer than 4 fixed cells and small sum.
-    is_free = len(meta.fixed_positions) < 4 and meta.tile_sum < 180000
+    # BookBuilder.py:272-285: free patterns are always is_free; otherwise
+    # fewer than 4 fixed cells and a small sum.
+    is_free = pattern.startswith("free") or (len(meta.fixed_positions) < 4 and meta.tile_sum < 180000)
     return {
         "pattern": pattern,
         "target": int(target),

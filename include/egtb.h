@@ -196,6 +196,23 @@ EGTB_EXPORT int egtb_lookup_batch_f64(egtb_table *t, const uint64_t *boards, siz
                                       int32_t *status, int32_t *op_used);
 EGTB_EXPORT void egtb_table_free(egtb_table *t);
 
+/* Answer from the symmetry op whose best rate is largest instead of the
+ * first op that answers; a board whose rates are all 0 then counts as not
+ * answered (status EGTB_LOOKUP_FALLBACK).  The reference reader does this
+ * for 4442f, 4442ff and 4tiler (prefer_max_result_,
+ * ReaderRuntime.cpp:827-832,1053). */
+EGTB_EXPORT int egtb_table_set_prefer_max(egtb_table *t, int32_t on);
+
+/* Moves and canonical forms of n boards on the device.  out4 (optional,
+ * n x 4): the boards after left, right, up, down, an impossible move giving
+ * the board unchanged (MoverEngine::move_all_dir, CommonMover.h:136-139).
+ * canon (optional, n): each board's canonical form under symm_mode 0..3
+ * (Calculator.h:80-98,192-203).  Replaces mover_runtime.std.move_all_dir and
+ * mover_runtime.canonical_* as used by start_build's free-pattern seeds
+ * (engine_core/BookBuilder.py:146-181). */
+EGTB_EXPORT int egtb_board_moves(const uint64_t *boards, size_t n, int32_t symm_mode, uint64_t *out4,
+                                 uint64_t *canon, int device);
+
 /* ---- serving: continuous batching over resident tables -------------------- */
 
 /* A request loop that keeps tables in HBM and answers position queries.

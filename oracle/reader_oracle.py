@@ -103,7 +103,10 @@ def is_pattern(b, masks):
     return not masks or any((b & m) == m for m in masks)
 
 
-def move_on_dic(board, folder, pattern_full, nums_adjust, masks, symm, dtype="float32"):
+def move_on_dic(board, folder, pattern_full, nums_adjust, masks, symm, dtype="float32", prefer_max=False):
+    """evaluate_classic_result_candidates (ReaderRuntime.cpp:739-840) for one
+    folder.  prefer_max: every op is tried and the op whose best (rounded)
+    rate is strictly largest, starting from 0, answers (:827-832)."""
     board = np.asarray(board, dtype=np.int64).reshape(4, 4)
     v = int(board.sum()) + nums_adjust
     nums = abs(v) // 2 if v >= 0 else -(abs(v) // 2)
@@ -113,6 +116,7 @@ def move_on_dic(board, folder, pattern_full, nums_adjust, masks, symm, dtype="fl
     path = os.path.join(folder, f"{pattern_full}_{nums}.book")
     if not os.path.exists(path):
         return blank, ""
+    best, found = 0.0, None
     for op in range(8):
         enc = encode(apply_operation(board, op))
         if not is_pattern(enc, masks):
@@ -132,5 +136,10 @@ def move_on_dic(board, folder, pattern_full, nums_adjust, masks, symm, dtype="fl
         num = [e for e in entries if e[1] is not None]
         oth = [e for e in entries if e[1] is None]
         num.sort(key=lambda e: -e[1])
-        return dict(num + oth), dtype
+        if not prefer_max:
+            return dict(num + oth), dtype
+        if num[0][1] > best:
+            best, found = num[0][1], dict(num + oth)
+    if prefer_max and found is not None:
+        return found, dtype
     return blank, ""

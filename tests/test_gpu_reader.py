@@ -1,6 +1,7 @@
 """GPU reader / lookup parity: BookReader.move_on_dic over HBM tables
 against the CPU restatement of the reference reader (on-disk binary
 search), the fallback against its definition, and table sources agreeing."""
+import importlib
 import math
 
 import numpy as np
@@ -140,3 +141,25 @@ def test_fallback_and_table_sources_agree(pkg, built):
     t_mem.close()
     t_file.close()
     res.close()
+
+
+def test_prefer_max_matches_reference_reader(pkg, built):
+    """prefer_max (the reference reader's rule for 4442f/4442ff/4tiler): all
+    ops are tried and the one with the largest rounded best rate answers."""
+    folder, prefix = built
+    meta = pkg.patterns.pattern_catalog[PATTERN]
+    t = pkg.Table.open(prefix, PATTERN, TARGET, "float32", prefer_max=True)
+    boards = sample_boards(prefix, pkg.patterns.build_params(PATTERN, TARGET)["steps"], 400, 5)
+    reader_mod = importlib.import_module("2048endgametablebase_amd.BookReader")
+    enc = [reader_mod.encode_board(m) for m in boards]
+    got = t.move_on_dic_batch(enc)
+    differs_from_first = 0
+    for m, g in zip(boards, got):
+        want = ro.move_on_dic(m, folder, f"{PATTERN}_{2**TARGET}", meta.nums_adjust, list(meta.pattern_masks),
+                              meta.symm_mode, prefer_max=True)
+        assert g == want, (m.tolist(), g, want)
+        first = ro.move_on_dic(m, folder, f"{PATTERN}_{2**TARGET}", meta.nums_adjust, list(meta.pattern_masks),
+                               meta.symm_mode)
+        differs_from_first += first != want
+    assert differs_from_first > 0  # the rule is exercised
+    t.close()
