@@ -47,7 +47,8 @@ class _Args:
 
     def __init__(self, seeds, masks, shifts, symm, target, steps, docheck, pathname, is_free,
                  spawn_rate4, dtype, write_files, keep_raw, device, radix_bits, sort_schedule=0,
-                 chunk_mkeys=0, host_tables=False, write_raw_files=False, deletion_threshold=0.0):
+                 chunk_mkeys=0, host_tables=False, write_raw_files=False, deletion_threshold=0.0,
+                 solve_hash_mb=0):
         self.seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
         self.masks = np.ascontiguousarray(np.asarray(list(masks) or [0], dtype=np.uint64))
         self.shifts = np.ascontiguousarray(np.asarray(list(shifts) or [0], dtype=np.uint8))
@@ -74,6 +75,7 @@ class _Args:
         o.host_tables = int(bool(host_tables))
         o.write_raw_files = int(write_raw_files)
         o.deletion_threshold = float(deletion_threshold)
+        o.solve_hash_mb = int(solve_hash_mb)
 
 
 class BuildResult:
@@ -152,7 +154,7 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
                       write_files: bool = True, keep: bool = False, keep_raw: bool = False,
                       device: int = 0, radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
                       host_tables: bool = False, write_raw_files: int = 0,
-                      deletion_threshold: float = 0.0) -> Optional[BuildResult]:
+                      deletion_threshold: float = 0.0, solve_hash_mb: int = 0) -> Optional[BuildResult]:
     """The native build call (reference formation_core.run_pattern_build,
     native_core/src/bindings_formation.cpp:220-229) with the PatternSpec and
     RunOptions fields spelled out.  ``keep=True`` returns the solved layers
@@ -161,7 +163,7 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
     again (BookSolver.cpp:476-511)."""
     a = _Args(arr_init, masks, shifts, symm_mode, target, steps, docheck_step, pathname, is_free,
               spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits, sort_schedule,
-              chunk_mkeys, host_tables, write_raw_files, deletion_threshold)
+              chunk_mkeys, host_tables, write_raw_files, deletion_threshold, solve_hash_mb)
     L = native.lib()
     seeds_p = a.seeds.ctypes.data_as(native.u64p)
     if keep or keep_raw:
@@ -289,7 +291,7 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
 def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "float32",
                     spawn_rate4: float = 0.1, keep_raw: bool = False, device: int = 0,
                     radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
-                    host_tables: bool = False) -> BuildResult:
+                    host_tables: bool = False, solve_hash_mb: int = 0) -> BuildResult:
     """Build a catalogue pattern (layer 0 as start_build makes it) and keep
     every solved layer in HBM."""
     p = build_params(pattern, target)
@@ -297,4 +299,5 @@ def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "flo
                              p["docheck_step"], "", is_free=p["is_free"], spawn_rate4=spawn_rate4,
                              success_rate_dtype=success_rate_dtype, write_files=False, keep=True,
                              keep_raw=keep_raw, device=device, radix_bits=radix_bits,
-                             sort_schedule=sort_schedule, chunk_mkeys=chunk_mkeys, host_tables=host_tables)
+                             sort_schedule=sort_schedule, chunk_mkeys=chunk_mkeys, host_tables=host_tables,
+                             solve_hash_mb=solve_hash_mb)

@@ -1,6 +1,7 @@
 """Autotuner of the dedup sort: digit width (4, 6, 8, 11 bits) and schedule
-(per-digit three-kernel passes vs single-pass decoupled look-back), chosen
-per key-count bucket by timing sort + unique of real layer keys on the GPU.
+(per-digit three-kernel passes, single-pass decoupled look-back, or sample
+sort with per-bucket sort + unique in LDS up to 2^21 keys), chosen per
+key-count bucket by timing sort + unique of real layer keys on the GPU.
 
     python -m ... autotune   (see main below)
 
@@ -21,6 +22,7 @@ from .BookBuilder import build_in_memory
 
 BUCKETS = [1 << 12, 1 << 14, 1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24]
 CANDIDATES = [(b, s) for s in (1, 2) for b in (4, 6, 8, 11)]
+SAMPLE_SORT_MAX = 1 << 21  # kBucketMaxKeys of the sample-sort dedup
 
 
 def sample_keys(pattern: str = "442", target: int = 8, seed: int = 0):
@@ -48,7 +50,7 @@ def tune(pattern: str = "442", target: int = 8, buckets=BUCKETS, reps: int = 3, 
     for n in buckets:
         keys = pool[rng.integers(0, pool.size, n)]
         timings = {}
-        for bits, sched in CANDIDATES:
+        for bits, sched in CANDIDATES + ([(8, 3)] if n <= SAMPLE_SORT_MAX else []):
             timings[(bits, sched)] = time_sort(keys, bits, sched, reps)
         (bits, sched), best = min(timings.items(), key=lambda kv: kv[1])
         rows.append({"max_n": int(n), "bits": bits, "schedule": sched})

@@ -41,6 +41,56 @@ __device__ __forceinline__ T table_lookup_dev(const TableDev<T> &t, uint64_t key
     return table_lookup_n(t, t.n, key);
 }
 
+// Open-addressing hash of one solved layer, the structure the backward solve
+// looks successors up in: 16-byte slots {key, rate bits}, key 0 marks an
+// empty slot (every stored board holds tiles, so no key is 0), linear
+// probing from hash_mix(key).  The capacity is a power of two at least twice
+// the layer's size bound, so every probe run ends at an empty slot; a
+// lookup usually reads one cache line.
+struct HashDev {
+    const ulonglong2 *slots;
+    uint64_t mask;  // capacity - 1
+};
+
+// murmur3's 64-bit finaliser: board keys share most of their bits
+__device__ __forceinline__ uint64_t hash_mix(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T rate_from_bits(uint64_t v) {
+    if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, (uint32_t)v);
+    else return __builtin_bit_cast(T, v);
+}
+
+uint64_t hash_capacity(uint64_t bound);
+// Insert records [0, n) of a layer (n = *n_ptr when non-null, else bound).
+template <typename T>
+void launch_hash_build(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t bound, const uint64_t *n_ptr,
+                       ulonglong2 *slots, uint64_t capacity);
+
+// Sample-sort dedup (egtb_bucket.hip): sorted unique keys of up to
+// kBucketMaxKeys candidates in six launches.  BucketWork holds its device
+// scratch: splitters (1023 x u64), per-bucket counts, starts, cursors and
+// unique counts (1024 x u32 each) and each key's bucket.
+constexpr uint64_t kBucketMaxKeys = 1ull << 21;
+struct BucketWork {
+    uint64_t *spl;
+    uint32_t *counts, *starts, *cursor, *ucount;
+    uint16_t *ids;  // bucket of each key (n entries)
+};
+uint32_t bucket_count_for(uint64_t n);
+void configure_bucket_kernels();
+// keys is read only; tmp holds n keys; scratch holds 2n keys (used only by a
+// bucket too large for LDS).
+void launch_bucket_dedup(hipStream_t s, const uint64_t *keys, uint64_t *tmp, uint64_t *scratch, uint64_t n,
+                         BucketWork &bw, uint64_t *out, uint64_t *count_out);
+
 struct LaunchShape {
     uint32_t blocks;   // persistent grid size (upper bound)
     uint32_t threads;  // threads per block
@@ -105,8 +155,8 @@ void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, 
 
 template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
-                  const PatternDev &p, int do_check, const TableDev<T> &t1, const TableDev<T> &t2, double w2,
-                  double w4, T max_scale, T *rates);
+                  const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
+                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates);
 
 template <typename T>
 void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,

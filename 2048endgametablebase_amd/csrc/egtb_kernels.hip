@@ -19,6 +19,8 @@
 
 #include "egtb_device.h"
 #include "egtb_kernels.h"
+#include "egtb_runtime.h"
+#include "egtb_wave.h"
 
 namespace egtb {
 
@@ -47,51 +49,6 @@ __global__ void k_build_lut(uint16_t *lut) {
         }
     }
     lut[row] = (uint16_t)((out[0] << 12) | (out[1] << 8) | (out[2] << 4) | out[3]);
-}
-
-// ---------------------------------------------------------------------------
-// Wave / block helpers
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-__device__ __forceinline__ uint64_t lanes_below(uint64_t m) { return m & __lanemask_lt(); }
-
-template <typename T>
-__device__ __forceinline__ T wave_inclusive_sum(T v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        T y = __shfl_up(v, off, 64);
-        if (lane >= (uint32_t)off) v += y;
-    }
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_reduce_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Exclusive scan of one value per thread over the block; `total` receives
-// the block sum.  `scratch` needs blockDim/64 entries.  Ends synchronised.
-template <typename T>
-__device__ __forceinline__ T block_exclusive_sum(T v, T *scratch, T &total) {
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-    const T inc = wave_inclusive_sum(v);
-    if (lane == 63) scratch[wave] = inc;
-    __syncthreads();
-    T before = 0, sum = 0;
-    for (uint32_t w = 0; w < nwaves; ++w) {
-        const T t = scratch[w];
-        if (w < wave) before += t;
-        sum += t;
-    }
-    __syncthreads();
-    total = sum;
-    return before + inc - v;
 }
 
 // ---------------------------------------------------------------------------
@@ -357,19 +314,6 @@ __device__ __forceinline__ uint64_t lb_lookback(const uint64_t *status, uint64_t
         }
     }
     return prefix;
-}
-
-// Lanes of this wave whose digit equals mine, among the lanes in `valid`.
-template <int BITS>
-__device__ __forceinline__ uint64_t match_digit(uint32_t digit, uint64_t valid) {
-    uint64_t m = valid;
-#pragma unroll
-    for (int b = 0; b < BITS; ++b) {
-        const bool bit = (digit >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        m &= bit ? bb : ~bb;
-    }
-    return m;
 }
 
 template <int BITS>
@@ -780,13 +724,22 @@ __global__ void k_bucket_index(const uint64_t *__restrict__ keys, uint64_t n_bou
 // canonical moved board (absent -> 0).  The fused update is what the
 // reference's -Ofast -march=x86-64-v3 build computes (CMakeLists.txt:82-83).
 template <typename T>
+__device__ __forceinline__ uint64_t rate_bits(T v) {
+    if constexpr (sizeof(T) == 4) return (uint64_t)__builtin_bit_cast(uint32_t, v);
+    else return __builtin_bit_cast(uint64_t, v);
+}
+
+template <typename T>
 __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ keys, uint64_t n,
                                                 const uint16_t *__restrict__ glut, PatternDev p,
-                                                int do_check, TableDev<T> t1, TableDev<T> t2,
-                                                double w2, double w4, T max_scale,
+                                                int do_check, HashDev h1, HashDev h2, TableDev<T> t1,
+                                                TableDev<T> t2, double w2, double w4, T max_scale,
                                                 T *__restrict__ rates) {
     extern __shared__ uint16_t lut[];
     stage_lut(lut, glut);
+    // hash probes when both successor layers have a hash, else binary
+    // searches narrowed by the bucket index (uniform over the launch)
+    const bool use_hash = h1.slots != nullptr && h2.slots != nullptr;
     const uint64_t n1 = t1.n_ptr ? *t1.n_ptr : t1.n, n2 = t2.n_ptr ? *t2.n_ptr : t2.n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -802,11 +755,10 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
             const int k = __builtin_ctz(em);
             em &= em - 1;
             // the 8 successors of this cell: spawn 2 (j < 4) and spawn 4
-            // (j >= 4) times the four moves; their searches advance in
-            // lockstep so the loads of one step are all in flight together
-            uint64_t key[8];
-            uint32_t lo[8], hi[8];
-            uint32_t live = 0;
+            // (j >= 4) times the four moves; their probes advance in
+            // lockstep so the loads of one round are all in flight together
+            uint64_t key[8], slot[8];
+            uint32_t active = 0;
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const uint64_t sp_board = b | ((uint64_t)(s + 1) << (4 * k));
@@ -816,46 +768,93 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                 for (int d = 0; d < 4; ++d) {
                     const int j = s * 4 + d;
                     key[j] = canonical(mv[d], p.symm);
+                    if (mv[d] != sp_board && is_pattern(mv[d], p)) active |= 1u << j;
+                }
+            }
+            T r[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] = (T)0;
+            if (!use_hash) {
+                // bucket-narrowed binary searches, advanced in lockstep
+                uint32_t lo[8], hi[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
                     lo[j] = hi[j] = 0;
-                    const uint64_t nn = s == 0 ? n1 : n2;
-                    if (mv[d] != sp_board && is_pattern(mv[d], p) && nn) {
-                        const TableDev<T> &t = s == 0 ? t1 : t2;
+                    const TableDev<T> &t = j < 4 ? t1 : t2;
+                    if (((active >> j) & 1u) && (j < 4 ? n1 : n2)) {
                         const uint64_t bk = key[j] >> (64 - t.bits);
                         lo[j] = t.idx[bk];
                         hi[j] = t.idx[bk + 1];
-                        live |= 1u << j;
                     }
                 }
-            }
-            uint32_t active = live;
-            while (active) {
+                uint32_t open = active;
+                while (open) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if (lo[j] < hi[j]) {
+                            const uint32_t mid = (lo[j] + hi[j]) >> 1;
+                            if ((j < 4 ? t1.keys : t2.keys)[mid] < key[j]) lo[j] = mid + 1;
+                            else hi[j] = mid;
+                        }
+                    }
+                    open = 0;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) open |= (lo[j] < hi[j] ? 1u : 0u) << j;
+                }
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    if (lo[j] < hi[j]) {
-                        const uint32_t mid = (lo[j] + hi[j]) >> 1;
-                        const uint64_t km = (j < 4 ? t1.keys : t2.keys)[mid];
-                        if (km < key[j]) lo[j] = mid + 1;
-                        else hi[j] = mid;
-                    }
+                    const TableDev<T> &t = j < 4 ? t1 : t2;
+                    if (((active >> j) & 1u) && lo[j] < (j < 4 ? n1 : n2) && t.keys[lo[j]] == key[j])
+                        r[j] = t.rates[lo[j]];
                 }
                 active = 0;
+            }
 #pragma unroll
-                for (int j = 0; j < 8; ++j) active |= (lo[j] < hi[j] ? 1u : 0u) << j;
+            for (int j = 0; j < 8; ++j) slot[j] = hash_mix(key[j]) & (j < 4 ? h1.mask : h2.mask);
+            while (active) {
+                ulonglong2 ent[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if ((active >> j) & 1u) ent[j] = (j < 4 ? h1.slots : h2.slots)[slot[j]];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if ((active >> j) & 1u) {
+                        if (ent[j].x == key[j]) {
+                            r[j] = rate_from_bits<T>(ent[j].y);
+                            active &= ~(1u << j);
+                        } else if (ent[j].x == 0) {
+                            active &= ~(1u << j);  // absent: rate 0
+                        } else {
+                            slot[j] = (slot[j] + 1) & (j < 4 ? h1.mask : h2.mask);
+                        }
+                    }
+                }
             }
             T best[2] = {(T)0, (T)0};
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if ((live >> j) & 1u) {
-                    const TableDev<T> &t = j < 4 ? t1 : t2;
-                    const uint64_t nn = j < 4 ? n1 : n2;
-                    const T r = (lo[j] < nn && t.keys[lo[j]] == key[j]) ? t.rates[lo[j]] : (T)0;
-                    best[j >> 2] = best[j >> 2] < r ? r : best[j >> 2];
-                }
-            }
+            for (int j = 0; j < 8; ++j) best[j >> 2] = best[j >> 2] < r[j] ? r[j] : best[j >> 2];
             sp = __fma_rn((double)best[0], w2, sp);
             sp = __fma_rn((double)best[1], w4, sp);
         }
         rates[i] = e > 0 ? (T)(sp / (double)e) : (T)0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_hash_build(const uint64_t *__restrict__ keys, const T *__restrict__ rates,
+                                                    uint64_t bound, const uint64_t *__restrict__ n_ptr,
+                                                    ulonglong2 *__restrict__ slots, uint64_t mask) {
+    const uint64_t n = n_ptr ? *n_ptr : bound;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        if (k == 0) continue;
+        uint64_t sl = hash_mix(k) & mask;
+        // keys of a layer are unique, so the claim succeeds at the first
+        // empty slot of the run; the table is at most half full
+        while (atomicCAS((unsigned long long *)&slots[sl].x, 0ull, (unsigned long long)k) != 0ull)
+            sl = (sl + 1) & mask;
+        slots[sl].y = rate_bits(rates[i]);
     }
 }
 
@@ -895,12 +894,6 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_count(const T *__restr
         for (int w = 0; w < kCompThreads / 64; ++w) t += scratch[w];
         counts[blockIdx.x] = t;
     }
-}
-
-template <typename T>
-__device__ __forceinline__ uint64_t rate_bits(T v) {
-    if constexpr (sizeof(T) == 4) return (uint64_t)__builtin_bit_cast(uint32_t, v);
-    else return __builtin_bit_cast(uint64_t, v);
 }
 
 template <typename T>
@@ -1067,11 +1060,17 @@ template <int BITS>
 static void classic_pass_impl(hipStream_t s, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
                               uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
     const uint32_t ntiles = blocks_for(n, kSortTile);
-    hipLaunchKernelGGL((k_radix_hist<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, n, shift,
-                       counts, ntiles);
+    {
+        Timer t(s, "radix_hist", 8.0 * (double)n);  // reads every key once
+        hipLaunchKernelGGL((k_radix_hist<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, n,
+                           shift, counts, ntiles);
+    }
     launch_scan_u32(s, counts, (uint64_t)ntiles << BITS, offsets, partial);
-    hipLaunchKernelGGL((k_radix_scatter<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst, n,
-                       shift, offsets, ntiles);
+    {
+        Timer t(s, "radix_scatter", 16.0 * (double)n);  // reads and writes every key once
+        hipLaunchKernelGGL((k_radix_scatter<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst,
+                           n, shift, offsets, ntiles);
+    }
 }
 
 uint64_t *launch_classic_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
@@ -1135,13 +1134,29 @@ void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, 
 
 template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
-                  const PatternDev &p, int do_check, const TableDev<T> &t1, const TableDev<T> &t2, double w2,
-                  double w4, T max_scale, T *rates) {
+                  const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
+                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates) {
     if (n == 0) return;
     const uint32_t want = blocks_for(n, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     hipLaunchKernelGGL((k_solve<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n,
-                       lut, p, do_check, t1, t2, w2, w4, max_scale, rates);
+                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates);
+}
+
+uint64_t hash_capacity(uint64_t bound) {
+    uint64_t c = 64;
+    while (c < 2 * bound) c <<= 1;
+    return c;
+}
+
+template <typename T>
+void launch_hash_build(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t bound, const uint64_t *n_ptr,
+                       ulonglong2 *slots, uint64_t capacity) {
+    (void)hipMemsetAsync(slots, 0, capacity * sizeof(ulonglong2), s);
+    if (bound == 0) return;
+    const uint32_t grid = blocks_for(bound, 256) < 8192 ? blocks_for(bound, 256) : 8192;
+    hipLaunchKernelGGL((k_hash_build<T>), dim3(grid), dim3(256), 0, s, keys, rates, bound, n_ptr, slots,
+                       capacity - 1);
 }
 
 template <typename T>
@@ -1189,8 +1204,11 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
 
 #define EGTB_INSTANTIATE(T)                                                                                   \
     template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
-                                  const uint16_t *, const PatternDev &, int, const TableDev<T> &,             \
-                                  const TableDev<T> &, double, double, T, T *);                               \
+                                  const uint16_t *, const PatternDev &, int, const HashDev &,                 \
+                                  const HashDev &, const TableDev<T> &, const TableDev<T> &, double, double,  \
+                                  T, T *);                                                                    \
+    template void launch_hash_build<T>(hipStream_t, const uint64_t *, const T *, uint64_t, const uint64_t *,  \
+                                       ulonglong2 *, uint64_t);                                               \
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *,           \
                                     uint64_t *, uint64_t *, uint64_t *, T *, unsigned long long *,            \

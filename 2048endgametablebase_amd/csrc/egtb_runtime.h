@@ -75,6 +75,7 @@ struct HostStage {
 // Scratch of the dedup / compaction pipelines of one stream.
 struct SortWork {
     DBuf counts, offsets, partial, tile_counts, tile_offs, maxbits;
+    DBuf bk_u64, bk_u32, bk_scratch, bk_ids;  // sample-sort dedup
     DBuf lb_status_sort, lb_status_uniq, lb_misc;
     LookbackWork lw;
     // grow the look-back state for a sort of `sort_tiles` tiles of `bins`
@@ -149,6 +150,8 @@ struct Book {
     uint64_t *n_dev = nullptr;  // &LayerDev::n_live
     uint64_t n = 0;             // host copy, valid once the build finished
     uint32_t bits = 1;
+    const ulonglong2 *hslots = nullptr;  // solve-side hash (HashDev) while the layer has readers
+    uint64_t hmask = 0;
 };
 
 struct LayerInfo {
@@ -190,6 +193,8 @@ struct Result {
     // no longer needed, then in host memory
     bool host_tables = false;
     std::vector<DBuf> book_bufs;  // 3 per layer: keys, rates, index
+    std::vector<DBuf> hash_bufs;  // per layer, freed after the layer's last solve
+    double hash_max_bytes = 0;    // larger layers are searched, not hashed
     std::vector<HostArray> host_keys, host_rates;
     uint64_t streamed_bytes = 0, chunked_steps = 0, chunks = 0, peak_device_bytes = 0;
     double stream_ms = 0;
@@ -203,6 +208,10 @@ uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, u
                      int bits_req, SortWork &w, int schedule);
 void unique_into(hipStream_t s, const uint64_t *sorted, uint64_t n, uint64_t *out, uint64_t *count_out,
                  SortWork &w, int schedule);
+// Sorted unique keys of keys[0..n) into out (size to count_out, on the
+// device).  keys and tmp (n keys) are clobbered; out is a third buffer.
+void dedup_into(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
+                uint64_t *out, uint64_t *count_out, int bits_req, SortWork &w, int schedule);
 void write_file_atomic(const std::string &path, const void *data, size_t bytes);
 void progress_set(uint32_t cur, uint32_t total);
 void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *o,

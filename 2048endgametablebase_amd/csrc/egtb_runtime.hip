@@ -68,6 +68,7 @@ DeviceCtx &device_ctx(int dev) {
         throw Error(EGTB_ERR_HIP, "device LDS per block (" + std::to_string(lds) +
                                       " B) cannot hold the 128 KiB row table");
     configure_kernels();
+    configure_bucket_kernels();
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&c->lut, kLutEntries * sizeof(uint16_t)));
     launch_build_lut(c->stream, c->lut);
@@ -246,11 +247,23 @@ uint32_t SortWork::read_error(hipStream_t s) {
     return e;
 }
 
+// 1: per-digit radix (3 kernels a digit), 2: single-pass look-back radix,
+// 3: sample-sort dedup (layers up to kBucketMaxKeys; else the table's radix).
 int choose_schedule(uint64_t n, int requested) {
     if (requested == 1 || requested == 2) return requested;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     const TuneRow *r = tune_row(n);
-    return r && (r->schedule == 1 || r->schedule == 2) ? r->schedule : 1;
+    const int sc = requested == 3 ? 3 : (r ? r->schedule : 1);
+    if (sc == 3) return n <= kBucketMaxKeys ? 3 : (r && (r->schedule == 1 || r->schedule == 2) ? r->schedule : 1);
+    return sc == 2 ? 2 : 1;
+}
+
+static int radix_schedule(uint64_t n, int requested) {
+    const int sc = choose_schedule(n, requested);
+    if (sc != 3) return sc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const TuneRow *r = tune_row(n);
+    return r && r->schedule == 2 ? 2 : 1;
 }
 
 static int tuned_bits(uint64_t n, int span, int requested) {
@@ -270,7 +283,7 @@ uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, u
     const int bits = tuned_bits(n, hi - lo, bits_req);
     const uint64_t ntiles = (n + radix_tile(bits) - 1) / radix_tile(bits);
     uint64_t *out;
-    if (choose_schedule(n, schedule) == 2) {
+    if (radix_schedule(n, schedule) == 2) {
         w.ensure_lookback(s, ntiles, 1 << bits, 0);
         out = launch_onesweep_sort(s, bits, keys, tmp, n, lo, hi, w.lw);
     } else {
@@ -287,7 +300,7 @@ uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, u
 
 void unique_into(hipStream_t s, const uint64_t *sorted, uint64_t n, uint64_t *out, uint64_t *count_out,
                  SortWork &w, int schedule) {
-    if (choose_schedule(n, schedule) == 2) {
+    if (radix_schedule(n, schedule) == 2) {
         w.ensure_lookback(s, 0, 0, unique_tiles(n));
         launch_unique(s, sorted, n, out, count_out, w.lw);
     } else {
@@ -298,6 +311,34 @@ void unique_into(hipStream_t s, const uint64_t *sorted, uint64_t n, uint64_t *ou
         launch_unique_classic(s, sorted, n, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
                               (uint64_t *)w.partial.p, out, count_out);
     }
+}
+
+void dedup_into(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
+                uint64_t *out, uint64_t *count_out, int bits_req, SortWork &w, int schedule) {
+    if (n >= 2 && (or_all ^ and_all) != 0 && choose_schedule(n, schedule) == 3) {
+        Timer t(s, "sort", 16.0 * (double)n);
+        w.bk_u64.ensure(1024 * 8, s);
+        w.bk_u32.ensure(4 * 1024 * 4, s);
+        w.bk_scratch.ensure(2 * n * 8, s);
+        w.bk_ids.ensure(n * 2 + 64, s);
+        BucketWork bw;
+        bw.spl = (uint64_t *)w.bk_u64.p;
+        bw.counts = (uint32_t *)w.bk_u32.p;
+        bw.starts = bw.counts + 1024;
+        bw.cursor = bw.counts + 2048;
+        bw.ucount = bw.counts + 3072;
+        bw.ids = (uint16_t *)w.bk_ids.p;
+        launch_bucket_dedup(s, keys, tmp, (uint64_t *)w.bk_scratch.p, n, bw, out, count_out);
+        HIPCHK(hipGetLastError());
+        return;
+    }
+    uint64_t *sorted;
+    {
+        Timer t(s, "sort", 16.0 * (double)n);
+        sorted = radix_sort(s, keys, tmp, n, or_all, and_all, bits_req, w, schedule);
+    }
+    Timer t(s, "unique", 16.0 * (double)n);
+    unique_into(s, sorted, n, out, count_out, w, schedule);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,6 +529,26 @@ static uint32_t index_bits_for(uint64_t n) {
     return b;
 }
 
+// The solve-side hash of book `layer` (at most `bound` live records, count
+// in device memory).
+// Layers whose hash would exceed this many MiB are searched instead: a
+// hash is built with one random atomic per record into a table twice the
+// layer's size, which outgrows the caches on the largest layers.
+constexpr int kDefaultSolveHashMB = 1024;
+
+template <typename T>
+static void build_hash(hipStream_t s, Result &res, int layer, uint64_t bound) {
+    Book &bk = res.books[layer];
+    const uint64_t cap = hash_capacity(bound);
+    if ((double)cap * sizeof(ulonglong2) > res.hash_max_bytes) return;  // the solve searches this layer
+    DBuf &h = res.hash_bufs[layer];
+    h.alloc(cap * sizeof(ulonglong2), s);
+    Timer t(s, "hash", (double)bound * (8 + sizeof(T)) + (double)cap * sizeof(ulonglong2));
+    launch_hash_build<T>(s, bk.keys, (const T *)bk.rates, bound, bk.n_dev, (ulonglong2 *)h.p, cap);
+    bk.hslots = (const ulonglong2 *)h.p;
+    bk.hmask = cap - 1;
+}
+
 template <typename T>
 static TableDev<T> table_of(const Book &bk) {
     TableDev<T> t;
@@ -499,6 +560,19 @@ static TableDev<T> table_of(const Book &bk) {
     t.bits = bk.bits;
     t.present = 1;
     return t;
+}
+
+static HashDev hash_of(const Book &bk) {
+    HashDev h;
+    h.slots = bk.hslots;
+    h.mask = bk.hmask;
+    return h;
+}
+
+static void drop_hash(Result &res, int layer) {
+    res.hash_bufs[layer].release();
+    res.books[layer].hslots = nullptr;
+    res.books[layer].hmask = 0;
 }
 
 // Carve book storage for a layer of at most `bound` live records, compact
@@ -534,6 +608,7 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
         bk.bits = index_bits_for(bound);
         bk.idx = (uint32_t *)carve(((1ull << bk.bits) + 1) * 4, 2);
         launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
+        build_hash<T>(s, res, layer, bound);
     }
 }
 
@@ -928,6 +1003,7 @@ static void load_book_file(hipStream_t s, Result &res, int layer, const std::str
         launch_unpack_records(s, (const uint32_t *)packed.p, n, (int)sizeof(T), bk.keys, bk.rates);
     }
     launch_bucket_index(s, bk.keys, n, bk.n_dev, bk.bits, bk.idx);
+    build_hash<T>(s, res, layer, n);
     HIPCHK(hipStreamSynchronize(s));  // the staging buffers are reused
 }
 
@@ -991,6 +1067,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     res.books.assign(steps, Book{});
     res.info.assign(steps, LayerInfo{});
     if (o.host_tables) res.raw_arena.set_stream(s);
+    res.hash_bufs.clear();
+    res.hash_bufs.resize(steps);
+    res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
+                         : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
     SortWork w;
     HostStage stage, meta_host_buf;
     DBuf cand[2], sort_tmp, counts, offs, rates, packed;
@@ -1052,7 +1132,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     bool chunked = false;
     std::vector<DBuf> runs_cur, runs_next;
     std::vector<uint64_t> runs_cur_n, runs_next_n;
-    DBuf chunk2, chunk4, run_tmp, cnt_dev, gidx, gval;
+    DBuf chunk2, chunk4, run_tmp, run_out, cnt_dev, gidx, gval;
     cnt_dev.alloc(64, s);
     auto budget_keys = [&]() -> uint64_t {
         if (o.chunk_mkeys > 0) return (uint64_t)o.chunk_mkeys << 20;
@@ -1066,17 +1146,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                       std::vector<uint64_t> &runs_n) {
         if (n == 0) return;
         run_tmp.ensure(n * 8, s);
-        uint64_t *sorted;
-        {
-            Timer t(s, "sort", 16.0 * (double)n);
-            sorted = radix_sort(s, keys, (uint64_t *)run_tmp.p, n, or_all, and_all, o.radix_bits, w,
-                                o.sort_schedule);
-        }
-        uint64_t *u = sorted == keys ? (uint64_t *)run_tmp.p : keys;
-        {
-            Timer t(s, "unique", 16.0 * (double)n);
-            unique_into(s, sorted, n, u, (uint64_t *)cnt_dev.p, w, o.sort_schedule);
-        }
+        run_out.ensure(n * 8, s);
+        uint64_t *u = (uint64_t *)run_out.p;
+        dedup_into(s, keys, (uint64_t *)run_tmp.p, n, or_all, and_all, u, (uint64_t *)cnt_dev.p, o.radix_bits, w,
+                   o.sort_schedule);
         uint64_t m = 0;
         HIPCHK(hipMemcpyAsync(&m, cnt_dev.p, 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -1101,17 +1174,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         runs.clear();
         runs_n.clear();
         run_tmp.ensure(tot * 8, s);
-        uint64_t *sorted;
-        {
-            Timer t(s, "sort", 16.0 * (double)tot);
-            sorted = radix_sort(s, (uint64_t *)cat.p, (uint64_t *)run_tmp.p, tot, H[L].or_all, H[L].and_all,
-                                o.radix_bits, w, o.sort_schedule);
-        }
         res.raw[L] = (uint64_t *)res.raw_arena.alloc(tot * 8);
-        {
-            Timer t(s, "unique", 16.0 * (double)tot);
-            unique_into(s, sorted, tot, res.raw[L], &M[L].n_raw, w, o.sort_schedule);
-        }
+        dedup_into(s, (uint64_t *)cat.p, (uint64_t *)run_tmp.p, tot, H[L].or_all, H[L].and_all, res.raw[L],
+                   &M[L].n_raw, o.radix_bits, w, o.sort_schedule);
         return tot;
     };
 
@@ -1158,17 +1223,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                               (const uint64_t *)offs.p, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, nullptr,
                               nullptr);
             }
-            uint64_t *sorted;
-            {
-                Timer t(s, "sort", 16.0 * (double)n_c);
-                sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all,
-                                    o.radix_bits, w, o.sort_schedule);
-            }
             res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            {
-                Timer t(s, "unique", 16.0 * (double)n_c);
-                unique_into(s, sorted, n_c, res.raw[i], &M[i].n_raw, w, o.sort_schedule);
-            }
+            dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all, res.raw[i],
+                       &M[i].n_raw, o.radix_bits, w, o.sort_schedule);
             bound_next = n_c;
         } else {
             // parent ranges whose candidates fit the budget, from the prefix
@@ -1232,14 +1289,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             const uint64_t n_c = c4_prev;
             DBuf &cur = cand[L & 1];
             sort_tmp.ensure(n_c * 8, s);
-            uint64_t *sorted;
-            {
-                Timer t(s, "sort", 16.0 * (double)n_c);
-                sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all,
-                                    o.radix_bits, w, o.sort_schedule);
-            }
             res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            unique_into(s, sorted, n_c, res.raw[L], &M[L].n_raw, w, o.sort_schedule);
+            dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all, res.raw[L],
+                       &M[L].n_raw, o.radix_bits, w, o.sort_schedule);
         } else {
             merge_runs(L, runs_cur, runs_cur_n);
         }
@@ -1250,6 +1302,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     chunk2.release();
     chunk4.release();
     run_tmp.release();
+    run_out.release();
     cand[0].release();
     cand[1].release();
     sort_tmp.release();
@@ -1310,10 +1363,11 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         const Book &b1 = res.books[i + 1], &b2 = res.books[i + 2];
         {
             Timer t(s, "solve", (double)n * (8 + sizeof(T)));
-            launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, table_of<T>(b1),
-                            table_of<T>(b2), w2, w4, max_scale, (T *)rates.p);
+            launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
+                            hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p);
         }
         make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0);
+        drop_hash(res, i + 2);  // its last reader was this layer's solve
         HIPCHK(hipGetLastError());
         if (streamer) {
             // book i+2 was last read by this layer's solve; raw layers above
@@ -1368,6 +1422,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             }
         }
     }
+    if (steps > 1) drop_hash(res, 1);
     if (streamer) {
         streamer->retire(1, s);
         streamer->retire(0, s);
@@ -1612,10 +1667,11 @@ int egtb_bench_sort(const uint64_t *keys, size_t n, int bits, int schedule, int 
         if (!keys || n == 0 || reps <= 0 || !ms_per_sort) throw Error(EGTB_ERR_ARG, "invalid sort benchmark");
         DeviceCtx &C = device_ctx(0);
         hipStream_t s = C.stream;
-        DBuf src, a, t, cnt;
+        DBuf src, a, t, u, cnt;
         src.alloc(n * 8, s);
         a.alloc(n * 8, s);
         t.alloc(n * 8, s);
+        u.alloc(n * 8, s);
         cnt.alloc(8, s);
         HIPCHK(hipMemcpyAsync(src.p, keys, n * 8, hipMemcpyHostToDevice, s));
         uint64_t o = 0, an = ~0ull;
@@ -1631,9 +1687,8 @@ int egtb_bench_sort(const uint64_t *keys, size_t n, int bits, int schedule, int 
         for (int r = 0; r <= reps; ++r) {  // first round warms up
             HIPCHK(hipMemcpyAsync(a.p, src.p, n * 8, hipMemcpyDeviceToDevice, s));
             HIPCHK(hipEventRecord(e0, s));
-            uint64_t *out = radix_sort(s, (uint64_t *)a.p, (uint64_t *)t.p, n, o, an, bits, w, schedule);
-            unique_into(s, out, n, (uint64_t *)t.p == out ? (uint64_t *)a.p : (uint64_t *)t.p, (uint64_t *)cnt.p, w,
-                        schedule);
+            dedup_into(s, (uint64_t *)a.p, (uint64_t *)t.p, n, o, an, (uint64_t *)u.p, (uint64_t *)cnt.p, bits, w,
+                       schedule);
             HIPCHK(hipEventRecord(e1, s));
             HIPCHK(hipEventSynchronize(e1));
             float ms = 0;
@@ -1644,6 +1699,36 @@ int egtb_bench_sort(const uint64_t *keys, size_t n, int bits, int schedule, int 
         (void)hipEventDestroy(e1);
         if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
         *ms_per_sort = total / reps;
+    });
+}
+
+int egtb_sort_unique_u64(const uint64_t *keys, size_t n, int schedule, uint64_t *out, size_t *n_out) {
+    return guarded([&] {
+        if ((n && (!keys || !out)) || !n_out) throw Error(EGTB_ERR_ARG, "invalid dedup arguments");
+        if (schedule < 0 || schedule > 3) throw Error(EGTB_ERR_ARG, "schedule must be 0..3");
+        *n_out = 0;
+        if (n == 0) return;
+        DeviceCtx &C = device_ctx(0);
+        hipStream_t s = C.stream;
+        DBuf a, t, u, cnt;
+        a.alloc(n * 8, s);
+        t.alloc(n * 8, s);
+        u.alloc(n * 8, s);
+        cnt.alloc(8, s);
+        HIPCHK(hipMemcpyAsync(a.p, keys, n * 8, hipMemcpyHostToDevice, s));
+        uint64_t o = 0, an = ~0ull;
+        for (size_t i = 0; i < n; ++i) {
+            o |= keys[i];
+            an &= keys[i];
+        }
+        SortWork w;
+        dedup_into(s, (uint64_t *)a.p, (uint64_t *)t.p, n, o, an, (uint64_t *)u.p, (uint64_t *)cnt.p, 0, w, schedule);
+        uint64_t m = 0;
+        HIPCHK(hipMemcpyAsync(&m, cnt.p, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (m) HIPCHK(hipMemcpy(out, u.p, m * 8, hipMemcpyDeviceToHost));
+        if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
+        *n_out = m;
     });
 }
 

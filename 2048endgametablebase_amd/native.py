@@ -56,7 +56,8 @@ class Options(ctypes.Structure):
         ("host_tables", ctypes.c_int32),
         ("write_raw_files", ctypes.c_int32),
         ("deletion_threshold", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 2),
+        ("solve_hash_mb", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -150,6 +151,8 @@ SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_int32)]),
     "egtb_table_free": (None, [ctypes.c_void_p]),
     "egtb_table_set_prefer_max": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "egtb_sort_unique_u64": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, u64p,
+                                             ctypes.POINTER(ctypes.c_size_t)]),
     "egtb_board_moves": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int32, u64p, u64p, ctypes.c_int]),
     "egtb_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "egtb_server_attach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),

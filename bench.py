@@ -25,6 +25,38 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "solved states/s for 442@512 build (batched vs plain); query QPS at p99 latency"
+# single-kernel stats families and the kernels they time
+KERNEL_OF = {"radix_scatter": "egtb::k_radix_scatter<8, 16>", "radix_hist": "egtb::k_radix_hist<8, 16>",
+             "solve": "egtb::k_solve<float>", "expand": "egtb::k_expand<true>"}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_442_512_plain_build.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy calibration),
+    or None when the summary has no row for it."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 5 and cells[0].strip("`") == kernel:
+                    return round((float(cells[3]) + float(cells[4])) * 1e6, 0)
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def algorithmic_build_bytes(res):
+    """Compulsory bytes of one build: forward Bf(i) = 8 N_{i-1} + 16 C_i + 8 N_i
+    over steps 1..steps-2, backward Bb(i) = 12 N_i + 12 (N'_{i+1} + N'_{i+2})
+    + 12 N'_i over layers 0..steps-3 (float32 rates)."""
+    info = [res.layer_info(i) for i in range(res.steps)]
+    n = [x[0] for x in info]
+    live = [x[1] for x in info]
+    fwd = sum(8 * n[i - 1] + 16 * (info[i][2] + info[i][3]) + 8 * n[i] for i in range(1, res.steps - 1))
+    bwd = sum(12 * n[i] + 12 * (live[i + 1] + live[i + 2]) + 12 * live[i] for i in range(0, res.steps - 2))
+    return fwd + bwd
 
 
 def kernel_stats(native, names):
@@ -144,41 +176,62 @@ def main():
         one_build().close()
     torch.cuda.synchronize()
 
-    L.egtb_stats_reset()
-    L.egtb_stats_enable(1)
+    # timed region: statistics off, so no event pairs are recorded
+    L.egtb_stats_enable(0)
     totals = []
     fwd = bwd = 0.0
     n_states = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    first = None
     for _ in range(args.steps):
         res = one_build()
         f, b, tt = res.timing()
         fwd += f
         bwd += b
         totals.append(tt)
-        if n_states == 0:
-            layers = res.steps
-            n_states = sum(res.layer_info(i)[0] for i in range(res.steps))
-            live = sum(res.layer_info(i)[1] for i in range(res.steps))
-            cands = sum(res.layer_info(i)[2] + res.layer_info(i)[3] for i in range(res.steps))
-        res.close()
+        if first is None:
+            first = res  # counted after the timed region
+        else:
+            res.close()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    L.egtb_stats_enable(0)
-    stats = kernel_stats(native, ["expand_count", "expand", "sort", "unique", "solve", "compact"])
-
+    layers = first.steps
+    info = [first.layer_info(i) for i in range(first.steps)]
+    n_states = sum(x[0] for x in info)
+    live = sum(x[1] for x in info)
+    cands = sum(x[2] + x[3] for x in info)
+    first.close()
     ms_per_step = wall * 1000.0 / args.steps
     value = n_states / (wall / args.steps)
-    dom = max(stats.items(), key=lambda kv: kv[1]["ms"]) if stats else None
+
+    # instrumented builds: HIP events around each launch group on the
+    # library's stream (the stream the kernels run on), per kernel family
+    stat_builds = 2
+    L.egtb_stats_reset()
+    L.egtb_stats_enable(1)
+    for _ in range(stat_builds):
+        r = one_build()
+        alg_bytes = algorithmic_build_bytes(r)
+        r.close()
+    torch.cuda.synchronize()
+    L.egtb_stats_enable(0)
+    stats = kernel_stats(native, ["expand_count", "expand", "sort", "unique", "solve", "compact", "hash",
+                                  "radix_hist", "radix_scatter"])
+    single = {k: v for k, v in stats.items() if k in KERNEL_OF and v["launches"]}
     roofline = None
-    if dom:
-        name, st = dom
-        avg_ms = st["ms"] / max(st["launches"], 1)
-        achieved = (st["bytes"] / st["launches"]) / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+    if single:
+        name, st = max(single.items(), key=lambda kv: kv[1]["ms"])
+        avg_ms = st["ms"] / st["launches"]
+        per_launch = st["bytes"] / st["launches"]
+        achieved = per_launch / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
-                    "avg_launch_ms": round(avg_ms, 5), "launches_per_step": st["launches"] // args.steps}
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(KERNEL_OF[name]),
+                    "kernel": KERNEL_OF[name], "family": name, "alg_bytes_per_launch": round(per_launch, 0),
+                    "avg_launch_ms": round(avg_ms, 5), "launches_per_build": st["launches"] // stat_builds,
+                    "build": {"alg_bytes": int(alg_bytes),
+                              "achieved_GBps": round(alg_bytes / (ms_per_step / 1000.0) / 1e9, 2),
+                              "frac": round(alg_bytes / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 5)}}
 
     batched = None
     if not args.no_batched:
@@ -200,7 +253,7 @@ def main():
             cpu = {"error": str(exc)}
 
     line = {
-        "metric": "solved states/s for 442@512 build (plain GPU schedule)",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "states/s",
         "n_gpus": 1,
@@ -216,8 +269,9 @@ def main():
                    "layers": layers, "states_per_build": n_states,
                    "live_records": live, "candidates": cands, "fwd_ms": round(fwd / args.steps, 2),
                    "bwd_ms": round(bwd / args.steps, 2),
-                   "kernels": {k: {"ms_per_step": round(v["ms"] / args.steps, 3),
-                                   "launches_per_step": v["launches"] // args.steps} for k, v in stats.items()}},
+                   "schedule": "plain (per-layer launches)",
+                   "kernels": {k: {"ms_per_build": round(v["ms"] / stat_builds, 3),
+                                   "launches_per_build": v["launches"] // stat_builds} for k, v in stats.items()}},
         "roofline": roofline,
         "batched": batched,
         "serving": serving,

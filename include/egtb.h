@@ -68,7 +68,10 @@ typedef struct egtb_options {
                                         2: write them and keep them */
     double deletion_threshold;       /* RunOptions::deletion_threshold: after layer i is solved, book
                                         i+2 is rewritten with rates > threshold only (0 = off) */
-    int32_t reserved[2];
+    int32_t solve_hash_mb;           /* successor lookups of the solve: hash tables up to this many MiB
+                                        per layer, bucket-indexed binary search above; 0 = default,
+                                        -1 = always search */
+    int32_t reserved[1];
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file
@@ -284,6 +287,13 @@ EGTB_EXPORT void sort_uint64(uint64_t *arr, size_t arrsize, bool descending);
 EGTB_EXPORT size_t unique_sorted_u64_inplace(uint64_t *data, size_t size);
 
 /* ---- tuning ---------------------------------------------------------------- */
+
+/* Sorted unique copy of keys[0..n) into out (room for n), count in *n_out,
+ * with the dedup the builder uses for its layers: schedule 0 = tuned,
+ * 1/2 = radix sort (per-digit / look-back) + unique, 3 = sample sort with
+ * per-bucket sort and unique in LDS (n <= 2^21; larger n use radix).
+ * Replaces sort_array + parallel_unique (BookGeneratorUtils.cpp:138-206). */
+EGTB_EXPORT int egtb_sort_unique_u64(const uint64_t *keys, size_t n, int schedule, uint64_t *out, size_t *n_out);
 
 /* Install the sort tuning table used when options ask for autotuned values
  * (radix_bits = 0, sort_schedule = 0): a sort of n keys uses the first row

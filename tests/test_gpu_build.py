@@ -1,6 +1,7 @@
 """GPU parity of the plain build: the HIP path (through the C ABI) against
 the CPU oracle (bit-exact raw layers, solved keys and float32 rate bits) and
 against golden digests of the reference's own compiled builder."""
+import ctypes
 import hashlib
 import os
 
@@ -64,7 +65,7 @@ def test_matches_reference_golden(pkg, tag):
         res.close()
 
 
-@pytest.mark.parametrize("bits,schedule", [(b, s) for s in (1, 2) for b in (4, 6, 8, 11)])
+@pytest.mark.parametrize("bits,schedule", [(b, s) for s in (1, 2) for b in (4, 6, 8, 11)] + [(0, 3)])
 def test_sort_tunings_agree(pkg, bits, schedule):
     """Digit width and sort schedule are tuning knobs: every choice gives the
     reference's table."""
@@ -126,3 +127,63 @@ def test_dedup_primitives_match_numpy(pkg):
     b = rng.integers(0, 2**40, size=5000, dtype=np.uint64)
     L.sort_uint64(b.ctypes.data_as(pkg.native.u64p), b.size, True)
     assert np.array_equal(b, np.sort(b)[::-1])
+
+
+def _dedup_cases():
+    rng = np.random.default_rng(5)
+
+    def boards(n):
+        # nibble-structured keys like real layers: small tile values, pinned 0xF cells
+        v = rng.integers(0, 10, size=(n, 10), dtype=np.uint64)
+        k = np.full(n, 0xFFFFFF, dtype=np.uint64)
+        for j in range(10):
+            k |= v[:, j] << np.uint64(24 + 4 * j)
+        return k
+
+    out = {}
+    for n in [1, 2, 3, 100, 2047, 2048, 2049, 5000, 8193, 100_000]:
+        a = rng.integers(0, 2**63, size=n, dtype=np.uint64)
+        out[f"random_{n}"] = np.concatenate([a, a[: n // 3]])
+    out["boards_1M"] = boards(1_000_000)
+    out["boards_max"] = boards(1 << 21)          # the largest sample-sort input
+    out["boards_over"] = boards((1 << 21) + 7)   # falls back to the radix sort
+    heavy = boards(60_000)
+    heavy[:30_000] = heavy[0]                    # one bucket far past the LDS capacity
+    out["heavy_duplicates"] = heavy
+    out["all_equal"] = np.full(5000, 0x1234, dtype=np.uint64)
+    skew = boards(300_000)
+    skew[::2] = skew[::2] & np.uint64(0xFFFFFFFFFF)  # half the keys in a narrow range
+    out["skewed"] = skew
+    return out
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
+def test_sort_unique_matches_numpy(pkg, schedule):
+    """egtb_sort_unique_u64 (the builder's layer dedup) under every schedule."""
+    L = pkg.native.lib()
+    for name, keys in _dedup_cases().items():
+        print(f"case dedup {name} schedule {schedule}", flush=True)
+        keys = np.ascontiguousarray(keys)
+        out = np.empty(keys.size, dtype=np.uint64)
+        m = ctypes.c_size_t()
+        pkg.native.check(L.egtb_sort_unique_u64(keys.ctypes.data_as(pkg.native.u64p), keys.size, schedule,
+                                                out.ctypes.data_as(pkg.native.u64p), ctypes.byref(m)))
+        want = np.unique(keys)
+        assert m.value == want.size, name
+        assert np.array_equal(out[: m.value], want), name
+
+
+@pytest.mark.parametrize("solve_hash_mb", [-1, 1, 100000])
+def test_solve_lookup_paths_agree(pkg, solve_hash_mb):
+    """Successor lookups by hash, by bucket-indexed binary search, or mixed
+    per layer (a 1 MiB limit) give the reference's table."""
+    g = load_golden("442_128")
+    gp = golden_params(g)
+    res = pkg.run_pattern_build(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                                gp["docheck"], "", write_files=False, keep=True, solve_hash_mb=solve_hash_mb)
+    try:
+        for layer in g["layers"]:
+            keys, rates = res.book(layer["i"])
+            assert book_sha(keys, rates) == layer["sha256"], f"layer {layer['i']}, solve_hash_mb {solve_hash_mb}"
+    finally:
+        res.close()
