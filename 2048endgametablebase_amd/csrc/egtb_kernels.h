@@ -104,7 +104,7 @@ void launch_build_lut(hipStream_t s, uint16_t *lut);
 void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                    const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
-                   uint64_t *or_and4, uint64_t offs_base = 0);
+                   uint64_t *or_and4, uint64_t offs_base = 0, uint64_t cap2 = ~0ull, uint64_t cap4 = ~0ull);
 void launch_gather_u64(hipStream_t s, const uint64_t *src, const uint64_t *idx, uint32_t m, uint64_t *out);
 
 uint64_t scan_partials_needed(uint64_t n);
@@ -120,10 +120,11 @@ constexpr int kHistWords = 16 * 2048;
 struct LookbackWork {
     uint64_t *status_sort = nullptr;  // [tiles * 2048]
     uint64_t *status_uniq = nullptr;  // [tiles]
-    unsigned long long *counters = nullptr;  // [0] sort tiles, [1] unique tiles
+    uint64_t *status_scan = nullptr;  // [tiles] of the chained count scans
+    unsigned long long *counters = nullptr;  // [0] sort tiles, [1] unique tiles, [3] scan tiles
     uint32_t *hist = nullptr;         // [2 * kHistWords]
     uint32_t *err = nullptr;
-    unsigned long long claimed = 0, claimed_uniq = 0;
+    unsigned long long claimed = 0, claimed_uniq = 0, claimed_scan = 0;
     uint32_t epoch = 0;
     int hist_parity = 0;
     int dirty[2] = {0, 0};  // histogram words left non-zero in each buffer
@@ -138,7 +139,9 @@ uint32_t radix_tile(int bits);
 // [digit][tile] counts and a scatter kernel.  counts/offsets hold
 // tiles << bits words, partial the scan partials.
 uint64_t *launch_classic_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
-                              uint32_t *counts, uint32_t *offsets, uint32_t *partial);
+                              uint32_t *counts, uint32_t *offsets, LookbackWork &lw);
+// tiles of the chained scan of n words (status words needed)
+uint64_t scan_chained_tiles(uint64_t n);
 // Stable LSD sort of keys[0..n) on bits [lo, hi); returns keys or tmp,
 // whichever holds the result.
 uint64_t *launch_onesweep_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,

@@ -66,7 +66,9 @@ __global__ void k_build_lut(uint16_t *lut) {
 //   OR / AND of every child into or_and2 / or_and4 (varying-bit span used to
 //   size the radix passes).
 // WRITE = true: write children at the exclusive prefix `offs` (packed the
-//   same way) into out2 / out4.
+//   same way) into out2 / out4; indices at or past cap2 / cap4 are dropped
+//   (a write queued before its totals were known; the host then grows the
+//   buffers and writes again).
 template <bool WRITE>
 __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ par, uint64_t n_bound,
                                                  const uint64_t *__restrict__ n_ptr,
@@ -76,7 +78,8 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                                                  uint64_t *__restrict__ out2,
                                                  uint64_t *__restrict__ out4,
                                                  uint64_t *__restrict__ or_and2,
-                                                 uint64_t *__restrict__ or_and4) {
+                                                 uint64_t *__restrict__ or_and4, uint64_t cap2,
+                                                 uint64_t cap4) {
     extern __shared__ uint16_t lut[];
     __shared__ uint64_t red[4][16];
     stage_lut(lut, glut);
@@ -107,11 +110,11 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                         if (mv[d] != sp && is_pattern(mv[d], p)) {
                             const uint64_t c = canonical(mv[d], p.symm);
                             if (s == 1) {
-                                if (WRITE) out2[w2 + c2] = c;
+                                if (WRITE && w2 + c2 < cap2) out2[w2 + c2] = c;
                                 else { or2 |= c; and2 &= c; }
                                 ++c2;
                             } else {
-                                if (WRITE) out4[w4 + c4] = c;
+                                if (WRITE && w4 + c4 < cap4) out4[w4 + c4] = c;
                                 else { or4 |= c; and4 &= c; }
                                 ++c4;
                             }
@@ -638,6 +641,61 @@ __global__ __launch_bounds__(kUniqThreads) void k_unique_lb(const uint64_t *__re
         if ((f >> j) & 1u) out[pos++] = v[j];
 }
 
+// Exclusive scan of u32 counts in one launch: tiles are claimed in order by
+// a ticket, publish their sum, and take their prefix by decoupled look-back
+// (values stay below 2^42: the counts of one sort total at most n keys).
+__global__ __launch_bounds__(kScanThreads) void k_scan_chained(const uint32_t *__restrict__ in, uint64_t n,
+                                                               uint32_t *__restrict__ out,
+                                                               uint64_t *__restrict__ status,
+                                                               unsigned long long *__restrict__ tile_counter,
+                                                               unsigned long long tile_base, uint32_t epoch,
+                                                               uint32_t *__restrict__ err) {
+    __shared__ uint32_t tile_v[kScanTile];
+    __shared__ uint32_t scratch[kScanThreads / 64];
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_off;
+    if (threadIdx.x == 0) s_tile = (uint32_t)(atomicAdd(tile_counter, 1ull) - tile_base);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * kScanTile;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
+        tile_v[j * kScanThreads + threadIdx.x] = idx < n ? in[idx] : 0u;
+    }
+    __syncthreads();
+    uint32_t local[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        local[j] = sum;
+        sum += tile_v[threadIdx.x * kScanItems + j];
+    }
+    uint32_t total;
+    const uint32_t ex = block_exclusive_sum(sum, scratch, total);
+    if (threadIdx.x == 0) {
+        if (tile == 0) {
+            lb_store(status, lb_pack(epoch, 2, total));
+            s_off = 0;
+        } else {
+            lb_store(status + tile, lb_pack(epoch, 1, total));
+            const uint64_t pre = lb_lookback(status, 1, tile, epoch, err);
+            lb_store(status + tile, lb_pack(epoch, 2, pre + total));
+            s_off = (uint32_t)pre;
+        }
+    }
+    __syncthreads();
+    const uint32_t off = s_off + ex;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) tile_v[threadIdx.x * kScanItems + j] = off + local[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
+        if (idx < n) out[idx] = tile_v[j * kScanThreads + threadIdx.x];
+    }
+}
+
 // Classic unique: tile counts, scan, write.
 __global__ __launch_bounds__(kUniqThreads) void k_unique_count(const uint64_t *__restrict__ a, uint64_t n,
                                                                uint64_t *__restrict__ counts) {
@@ -989,17 +1047,18 @@ void launch_build_lut(hipStream_t s, uint16_t *lut) {
 void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                    const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
-                   uint64_t *or_and4, uint64_t offs_base) {
+                   uint64_t *or_and4, uint64_t offs_base, uint64_t cap2, uint64_t cap4) {
     if (n_bound == 0) return;
     const uint32_t want = blocks_for(n_bound, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     const size_t lds = kLutEntries * sizeof(uint16_t);
     if (write)
         hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
-                           do_check, counts, offs, offs_base, out2, out4, or_and2, or_and4);
+                           do_check, counts, offs, offs_base, out2, out4, or_and2, or_and4, cap2, cap4);
     else
         hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
-                           do_check, counts, offs, (uint64_t)0, out2, out4, or_and2, or_and4);
+                           do_check, counts, offs, (uint64_t)0, out2, out4, or_and2, or_and4, (uint64_t)0,
+                           (uint64_t)0);
 }
 
 template <typename TI, typename TO>
@@ -1056,16 +1115,26 @@ static uint64_t *onesweep_impl(hipStream_t s, uint64_t *keys, uint64_t *tmp, uin
     return src;
 }
 
+uint64_t scan_chained_tiles(uint64_t n) { return blocks_for(n ? n : 1, kScanTile); }
+
+static void launch_scan_chained(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out, LookbackWork &lw) {
+    const uint32_t tiles = (uint32_t)scan_chained_tiles(n);
+    const uint32_t epoch = lw.next_epoch();
+    hipLaunchKernelGGL(k_scan_chained, dim3(tiles), dim3(kScanThreads), 0, s, in, n, out, lw.status_scan,
+                       lw.counters + 3, lw.claimed_scan, epoch, lw.err);
+    lw.claimed_scan += tiles;
+}
+
 template <int BITS>
 static void classic_pass_impl(hipStream_t s, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
-                              uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
+                              uint32_t *counts, uint32_t *offsets, LookbackWork &lw) {
     const uint32_t ntiles = blocks_for(n, kSortTile);
     {
         Timer t(s, "radix_hist", 8.0 * (double)n);  // reads every key once
         hipLaunchKernelGGL((k_radix_hist<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, n,
                            shift, counts, ntiles);
     }
-    launch_scan_u32(s, counts, (uint64_t)ntiles << BITS, offsets, partial);
+    launch_scan_chained(s, counts, (uint64_t)ntiles << BITS, offsets, lw);
     {
         Timer t(s, "radix_scatter", 16.0 * (double)n);  // reads and writes every key once
         hipLaunchKernelGGL((k_radix_scatter<BITS, kSortItems>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst,
@@ -1074,15 +1143,15 @@ static void classic_pass_impl(hipStream_t s, const uint64_t *src, uint64_t *dst,
 }
 
 uint64_t *launch_classic_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
-                              uint32_t *counts, uint32_t *offsets, uint32_t *partial) {
+                              uint32_t *counts, uint32_t *offsets, LookbackWork &lw) {
     if (n < 2 || hi <= lo) return keys;
     uint64_t *src = keys, *dst = tmp;
     for (int shift = lo; shift < hi; shift += bits) {
         switch (bits) {
-            case 4: classic_pass_impl<4>(s, src, dst, n, shift, counts, offsets, partial); break;
-            case 6: classic_pass_impl<6>(s, src, dst, n, shift, counts, offsets, partial); break;
-            case 11: classic_pass_impl<11>(s, src, dst, n, shift, counts, offsets, partial); break;
-            default: classic_pass_impl<8>(s, src, dst, n, shift, counts, offsets, partial); break;
+            case 4: classic_pass_impl<4>(s, src, dst, n, shift, counts, offsets, lw); break;
+            case 6: classic_pass_impl<6>(s, src, dst, n, shift, counts, offsets, lw); break;
+            case 11: classic_pass_impl<11>(s, src, dst, n, shift, counts, offsets, lw); break;
+            default: classic_pass_impl<8>(s, src, dst, n, shift, counts, offsets, lw); break;
         }
         std::swap(src, dst);
     }

@@ -214,16 +214,32 @@ int choose_radix_bits(int span, int requested) {
     return p11 < p8 ? 11 : 8;
 }
 
-void SortWork::ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles) {
+void SortWork::ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles,
+                               uint64_t scan_tiles) {
     if (!lb_misc.p) {
         // counters (2 x u64), error flag, double histogram
         const size_t bytes = 64 + (size_t)2 * kHistWords * 4;
         lb_misc.alloc(bytes, s);
         HIPCHK(hipMemsetAsync(lb_misc.p, 0, bytes, s));
+        // counters[0], [1] at bytes 0-15, error flag at 16, counters[3] at 24
         lw.counters = (unsigned long long *)lb_misc.p;
         lw.err = (uint32_t *)((char *)lb_misc.p + 16);
         lw.hist = (uint32_t *)((char *)lb_misc.p + 64);
-        lw.claimed = lw.claimed_uniq = 0;
+        lw.claimed = lw.claimed_uniq = lw.claimed_scan = 0;
+    }
+    if (lw.epoch > (1u << 20) - 64) {
+        // status words are told apart by a 20-bit epoch: clear them before
+        // an epoch value could come round again
+        if (lb_status_sort.p) HIPCHK(hipMemsetAsync(lb_status_sort.p, 0, lb_status_sort.bytes, s));
+        if (lb_status_uniq.p) HIPCHK(hipMemsetAsync(lb_status_uniq.p, 0, lb_status_uniq.bytes, s));
+        if (lb_status_scan.p) HIPCHK(hipMemsetAsync(lb_status_scan.p, 0, lb_status_scan.bytes, s));
+        lw.epoch = 0;
+    }
+    const size_t want_scan = (size_t)scan_tiles * 8;
+    if (want_scan > lb_status_scan.bytes) {
+        lb_status_scan.alloc(want_scan + want_scan / 2 + 64, s);
+        HIPCHK(hipMemsetAsync(lb_status_scan.p, 0, lb_status_scan.bytes, s));
+        lw.status_scan = (uint64_t *)lb_status_scan.p;
     }
     const size_t want_sort = (size_t)sort_tiles * (size_t)bins * 8;
     if (want_sort > lb_status_sort.bytes) {
@@ -290,9 +306,9 @@ uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, u
         const uint64_t nc = ntiles << bits;
         w.counts.ensure(nc * 4, s);
         w.offsets.ensure(nc * 4, s);
-        w.partial.ensure(scan_partials_needed(nc) * 8 + 64, s);
+        w.ensure_lookback(s, 0, 0, 0, scan_chained_tiles(nc));
         out = launch_classic_sort(s, bits, keys, tmp, n, lo, hi, (uint32_t *)w.counts.p, (uint32_t *)w.offsets.p,
-                                  (uint32_t *)w.partial.p);
+                                  w.lw);
     }
     HIPCHK(hipGetLastError());
     return out;
@@ -1180,11 +1196,33 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         return tot;
     };
 
+    // The sizes of step i (copied to H after its count pass, marked by
+    // meta_ev) are read while the device already runs the step's write,
+    // queued speculatively into the candidate buffers as they are; a step
+    // whose totals outgrow them grows them and writes again.
+    hipEvent_t meta_ev;
+    HIPCHK(hipEventCreateWithFlags(&meta_ev, hipEventDisableTiming));
+    struct EventGuard {
+        hipEvent_t e;
+        ~EventGuard() { (void)hipEventDestroy(e); }
+    } meta_ev_guard{meta_ev};
+    auto queue_meta = [&](int lo, int hi) {
+        HIPCHK(hipMemcpyAsync(H + lo, res.meta_dev() + lo, (size_t)(hi - lo + 1) * sizeof(LayerDev),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(meta_ev, s));
+    };
+    bool spec_written = false;
     uint64_t c4_prev = 0;
-    if (steps >= 3) launch_count(1, n_seeds);
+    if (steps >= 3) {
+        launch_count(1, n_seeds);
+        queue_meta(0, 2);
+    }
     for (int i = 1; i <= steps - 2; ++i) {
         progress_set((uint32_t)i, prog_total);
-        read_meta(s, res, H, i - 1, i + 1);
+        // poll: the host must wake as soon as the sizes land, while the
+        // device is still busy with the speculative write
+        for (hipError_t q; (q = hipEventQuery(meta_ev)) != hipSuccess;)
+            if (q != hipErrorNotReady) HIPCHK(q);
         const uint64_t n_par = H[i - 1].n_raw;
         if (i > 1) res.raw_arena.trim_last(res.raw[i - 1], n_par * 8);
         res.info[i - 1].n_raw = n_par;
@@ -1209,15 +1247,29 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         uint64_t bound_next;
         if (!chunked) {
             DBuf &cur = cand[i & 1], &nxt = cand[(i + 1) & 1];
+            bool write = !spec_written;
+            // candidate buffers keep a quarter of headroom over this step so
+            // the next speculative write usually fits
+            const uint64_t room = n_c + n_c / 4 + 64;
             if (cur.bytes < n_c * 8) {  // grow, keeping the spawn-4 prefix
                 DBuf g;
-                g.alloc(n_c * 8 + n_c * 4 + 256, s);
+                g.alloc(room * 8 + 256, s);
                 if (c4_prev) HIPCHK(hipMemcpyAsync(g.p, cur.p, c4_prev * 8, hipMemcpyDeviceToDevice, s));
                 cur = std::move(g);
+                write = true;
             }
-            nxt.ensure(c4 * 8, s);
+            if (nxt.bytes < c4 * 8) {
+                nxt.alloc(std::max(c4, room) * 8 + 256, s);
+                write = true;
+            } else if (nxt.bytes < room * 8) {  // headroom for the next step, keeping what is written
+                DBuf g;
+                g.alloc(room * 8 + 256, s);
+                if (!write && c4) HIPCHK(hipMemcpyAsync(g.p, nxt.p, c4 * 8, hipMemcpyDeviceToDevice, s));
+                nxt = std::move(g);
+            }
             sort_tmp.ensure(n_c * 8, s);
-            if (n_par && (c2 || c4)) {
+            if (spec_written && !write) stats_add_bytes("expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
+            if (write && n_par && (c2 || c4)) {
                 Timer t(s, "expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
                 launch_expand(s, true, C.shape, res.raw[i - 1], n_par, nullptr, C.lut, p, do_check, nullptr,
                               (const uint64_t *)offs.p, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, nullptr,
@@ -1274,7 +1326,22 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             runs_next.clear();
             runs_next_n.clear();
         }
-        if (i + 1 <= steps - 2) launch_count(i + 1, bound_next);
+        spec_written = false;
+        if (i + 1 <= steps - 2) {
+            launch_count(i + 1, bound_next);
+            queue_meta(i, i + 2);
+            if (!chunked && bound_next) {
+                // step i+1's write, sized on the device; writes past the
+                // buffers are dropped and redone once the totals are known
+                const DBuf &cur1 = cand[(i + 1) & 1], &nxt1 = cand[i & 1];
+                const uint64_t cap2 = cur1.bytes / 8 > c4 ? cur1.bytes / 8 - c4 : 0, cap4 = nxt1.bytes / 8;
+                Timer t(s, "expand", 0.0);
+                launch_expand(s, true, C.shape, res.raw[i], bound_next, &M[i].n_raw, C.lut, p,
+                              i + 1 > o.docheck_step ? 1 : 0, nullptr, (const uint64_t *)offs.p,
+                              (uint64_t *)cur1.p + c4, (uint64_t *)nxt1.p, nullptr, nullptr, 0, cap2, cap4);
+                spec_written = true;
+            }
+        }
         HIPCHK(hipGetLastError());
         c4_prev = c4;
     }
