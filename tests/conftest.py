@@ -42,8 +42,9 @@ def golden_tags():
 
 
 def book_bytes(keys, rates):
-    """Packed reference records {u64 board; f32 rate} (12 bytes each)."""
-    rec = np.empty(keys.size, dtype=np.dtype([("board", "<u8"), ("rate", "<f4")]))
+    """Packed reference records {u64 board; rate} in the rates' dtype (12 or
+    16 bytes each)."""
+    rec = np.empty(keys.size, dtype=np.dtype([("board", "<u8"), ("rate", rates.dtype.newbyteorder("<"))]))
     rec["board"] = keys
     rec["rate"] = rates
     return rec.tobytes()
@@ -56,4 +57,4 @@ def book_sha(keys, rates):
 def golden_params(g):
     return dict(seeds=[int(s, 16) for s in g["seeds"]], masks=[int(m, 16) for m in g["masks"]],
                 shifts=list(g["shifts"]), symm=g["symm"], target=g["target"], steps=g["steps"],
-                docheck=g["docheck"])
+                docheck=g["docheck"], dtype=g.get("dtype", "float32"))

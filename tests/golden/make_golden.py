@@ -7,8 +7,11 @@ writes tests/golden/<pattern>_<tile>.json holding, per layer:
   * ``n``      -- raw layer size N_i from the reference's stats.txt
                   (solved layers) -- absent for the two final layers;
   * ``live``   -- number of records in <i>.book (rate > 0);
-  * ``sha256`` -- SHA-256 of the .book bytes (12-byte {u64, f32} records);
+  * ``sha256`` -- SHA-256 of the .book bytes ({u64 board; rate} records,
+                  the rate in the run's success_rate_dtype);
   * ``sample`` -- up to 16 evenly spaced records as [board_hex, rate_bits].
+
+Runs in a dtype other than float32 are written to <pattern>_<tile>_<dtype>.json.
 
 Usage:  python tests/golden/make_golden.py [name ...]
 """
@@ -32,11 +35,15 @@ from oracle.ref import ref  # noqa: E402
 
 patterns = importlib.import_module("2048endgametablebase_amd.patterns")
 
-# (pattern, log2 target); the full-symmetry case uses the free8 spec below.
+# (pattern, log2 target[, dtype]); the full-symmetry case uses the free8
+# spec below.  uint32 is the reference's default success_rate_dtype.
 RUNS = [
     ("L3", 7), ("L3", 8), ("442", 7), ("442", 8), ("442", 9), ("t", 8), ("LL", 7),
     ("4431", 7), ("LL", 6), ("444", 6), ("4432", 6), ("free8", 6), ("t", 6),
+    ("442", 7, "uint32"), ("L3", 7, "uint32"), ("LL", 6, "uint32"), ("442", 7, "float64"),
+    ("442", 7, "uint64"),
 ]
+RATE_FORMAT = {"float32": "<f4", "uint32": "<u4", "float64": "<f8", "uint64": "<u8"}
 
 # free8 (docs_and_configs/patterns_config.json): no masks, every nibble a
 # success position, full symmetry, single seed.  Built from its seed as is.
@@ -55,18 +62,19 @@ def params(name, t):
                 steps=p["steps"], docheck=p["docheck_step"], is_free=p["is_free"])
 
 
-def digest_run(name, t, workdir):
+def digest_run(name, t, workdir, dtype="float32"):
     p = params(name, t)
     tile = 2 ** t
     prefix = os.path.join(workdir, f"{name}_{tile}_")
     ref.run_build(p["seeds"], p["masks"], p["shifts"], p["symm"], t, p["steps"], p["docheck"],
-                  prefix, is_free=p["is_free"], threads=8)
+                  prefix, is_free=p["is_free"], dtype=dtype, threads=8)
     lengths = {}
     with open(prefix + "stats.txt") as fh:
         for row in list(csv.reader(fh))[1:]:
             lengths[int(row[0])] = int(row[1])
     layers = []
-    rec = np.dtype([("board", "<u8"), ("rate", "<f4")])
+    rec = np.dtype([("board", "<u8"), ("rate", RATE_FORMAT[dtype])])
+    bits = np.uint32 if rec["rate"].itemsize == 4 else np.uint64
     for i in range(p["steps"]):
         path = prefix + f"{i}.book"
         raw = open(path, "rb").read() if os.path.exists(path) else b""
@@ -77,14 +85,14 @@ def digest_run(name, t, workdir):
             "n": lengths.get(i),
             "live": int(a.size),
             "sha256": hashlib.sha256(raw).hexdigest(),
-            "sample": [[f"{int(a['board'][j]):016x}", int(a["rate"][j:j + 1].view(np.uint32)[0])]
+            "sample": [[f"{int(a['board'][j]):016x}", int(a["rate"][j:j + 1].view(bits)[0])]
                        for j in idx] if a.size else [],
         })
     return {
         "pattern": name, "target": t, "steps": p["steps"], "docheck": p["docheck"],
         "seeds": [f"{s:016x}" for s in p["seeds"]], "masks": [f"{m:x}" for m in p["masks"]],
         "shifts": list(p["shifts"]), "symm": p["symm"], "spawn_rate4": 0.1,
-        "dtype": "float32", "source": "reference run_pattern_build_cpp, g++ -Ofast "
+        "dtype": dtype, "source": "reference run_pattern_build_cpp, g++ -Ofast "
         "-march=x86-64-v3 -flto (oracle/ref/Makefile)", "layers": layers,
     }
 
@@ -93,13 +101,15 @@ def main(argv):
     if not ref.available() and not ref.build():
         raise SystemExit("reference sources not present; cannot make golden vectors")
     want = set(argv)
-    for name, t in RUNS:
-        tag = f"{name}_{2 ** t}"
+    for run in RUNS:
+        name, t = run[0], run[1]
+        dtype = run[2] if len(run) > 2 else "float32"
+        tag = f"{name}_{2 ** t}" + ("" if dtype == "float32" else f"_{dtype}")
         if want and tag not in want and name not in want:
             continue
         work = tempfile.mkdtemp(prefix="egtb_golden_", dir="/tmp")
         try:
-            out = digest_run(name, t, work)
+            out = digest_run(name, t, work, dtype)
         finally:
             shutil.rmtree(work, ignore_errors=True)
         with open(os.path.join(os.path.dirname(__file__), f"{tag}.json"), "w") as fh:

@@ -50,7 +50,7 @@ def test_matches_reference_golden(pkg, tag):
     g = load_golden(tag)
     gp = golden_params(g)
     print(f"case golden {tag}", flush=True)
-    res = _build(pkg, gp)
+    res = _build(pkg, gp, dtype=gp["dtype"])
     try:
         bad = []
         for layer in g["layers"]:
@@ -110,6 +110,19 @@ def test_start_build_writes_reference_layout(pkg, tmp_path):
             assert by_layer[layer["i"]] == layer["n"]
     cfg_txt = open(prefix + "config.txt").read()
     assert "success_rate_dtype: float32" in cfg_txt and "4_spawn_rate: 0.1" in cfg_txt
+
+
+def test_start_build_default_dtype_is_uint32(pkg, tmp_path):
+    """With the default configuration (success_rate_dtype uint32, the
+    reference's default) start_build writes the reference's uint32 books."""
+    cfg = pkg.config.SingletonConfig().config
+    assert cfg["success_rate_dtype"] == "uint32"
+    prefix = str(tmp_path / "442_128_")
+    assert pkg.start_build("442", 7, prefix) is True
+    g = load_golden("442_128_uint32")
+    for layer in g["layers"]:
+        data = open(prefix + f"{layer['i']}.book", "rb").read()
+        assert hashlib.sha256(data).hexdigest() == layer["sha256"], layer["i"]
 
 
 def test_dedup_primitives_match_numpy(pkg):
