@@ -78,7 +78,13 @@ def std_round(x):
     return t
 
 
-def find_in_file(path, key, rec_fmt="<Qf"):
+# record layout and max scale per dtype (FormationRuntime.h:197-200,
+# ReaderRuntime.cpp:183-206: integer rates are divided by their max scale)
+REC_FORMAT = {"float32": ("<Qf", 1.0), "float64": ("<Qd", 1.0), "uint32": ("<QI", 4e9),
+              "uint64": ("<QQ", 1.6e18)}
+
+
+def find_in_file(path, key, rec_fmt="<Qf", scale=1.0):
     size = struct.calcsize(rec_fmt)
     with open(path, "rb") as fh:
         fh.seek(0, os.SEEK_END)
@@ -89,7 +95,7 @@ def find_in_file(path, key, rec_fmt="<Qf"):
             fh.seek(mid * size)
             board, rate = struct.unpack(rec_fmt, fh.read(size))
             if board == key:
-                return float(rate)
+                return float(rate) / scale if scale > 1.0 else float(rate)
             if board < key:
                 left = mid + 1
             else:
@@ -126,7 +132,7 @@ def move_on_dic(board, folder, pattern_full, nums_adjust, masks, symm, dtype="fl
             moved = orc.move(enc, index)
             if moved == enc or not is_pattern(moved, masks):
                 continue
-            val = find_in_file(path, orc.canonical(moved, symm))
+            val = find_in_file(path, orc.canonical(moved, symm), *REC_FORMAT[dtype])
             if "32" in dtype and abs(val) > 1e-7:
                 val = std_round(val * 1e9) / 1e9
             vals[{0: 2, 1: 1, 2: 3, 3: 0}[index]] = val

@@ -3,6 +3,7 @@ against the CPU restatement of the reference reader (on-disk binary
 search), the fallback against its definition, and table sources agreeing."""
 import importlib
 import math
+import os
 
 import numpy as np
 import pytest
@@ -163,3 +164,38 @@ def test_prefer_max_matches_reference_reader(pkg, built):
         differs_from_first += first != want
     assert differs_from_first > 0  # the rule is exercised
     t.close()
+
+
+@pytest.fixture(scope="module")
+def built_u32(pkg, tmp_path_factory):
+    """The same tables written by start_build with the default configuration
+    (uint32 rates, the reference's default)."""
+    cfg = pkg.config.SingletonConfig().config
+    assert cfg["success_rate_dtype"] == "uint32"
+    folder = tmp_path_factory.mktemp("tables_u32")
+    prefix = str(folder / f"{PATTERN}_{2**TARGET}_")
+    pkg.start_build(PATTERN, TARGET, prefix)
+    yield str(folder), prefix
+
+
+def test_move_on_dic_uint32_tables(pkg, built_u32):
+    """Readers over uint32 files: rates scaled by 4e9 and rounded as the
+    reference reader does, both through BookReader and find_classic_value."""
+    folder, prefix = built_u32
+    meta = pkg.patterns.pattern_catalog[PATTERN]
+    t = pkg.Table.open(prefix, PATTERN, TARGET, "uint32")
+    reader_mod = importlib.import_module("2048endgametablebase_amd.BookReader")
+    boards = sample_boards(prefix, pkg.patterns.build_params(PATTERN, TARGET)["steps"], 300, 9)
+    got = t.move_on_dic_batch([reader_mod.encode_board(m) for m in boards])
+    n_numeric = 0
+    for m, g in zip(boards, got):
+        want = ro.move_on_dic(m, folder, f"{PATTERN}_{2**TARGET}", meta.nums_adjust, list(meta.pattern_masks),
+                              meta.symm_mode, dtype="uint32")
+        assert g == want, (m.tolist(), g, want)
+        n_numeric += want[1] != ""
+    assert n_numeric > 150
+    t.close()
+    keys, _ = orc.read_book(prefix + "40.book")
+    for k in keys[:: max(1, keys.size // 50)].tolist():
+        want = ro.find_in_file(prefix + "40.book", k, *ro.REC_FORMAT["uint32"])
+        assert pkg.find_classic_value(folder, os.path.basename(prefix) + "40.book", k, "uint32") == want
