@@ -106,6 +106,7 @@ void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint6
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
                    uint64_t *or_and4, uint64_t offs_base = 0, uint64_t cap2 = ~0ull, uint64_t cap4 = ~0ull);
 void launch_gather_u64(hipStream_t s, const uint64_t *src, const uint64_t *idx, uint32_t m, uint64_t *out);
+void launch_copy_u64(hipStream_t s, const uint64_t *src, uint64_t *dst);
 
 uint64_t scan_partials_needed(uint64_t n);
 void launch_scan_u64(hipStream_t s, const uint64_t *in, uint64_t n_bound, const uint64_t *n_ptr, uint64_t *out,

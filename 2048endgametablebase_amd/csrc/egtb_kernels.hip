@@ -1013,6 +1013,10 @@ __global__ void k_gather_u64(const uint64_t *__restrict__ src, const uint64_t *_
     if (k < m) out[k] = src[idx[k]];
 }
 
+// *dst = *src, one thread: a device-side count into host-mapped memory, so
+// the host reads it after an event without a copy of its own.
+__global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst) { *dst = *src; }
+
 // Packed records -> SoA (the reader's table load).
 __global__ void k_unpack_records(const uint32_t *__restrict__ recs, uint64_t n, int words, uint64_t *__restrict__ keys,
                                  uint32_t *__restrict__ rates) {
@@ -1262,6 +1266,10 @@ void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, ui
 
 void launch_gather_u64(hipStream_t s, const uint64_t *src, const uint64_t *idx, uint32_t m, uint64_t *out) {
     if (m) hipLaunchKernelGGL(k_gather_u64, dim3((m + 255) / 256), dim3(256), 0, s, src, idx, m, out);
+}
+
+void launch_copy_u64(hipStream_t s, const uint64_t *src, uint64_t *dst) {
+    hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(1), 0, s, src, dst);
 }
 
 void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int rate_size, uint64_t *keys,

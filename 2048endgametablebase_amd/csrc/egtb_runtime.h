@@ -131,6 +131,37 @@ class Arena {
     hipStream_t stream_ = nullptr;
 };
 
+// Device memory of the books in host-tables mode: large chunks handed out
+// front to back, a chunk whose books have all gone to the host being reused
+// for later books. A build frees a few dozen chunks at its end instead of
+// three stream-ordered frees per layer, which cost seconds on the pool
+// (most take microseconds, one in a few hundred stalls for ~100 ms).
+// Build thread only.
+class BookRing {
+  public:
+    explicit BookRing(size_t chunk = (size_t)2 << 30) : chunk_(chunk) {}
+    BookRing(const BookRing &) = delete;
+    BookRing &operator=(const BookRing &) = delete;
+    ~BookRing() { free_all(); }
+    void reset(hipStream_t s, int layers);
+    void *alloc(int layer, size_t bytes);
+    void release(int layer);  // every array of the layer is no longer read
+    void free_all();
+
+  private:
+    struct Chunk {
+        char *p;
+        size_t cap, used;
+        int live;  // arrays handed out and not yet released
+    };
+    std::vector<Chunk> chunks_;
+    std::vector<int> spare_;                  // chunks with no live array
+    std::vector<std::vector<int>> of_layer_;  // chunk of each array, per layer
+    int cur_ = -1;
+    size_t chunk_;
+    hipStream_t stream_ = nullptr;
+};
+
 // Per-layer device record: sizes, key span and stats of layer i.
 struct LayerDev {
     uint64_t n_raw;    // unique raw size N_i
@@ -158,23 +189,30 @@ struct LayerInfo {
     uint64_t n_raw = 0, n_live = 0, c2 = 0, c4 = 0;
 };
 
-// Page-aligned host memory mapped on first write (no zero-fill pass), with
-// transparent huge pages requested so that filling it costs few page faults.
-struct HostArray {
+// Host memory that streamed books are copied into: anonymous pages mapped
+// on first write, with transparent huge pages requested so that faulting
+// them in costs few faults. `used` is the part handed out.
+struct HostSlab {
     char *p = nullptr;
-    size_t bytes = 0;
-    HostArray() = default;
-    HostArray(const HostArray &) = delete;
-    HostArray &operator=(const HostArray &) = delete;
-    HostArray(HostArray &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
-    HostArray &operator=(HostArray &&o) noexcept {
+    size_t bytes = 0, used = 0;
+    HostSlab() = default;
+    HostSlab(const HostSlab &) = delete;
+    HostSlab &operator=(const HostSlab &) = delete;
+    HostSlab(HostSlab &&o) noexcept { *this = std::move(o); }
+    HostSlab &operator=(HostSlab &&o) noexcept {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(used, o.used);
         return *this;
     }
-    ~HostArray() { reset(); }
-    void allocate(size_t n);
-    void reset();
+    ~HostSlab();
+    void map(size_t n);  // throws Error(EGTB_ERR_MEMORY)
+};
+
+// One streamed array (a range of a HostSlab).
+struct HostView {
+    char *p = nullptr;
+    size_t bytes = 0;
 };
 
 struct Result {
@@ -192,10 +230,11 @@ struct Result {
     // host-tables mode: books live in per-layer device buffers until they are
     // no longer needed, then in host memory
     bool host_tables = false;
-    std::vector<DBuf> book_bufs;  // 3 per layer: keys, rates, index
+    BookRing book_ring;           // books' device memory until they are on the host
     std::vector<DBuf> hash_bufs;  // per layer, freed after the layer's last solve
     double hash_max_bytes = 0;    // larger layers are searched, not hashed
-    std::vector<HostArray> host_keys, host_rates;
+    std::vector<HostSlab> host_slabs;            // own the memory of host_keys / host_rates
+    std::vector<HostView> host_keys, host_rates;  // per layer
     uint64_t streamed_bytes = 0, chunked_steps = 0, chunks = 0, peak_device_bytes = 0;
     double stream_ms = 0;
     LayerDev *meta_dev() const { return (LayerDev *)meta.p; }

@@ -4,6 +4,7 @@
 // and the C ABI declared in include/egtb.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -533,6 +534,69 @@ void Arena::release() {
     chunks_.clear();
 }
 
+// ---------------------------------------------------------------------------
+// BookRing
+// ---------------------------------------------------------------------------
+
+void BookRing::reset(hipStream_t s, int layers) {
+    free_all();
+    stream_ = s;
+    of_layer_.assign((size_t)layers, {});
+}
+
+void *BookRing::alloc(int layer, size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (cur_ < 0 || chunks_[cur_].cap - chunks_[cur_].used < bytes) {
+        // leave the current chunk; it is spare already if its arrays are gone
+        if (cur_ >= 0 && chunks_[cur_].live == 0) {
+            chunks_[cur_].used = 0;
+            spare_.push_back(cur_);
+        }
+        cur_ = -1;
+        for (size_t k = 0; k < spare_.size(); ++k)
+            if (chunks_[spare_[k]].cap >= bytes) {
+                cur_ = spare_[k];
+                spare_.erase(spare_.begin() + (std::ptrdiff_t)k);
+                break;
+            }
+        if (cur_ < 0) {
+            const size_t cap = std::max(bytes, chunk_);
+            void *p = nullptr;
+            const hipError_t e = hipMallocAsync(&p, cap, stream_);
+            if (e != hipSuccess)
+                throw Error(EGTB_ERR_MEMORY, "device allocation of " + std::to_string(cap) +
+                                                 " bytes for streamed books failed: " + hipGetErrorString(e));
+            chunks_.push_back(Chunk{(char *)p, cap, 0, 0});
+            cur_ = (int)chunks_.size() - 1;
+        }
+    }
+    Chunk &c = chunks_[cur_];
+    void *p = c.p + c.used;
+    c.used += bytes;
+    ++c.live;
+    of_layer_[(size_t)layer].push_back(cur_);
+    return p;
+}
+
+void BookRing::release(int layer) {
+    for (int k : of_layer_[(size_t)layer]) {
+        Chunk &c = chunks_[k];
+        if (--c.live == 0 && k != cur_) {
+            c.used = 0;
+            spare_.push_back(k);
+        }
+    }
+    of_layer_[(size_t)layer].clear();
+}
+
+void BookRing::free_all() {
+    for (auto &c : chunks_) (void)hipFreeAsync(c.p, stream_);
+    chunks_.clear();
+    spare_.clear();
+    cur_ = -1;
+    for (auto &v : of_layer_) v.clear();
+}
+
 size_t Arena::used() const {
     size_t u = 0;
     for (auto &c : chunks_) u += c.used;
@@ -602,9 +666,8 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     bk.n_dev = &md->n_live;
     auto carve = [&](size_t bytes, int which) -> void * {
         if (!res.host_tables) return res.book_arena.alloc(bytes);
-        DBuf &d = res.book_bufs[(size_t)layer * 3 + which];
-        d.alloc(bytes + 256, s);
-        return d.p;
+        (void)which;
+        return res.book_ring.alloc(layer, bytes + 256);
     };
     bk.keys = (uint64_t *)carve(bound * 8, 0);
     bk.rates = carve(bound * sizeof(T), 1);
@@ -628,20 +691,17 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     }
 }
 
-void HostArray::allocate(size_t n) {
-    reset();
-    if (!n) return;
+void HostSlab::map(size_t n) {
     void *q = mmap(nullptr, n, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
-    if (q == MAP_FAILED) throw Error(EGTB_ERR_MEMORY, "host memory for a streamed layer (" + std::to_string(n) + " B)");
+    if (q == MAP_FAILED)
+        throw Error(EGTB_ERR_MEMORY, "host memory for streamed books (" + std::to_string(n) + " B)");
     (void)madvise(q, n, MADV_HUGEPAGE);
     p = (char *)q;
     bytes = n;
 }
 
-void HostArray::reset() {
+HostSlab::~HostSlab() {
     if (p) (void)munmap(p, bytes);
-    p = nullptr;
-    bytes = 0;
 }
 
 // memcpy split across a fixed set of worker threads plus the caller.
@@ -658,6 +718,7 @@ class CopyPool {
         cv_.notify_all();
         for (auto &t : th_) t.join();
     }
+    // src == nullptr writes one byte per 4 KiB page (faults the range in)
     void copy(char *dst, const char *src, size_t bytes) {
         const int parts = (int)th_.size() + 1;
         {
@@ -675,7 +736,12 @@ class CopyPool {
     void part(int k, int parts) {
         const size_t per = ((bytes_ + parts - 1) / parts + 4095) & ~(size_t)4095;
         const size_t a = std::min(bytes_, per * (size_t)k), b = std::min(bytes_, a + per);
-        if (b > a) std::memcpy(dst_ + a, src_ + a, b - a);
+        if (b <= a) return;
+        if (src_) {
+            std::memcpy(dst_ + a, src_ + a, b - a);
+        } else {
+            for (size_t off = a; off < b; off += 4096) dst_[off] = 0;
+        }
     }
     void loop(int k) {
         uint64_t seen = 0;
@@ -702,22 +768,38 @@ class CopyPool {
     bool stop_ = false;
 };
 
-// Host-tables mode: a copy thread moves each finished book (keys + rates)
-// to host memory after the event that marks the book's last use, then frees
-// its device buffers. Device-to-host copies run on the thread's own stream
-// into a ring of pinned staging slots; the host drains slot k (a parallel
-// memcpy into the destination) while the copies into the later slots are in
-// flight.
+// Host-tables mode: every finished book (keys + rates) is copied to host
+// memory once the event that marks its last use on the device has passed.
+//  - The forward pass reserves each layer's upper bound (its raw size); a
+//    fault thread maps host slabs for it and faults their pages in while the
+//    device is still busy, so the copies land in resident pages.
+//  - Retiring a book stores its live size into host-mapped memory from the
+//    build stream. The issue thread waits for the retire event, takes that
+//    many records from the slabs and queues the book's arrays, in pieces, as
+//    device-to-host copies into a ring of page-locked staging slots on a
+//    stream of its own; it only waits when every slot is still being
+//    drained. (A size read by a copy of its own queues behind the copies
+//    already in flight and stalls the pipeline at every book.)
+//  - The drain thread takes the pieces in order, waits for each copy and
+//    moves the slot into the book's host memory with a thread pool.
+//  - The device memory of a book that has arrived goes back to the build
+//    thread (reclaim()).
+// Page-locking the destinations instead costs the device time while the
+// pages are mapped for it, and the system caps how much may be locked.
 class BookStreamer {
   public:
-    BookStreamer(Result &res, int rate_size) : res_(res), rate_size_(rate_size), pool_(kCopyWorkers) {
+    BookStreamer(Result &res, int rate_size, int steps)
+        : res_(res), rate_size_(rate_size), copy_pool_(kCopyWorkers), fault_pool_(kFaultWorkers) {
         HIPCHK(hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking));
+        HIPCHK(hipHostMalloc((void **)&sizes_, (size_t)steps * 8, hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHK(hipHostGetDevicePointer((void **)&sizes_dev_, sizes_, 0));
         for (int k = 0; k < kSlots; ++k) {
             HIPCHK(hipHostMalloc(&stage_[k], kStage, hipHostMallocDefault));
             HIPCHK(hipEventCreateWithFlags(&slot_ev_[k], hipEventDisableTiming));
         }
-        HIPCHK(hipHostMalloc((void **)&n_host_, 8, hipHostMallocDefault));
-        th_ = std::thread([this] { run(); });
+        issue_th_ = std::thread([this] { issue_loop(); });
+        drain_th_ = std::thread([this] { drain_loop(); });
+        fault_th_ = std::thread([this] { fault_loop(); });
     }
     ~BookStreamer() {
         try {
@@ -726,122 +808,309 @@ class BookStreamer {
         }
     }
 
-    // book `layer`'s device buffers are read by work recorded before `ev`
+    // a book of at most `records` records will be streamed
+    void reserve(uint64_t records) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            // 1/16 more for the slab tails that are too short for a book's next array
+            const uint64_t b = records * (8 + (uint64_t)rate_size_);
+            want_ += b + b / 16 + 2 * kAlign;
+        }
+        fault_cv_.notify_one();
+    }
+
+    // book `layer`'s device buffers are read by work queued on s so far
     void retire(int layer, hipStream_t s) {
         hipEvent_t ev;
+        launch_copy_u64(s, res_.books[layer].n_dev, sizes_dev_ + layer);
         HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         HIPCHK(hipEventRecord(ev, s));
         {
             std::lock_guard<std::mutex> lk(mu_);
-            q_.push_back({layer, ev});
+            jobs_.push_back({layer, ev});
         }
-        cv_.notify_one();
+        cv_.notify_all();
     }
 
+    // frees the device memory of the books that are on the host (build thread)
+    void reclaim() {
+        std::vector<int> done;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            done.swap(arrived_);
+        }
+        for (int layer : done) {
+            res_.book_ring.release(layer);
+            Book &bk = res_.books[layer];
+            bk.keys = nullptr;
+            bk.rates = nullptr;
+            bk.idx = nullptr;
+        }
+    }
+
+    // waits for every retired book; slabs no book reached are unmapped
     void finish() {
-        if (!th_.joinable()) return;
+        if (!issue_th_.joinable()) return;
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
         }
-        cv_.notify_one();
-        th_.join();
+        cv_.notify_all();
+        issue_th_.join();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            issue_done_ = true;
+        }
+        cv_.notify_all();
+        drain_th_.join();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fault_stop_ = true;
+        }
+        fault_cv_.notify_all();
+        fault_th_.join();
+        reclaim();
+        auto &sl = res_.host_slabs;
+        sl.erase(std::remove_if(sl.begin(), sl.end(), [](const HostSlab &x) { return x.used == 0; }), sl.end());
         for (int k = 0; k < kSlots; ++k) {
             (void)hipHostFree(stage_[k]);
             (void)hipEventDestroy(slot_ev_[k]);
         }
-        (void)hipHostFree(n_host_);
+        (void)hipHostFree(sizes_);
         (void)hipStreamDestroy(cs_);
         if (!err_.empty()) throw Error(EGTB_ERR_HIP, "streaming a layer to the host failed: " + err_);
     }
 
   private:
-    static constexpr size_t kStage = (size_t)32 << 20;
-    static constexpr int kSlots = 4;
+    static constexpr size_t kAlign = 256;
+    static constexpr size_t kSlabMin = (size_t)64 << 20, kSlabMax = (size_t)1 << 30;
+    static constexpr size_t kStage = (size_t)64 << 20;
+    static constexpr int kSlots = 8;
     static constexpr int kCopyWorkers = 7;
+    static constexpr int kFaultWorkers = 5;
     struct Job {
         int layer;
         hipEvent_t ev;
     };
+    // one device-to-host copy into staging slot `slot` (-1: a book with no
+    // records); `last` closes the book
     struct Piece {
-        const char *src;
+        int slot;
         char *dst;
         size_t bytes;
+        int layer;
+        hipEvent_t retired;
+        size_t book_bytes;
+        bool last;
     };
 
-    void copy_out(const std::vector<Piece> &pieces) {
-        size_t issued = 0, done = 0;
-        while (done < pieces.size()) {
-            while (issued < pieces.size() && issued - done < (size_t)kSlots) {
-                const int k = (int)(issued % kSlots);
-                HIPCHK(hipMemcpyAsync(stage_[k], pieces[issued].src, pieces[issued].bytes, hipMemcpyDeviceToHost,
-                                      cs_));
-                HIPCHK(hipEventRecord(slot_ev_[k], cs_));
-                ++issued;
+    void fail(const std::string &what) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (err_.empty()) err_ = what;
+        failed_ = true;
+    }
+
+    // `bytes` of host memory from the slabs whose pages are in (waiting for
+    // the fault thread while it still has reserved memory to map), else from
+    // a slab mapped here and faulted in by the copies
+    char *take(size_t bytes) {
+        bytes = (bytes + kAlign - 1) & ~(kAlign - 1);
+        std::unique_lock<std::mutex> lk(mu_);
+        auto &v = res_.host_slabs;
+        for (;;) {
+            // first fit over the faulted-in slabs that still have room;
+            // slabs with less than kAlign * 4096 left are closed
+            for (size_t k = cur_; k < ready_; ++k) {
+                HostSlab &sl = v[k];
+                if (sl.bytes - sl.used >= bytes) {
+                    char *p = sl.p + sl.used;
+                    sl.used += bytes;
+                    while (cur_ < ready_ && v[cur_].bytes - v[cur_].used < kAlign * 4096) ++cur_;
+                    return p;
+                }
             }
-            const int k = (int)(done % kSlots);
-            HIPCHK(hipEventSynchronize(slot_ev_[k]));
-            pool_.copy(pieces[done].dst, (const char *)stage_[k], pieces[done].bytes);
-            ++done;
+            if (bytes > kSlabMax || !(fault_busy_ || mapped_ < want_)) break;
+            slab_cv_.wait(lk);
+        }
+        // past the reservation: a slab of its own, after the faulted-in ones
+        HostSlab own;
+        own.map(std::max(bytes, kSlabMin));
+        own.used = bytes;
+        char *p = own.p;
+        v.push_back(std::move(own));
+        return p;
+    }
+
+    // maps and faults in slabs until the reservation is covered
+    void fault_loop() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            fault_cv_.wait(lk, [this] { return fault_stop_ || mapped_ < want_; });
+            if (fault_stop_) return;
+            const size_t sz = want_ >= kSlabMax ? kSlabMax
+                                                : std::max(kSlabMin, (want_ - mapped_ + ((size_t)2 << 20) - 1) &
+                                                                         ~(((size_t)2 << 20) - 1));
+            mapped_ += sz;
+            fault_busy_ = true;
+            lk.unlock();
+            HostSlab sl;
+            bool ok = true;
+            try {
+                sl.map(sz);
+                fault_pool_.copy(sl.p, nullptr, sl.bytes);
+            } catch (const std::exception &e) {
+                ok = false;
+                fail(e.what());
+            }
+            lk.lock();
+            fault_busy_ = false;
+            if (ok) {
+                // slabs [0, ready_) are faulted in; take() maps its own slabs
+                // only while this thread is idle, so they come after
+                auto &v = res_.host_slabs;
+                v.insert(v.begin() + (std::ptrdiff_t)ready_, std::move(sl));
+                ++ready_;
+            } else {
+                want_ = mapped_;  // no more mapping ahead
+            }
+            slab_cv_.notify_all();
         }
     }
 
-    static void split(std::vector<Piece> &out, const void *src, char *dst, size_t bytes) {
-        for (size_t off = 0; off < bytes; off += kStage)
-            out.push_back({(const char *)src + off, dst + off, std::min(kStage, bytes - off)});
-    }
-
-    void run() {
+    void issue_loop() {
         (void)hipSetDevice(res_.dev);
         for (;;) {
             Job j;
             {
                 std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
-                if (q_.empty()) return;
-                j = q_.front();
-                q_.pop_front();
+                cv_.wait(lk, [this] { return stop_ || !jobs_.empty(); });
+                if (jobs_.empty()) return;
+                j = jobs_.front();
+                jobs_.pop_front();
             }
             try {
-                const auto t0 = std::chrono::steady_clock::now();
-                HIPCHK(hipStreamWaitEvent(cs_, j.ev, 0));
-                Book &bk = res_.books[j.layer];
-                HIPCHK(hipMemcpyAsync(n_host_, bk.n_dev, 8, hipMemcpyDeviceToHost, cs_));
-                HIPCHK(hipStreamSynchronize(cs_));
-                const uint64_t n = *n_host_;
-                const size_t kb = n * 8, rb = n * (size_t)rate_size_;
-                res_.host_keys[j.layer].allocate(kb);
-                res_.host_rates[j.layer].allocate(rb);
-                std::vector<Piece> pieces;
-                split(pieces, bk.keys, res_.host_keys[j.layer].p, kb);
-                split(pieces, bk.rates, res_.host_rates[j.layer].p, rb);
-                copy_out(pieces);
-                res_.streamed_bytes += kb + rb;
-                for (int w = 0; w < 3; ++w) res_.book_bufs[(size_t)j.layer * 3 + w].release();
-                bk.keys = nullptr;
-                bk.rates = nullptr;
-                bk.idx = nullptr;
-                res_.stream_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                issue(j);
             } catch (const std::exception &e) {
-                if (err_.empty()) err_ = e.what();
+                fail(e.what());
+                // the drain thread still closes the book (and its event)
+                push_piece({-1, nullptr, 0, j.layer, j.ev, 0, true});
             }
-            (void)hipEventDestroy(j.ev);
+        }
+    }
+
+    void push_piece(const Piece &p) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            pieces_.push_back(p);
+        }
+        cv_.notify_all();
+    }
+
+    void issue(const Job &j) {
+        if (failed_) throw std::runtime_error("an earlier book failed");
+        Book &bk = res_.books[j.layer];
+        HIPCHK(hipEventSynchronize(j.ev));
+        const uint64_t n = ((volatile const uint64_t *)sizes_)[j.layer];
+        const size_t kb = n * 8, rb = n * (size_t)rate_size_;
+        char *hk = kb ? take(kb) : nullptr, *hr = rb ? take(rb) : nullptr;
+        res_.host_keys[j.layer] = {hk, kb};
+        res_.host_rates[j.layer] = {hr, rb};
+        struct Part {
+            const char *src;
+            char *dst;
+            size_t bytes;
+        };
+        std::vector<Part> parts;
+        for (size_t off = 0; off < kb; off += kStage)
+            parts.push_back({(const char *)bk.keys + off, hk + off, std::min(kStage, kb - off)});
+        for (size_t off = 0; off < rb; off += kStage)
+            parts.push_back({(const char *)bk.rates + off, hr + off, std::min(kStage, rb - off)});
+        if (parts.empty()) {
+            push_piece({-1, nullptr, 0, j.layer, j.ev, 0, true});
+            return;
+        }
+        for (size_t k = 0; k < parts.size(); ++k) {
+            int slot;
+            {
+                // the slot's previous piece must have been drained
+                std::unique_lock<std::mutex> lk(mu_);
+                slot_cv_.wait(lk, [this] { return issued_ - drained_ < (uint64_t)kSlots; });
+                slot = (int)(issued_ % kSlots);
+                ++issued_;
+            }
+            try {
+                HIPCHK(hipMemcpyAsync(stage_[slot], parts[k].src, parts[k].bytes, hipMemcpyDeviceToHost, cs_));
+                HIPCHK(hipEventRecord(slot_ev_[slot], cs_));
+            } catch (...) {
+                push_piece({slot, nullptr, 0, j.layer, nullptr, 0, false});  // hands the slot back
+                throw;
+            }
+            push_piece({slot, parts[k].dst, parts[k].bytes, j.layer, j.ev, kb + rb, k + 1 == parts.size()});
+        }
+    }
+
+    void drain_loop() {
+        (void)hipSetDevice(res_.dev);
+        auto busy_from = std::chrono::steady_clock::now();
+        bool busy = false;
+        for (;;) {
+            Piece p;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                if (pieces_.empty() && busy) {
+                    res_.stream_ms +=
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - busy_from).count();
+                    busy = false;
+                }
+                cv_.wait(lk, [this] { return issue_done_ || !pieces_.empty(); });
+                if (pieces_.empty()) return;
+                p = pieces_.front();
+                pieces_.pop_front();
+            }
+            if (!busy) {
+                busy = true;
+                busy_from = std::chrono::steady_clock::now();
+            }
+            if (p.slot >= 0) {
+                const hipError_t e = p.bytes ? hipEventSynchronize(slot_ev_[p.slot]) : hipSuccess;
+                if (e == hipSuccess) {
+                    copy_pool_.copy(p.dst, (const char *)stage_[p.slot], p.bytes);
+                } else {
+                    fail(std::string("device-to-host copy: ") + hipGetErrorString(e));
+                }
+                {
+                    std::lock_guard<std::mutex> lk(mu_);
+                    ++drained_;
+                }
+                slot_cv_.notify_all();
+            }
+            if (p.last) {
+                (void)hipEventDestroy(p.retired);
+                res_.streamed_bytes += p.book_bytes;
+                std::lock_guard<std::mutex> lk(mu_);
+                arrived_.push_back(p.layer);
+            }
         }
     }
 
     Result &res_;
     int rate_size_;
-    CopyPool pool_;
+    CopyPool copy_pool_, fault_pool_;
     hipStream_t cs_ = nullptr;
+    uint64_t *sizes_ = nullptr, *sizes_dev_ = nullptr;  // live size per layer, host-mapped
     void *stage_[kSlots] = {};
     hipEvent_t slot_ev_[kSlots] = {};
-    uint64_t *n_host_ = nullptr;
-    std::thread th_;
-    std::mutex mu_;
-    std::condition_variable cv_;
-    std::deque<Job> q_;
-    bool stop_ = false;
+    std::thread issue_th_, drain_th_, fault_th_;
+    std::mutex mu_;  // guards everything below and res_.host_slabs
+    std::condition_variable cv_, slot_cv_, fault_cv_, slab_cv_;
+    std::deque<Job> jobs_;
+    std::deque<Piece> pieces_;
+    std::vector<int> arrived_;
+    uint64_t issued_ = 0, drained_ = 0;
+    size_t want_ = 0, mapped_ = 0, cur_ = 0, ready_ = 0;
+    bool stop_ = false, issue_done_ = false, fault_stop_ = false, fault_busy_ = false;
     std::string err_;
+    std::atomic<bool> failed_{false};
 };
 
 // Raw layer files <pathname><i> (N_i x u64, little endian; reference
@@ -1083,6 +1352,17 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     res.books.assign(steps, Book{});
     res.info.assign(steps, LayerInfo{});
     if (o.host_tables) res.raw_arena.set_stream(s);
+    res.host_tables = o.host_tables != 0;
+    std::unique_ptr<BookStreamer> streamer;
+    if (res.host_tables) {
+        if (files) throw Error(EGTB_ERR_ARG, "host_tables and write_files cannot be combined");
+        res.book_ring.reset(s, steps);
+        res.host_keys.clear();
+        res.host_rates.clear();
+        res.host_keys.resize(steps);
+        res.host_rates.resize(steps);
+        streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T), steps);
+    }
     res.hash_bufs.clear();
     res.hash_bufs.resize(steps);
     res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
@@ -1227,6 +1507,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         if (i > 1) res.raw_arena.trim_last(res.raw[i - 1], n_par * 8);
         res.info[i - 1].n_raw = n_par;
         queue_raw(i - 1, n_par);
+        if (streamer) streamer->reserve(n_par);
         if (n_par * 64ull >= (1ull << 32))
             throw Error(EGTB_ERR_ARG, "layer " + std::to_string(i - 1) + " has too many states for one step");
         const uint64_t c2 = H[i].tot & 0xFFFFFFFFull, c4 = H[i].tot >> 32;
@@ -1352,6 +1633,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         res.raw_arena.trim_last(res.raw[L - 1], H[L - 1].n_raw * 8);
         res.info[L - 1].n_raw = H[L - 1].n_raw;
         queue_raw(L - 1, H[L - 1].n_raw);
+        if (streamer) streamer->reserve(H[L - 1].n_raw);
         if (!chunked) {
             const uint64_t n_c = c4_prev;
             DBuf &cur = cand[L & 1];
@@ -1365,6 +1647,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         read_meta(s, res, H, L, L);
         res.raw_arena.trim_last(res.raw[L], H[L].n_raw * 8);
         res.info[L].n_raw = H[L].n_raw;
+        if (streamer) streamer->reserve(H[L].n_raw);
     }
     chunk2.release();
     chunk4.release();
@@ -1378,17 +1661,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     const auto t_fwd = std::chrono::steady_clock::now();
 
     // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
-    res.host_tables = o.host_tables != 0;
-    std::unique_ptr<BookStreamer> streamer;
-    if (res.host_tables) {
-        if (files) throw Error(EGTB_ERR_ARG, "host_tables and write_files cannot be combined");
-        res.book_bufs.resize((size_t)steps * 3);
-        res.host_keys.clear();
-        res.host_rates.clear();
-        res.host_keys.resize(steps);
-        res.host_rates.resize(steps);
-        streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T));
-    }
+
     uint64_t max_n = 0;
     for (int i = 0; i < steps; ++i) max_n = std::max(max_n, res.info[i].n_raw);
     rates.ensure(max_n * sizeof(T) + 8, s);
@@ -1440,6 +1713,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             // book i+2 was last read by this layer's solve; raw layers above
             // i are consumed
             streamer->retire(i + 2, s);
+            streamer->reclaim();
             if (!o.keep_raw) res.raw_arena.release_after(res.raw[i]);
             size_t fr = 0, tot = 0;
             if (hipMemGetInfo(&fr, &tot) == hipSuccess) res.peak_device_bytes = std::max<uint64_t>(res.peak_device_bytes, tot - fr);
@@ -1496,6 +1770,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         streamer->finish();
         // hand the pooled chunks of this build back to the device
         hipMemPool_t pool;
+        res.book_ring.free_all();
         if (!o.keep_raw) res.raw_arena.release();
         HIPCHK(hipStreamSynchronize(s));
         if (hipDeviceGetDefaultMemPool(&pool, res.dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
@@ -1534,11 +1809,12 @@ void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *p
     if (!(o->spawn_rate4 >= 0.0 && o->spawn_rate4 <= 1.0)) throw Error(EGTB_ERR_ARG, "spawn_rate4 out of [0, 1]");
     const PatternDev p = make_pattern(pattern, o->target);
     const std::string dt = o->success_rate_dtype ? o->success_rate_dtype : "float32";
+    if (dt != "float32" && dt != "float64" && dt != "uint32" && dt != "uint64")
+        throw Error(EGTB_ERR_ARG, "success_rate_dtype '" + dt + "' is not supported");
     if (dt == "float32") run_build<float>(seeds, n_seeds, p, *o, res);
     else if (dt == "float64") run_build<double>(seeds, n_seeds, p, *o, res);
     else if (dt == "uint32") run_build<uint32_t>(seeds, n_seeds, p, *o, res);
-    else if (dt == "uint64") run_build<uint64_t>(seeds, n_seeds, p, *o, res);
-    else throw Error(EGTB_ERR_ARG, "success_rate_dtype '" + dt + "' is not supported");
+    else run_build<uint64_t>(seeds, n_seeds, p, *o, res);
 }
 
 }  // namespace egtb

@@ -30,14 +30,19 @@ def test_chunked_forward_matches_golden(pkg, tag, mkeys):
         res.close()
 
 
-@pytest.mark.parametrize("tag", ["442_256", "444_64"])
+@pytest.mark.parametrize("tag", ["442_256", "444_64", "442_128_float64", "442_128_uint32"])
 def test_host_tables_match_golden(pkg, tag):
+    """Books streamed to host memory give the reference's bytes, for 4- and
+    8-byte rates, and exactly the live records are copied."""
     gp = golden_params(load_golden(tag))
     res = pkg.run_pattern_build(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
-                                gp["docheck"], "", write_files=False, keep=True, host_tables=True)
+                                gp["docheck"], "", write_files=False, keep=True, host_tables=True,
+                                success_rate_dtype=gp["dtype"])
     try:
         mem = res.memory()
-        assert mem["streamed_bytes"] == 12 * sum(res.layer_info(i)[1] for i in range(res.steps))
+        print(f"case host tables {tag}: {mem}", flush=True)
+        rec = 8 + {"float32": 4, "uint32": 4, "float64": 8, "uint64": 8}[gp["dtype"]]
+        assert mem["streamed_bytes"] == rec * sum(res.layer_info(i)[1] for i in range(res.steps))
         _check(res, tag)
     finally:
         res.close()
