@@ -135,7 +135,8 @@ struct LookbackWork {
     }
 };
 
-uint32_t radix_tile(int bits);
+// keys per radix tile for a sort of n keys with the given schedule (1 or 2)
+uint32_t radix_tile(int bits, uint64_t n, int schedule);
 // Classic schedule: per digit a tile-histogram kernel, a scan of the
 // [digit][tile] counts and a scatter kernel.  counts/offsets hold
 // tiles << bits words, partial the scan partials.

@@ -298,9 +298,11 @@ uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, u
     const int lo = __builtin_ctzll(diff);
     const int hi = 64 - __builtin_clzll(diff);
     const int bits = tuned_bits(n, hi - lo, bits_req);
-    const uint64_t ntiles = (n + radix_tile(bits) - 1) / radix_tile(bits);
+    const int sched = radix_schedule(n, schedule);
+    const uint64_t tile = radix_tile(bits, n, sched);
+    const uint64_t ntiles = (n + tile - 1) / tile;
     uint64_t *out;
-    if (radix_schedule(n, schedule) == 2) {
+    if (sched == 2) {
         w.ensure_lookback(s, ntiles, 1 << bits, 0);
         out = launch_onesweep_sort(s, bits, keys, tmp, n, lo, hi, w.lw);
     } else {
