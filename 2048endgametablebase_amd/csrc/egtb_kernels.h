@@ -121,11 +121,10 @@ constexpr int kHistWords = 16 * 2048;
 struct LookbackWork {
     uint64_t *status_sort = nullptr;  // [tiles * 2048]
     uint64_t *status_uniq = nullptr;  // [tiles]
-    uint64_t *status_scan = nullptr;  // [tiles] of the chained count scans
-    unsigned long long *counters = nullptr;  // [0] sort tiles, [1] unique tiles, [3] scan tiles
+    unsigned long long *counters = nullptr;  // [0] sort tiles, [1] unique tiles
     uint32_t *hist = nullptr;         // [2 * kHistWords]
     uint32_t *err = nullptr;
-    unsigned long long claimed = 0, claimed_uniq = 0, claimed_scan = 0;
+    unsigned long long claimed = 0, claimed_uniq = 0;
     uint32_t epoch = 0;
     int hist_parity = 0;
     int dirty[2] = {0, 0};  // histogram words left non-zero in each buffer
@@ -137,21 +136,19 @@ struct LookbackWork {
 
 // keys per radix tile for a sort of n keys with the given schedule (1 or 2)
 uint32_t radix_tile(int bits, uint64_t n, int schedule);
-// Classic schedule: per digit a tile-histogram kernel, a scan of the
-// [digit][tile] counts and a scatter kernel.  counts/offsets hold
-// tiles << bits words, partial the scan partials.
+// Classic schedule: per digit a tile-histogram kernel, a per-digit scan of
+// the [digit][tile] counts and a scatter kernel.  counts holds
+// tiles << bits words, offsets tiles << bits words plus 2^bits digit totals
+// (tiles = n / radix_tile(bits, n, 1), rounded up).
 uint64_t *launch_classic_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
-                              uint32_t *counts, uint32_t *offsets, LookbackWork &lw);
+                              uint32_t *counts, uint32_t *offsets);
 // tiles of the chained scan of n words (status words needed)
-uint64_t scan_chained_tiles(uint64_t n);
 // Stable LSD sort of keys[0..n) on bits [lo, hi); returns keys or tmp,
 // whichever holds the result.
 uint64_t *launch_onesweep_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
                                LookbackWork &lw);
 
 uint64_t unique_tiles(uint64_t n);
-void launch_unique_classic(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts,
-                           uint64_t *tile_offs, uint64_t *partial, uint64_t *out, uint64_t *count_out);
 void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *out, uint64_t *count_out,
                    LookbackWork &lw);
 

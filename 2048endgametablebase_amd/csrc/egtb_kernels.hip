@@ -495,20 +495,57 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__r
     for (int i = threadIdx.x; i < BINS; i += kSortThreads) counts[(uint64_t)i * ntiles + blockIdx.x] = hist[i];
 }
 
+// Per-digit scan of one pass's [digit][tile] counts: block d owns digit d
+// and writes, for every tile, the keys of digit d in the tiles before it
+// (offsets) and the digit's total (totals[d]). All digits in one launch and
+// no block waits on another.
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_radix_binscan(const uint32_t *__restrict__ counts,
+                                                                uint32_t ntiles, uint32_t *__restrict__ offsets,
+                                                                uint32_t *__restrict__ totals) {
+    constexpr int PER = 4;
+    __shared__ uint32_t scratch[kSortWaves];
+    const uint32_t *c = counts + (uint64_t)blockIdx.x * ntiles;
+    uint32_t *o = offsets + (uint64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += kSortThreads * PER) {
+        const uint32_t t0 = base + threadIdx.x * PER;
+        uint32_t v[PER], sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            v[q] = t0 + q < ntiles ? c[t0 + q] : 0u;
+            sum += v[q];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_exclusive_sum(sum, scratch, total);
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            if (t0 + q < ntiles) {
+                o[t0 + q] = run;
+                run += v[q];
+            }
+        carry += total;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
 // Scatter one tile: stable local ranks by wave multisplit, reorder through
-// LDS by digit, then write each digit run contiguously at its global offset
-// (offsets = exclusive scan of counts in bin-major order).
+// LDS by digit, then write each digit run contiguously at its global offset:
+// the keys of smaller digits (exclusive scan of totals, done here) plus the
+// keys of the same digit in earlier tiles (offsets).
 template <int BITS, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *__restrict__ src,
                                                                 uint64_t *__restrict__ dst, uint64_t n,
                                                                 int shift,
                                                                 const uint32_t *__restrict__ offsets,
+                                                                const uint32_t *__restrict__ totals,
                                                                 uint32_t ntiles) {
     constexpr int BINS = 1 << BITS;
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ uint64_t stage[TILE];
     __shared__ uint32_t whist[kSortWaves][BINS];
     __shared__ uint32_t binbase[BINS];
+    __shared__ uint32_t gbase[BINS];
     __shared__ uint32_t scratch[kSortWaves];
 
     for (int i = threadIdx.x; i < kSortWaves * BINS; i += kSortThreads) (&whist[0][0])[i] = 0;
@@ -549,22 +586,29 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
         binbase[bin] = s;
     }
     __syncthreads();
-    // exclusive scan of the per-bin totals (each thread owns a run of bins)
+    // exclusive scans over the bins (each thread owns a run of bins): of this
+    // tile's per-bin totals, and of the digit totals of the whole pass
     {
         constexpr int PER = BINS >= kSortThreads ? BINS / kSortThreads : 1;
         const int first = threadIdx.x * PER;
-        uint32_t local[PER];
-        uint32_t s = 0;
+        uint32_t local[PER], glocal[PER];
+        uint32_t s = 0, g = 0;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             local[q] = s;
+            glocal[q] = g;
             s += (first + q < BINS) ? binbase[first + q] : 0u;
+            g += (first + q < BINS) ? totals[first + q] : 0u;
         }
         uint32_t total;
         const uint32_t ex = block_exclusive_sum(s, scratch, total);
+        const uint32_t gex = block_exclusive_sum(g, scratch, total);
 #pragma unroll
         for (int q = 0; q < PER; ++q)
-            if (first + q < BINS) binbase[first + q] = ex + local[q];
+            if (first + q < BINS) {
+                binbase[first + q] = ex + local[q];
+                gbase[first + q] = gex + glocal[q];
+            }
         __syncthreads();
     }
 #pragma unroll
@@ -580,7 +624,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     for (uint32_t i = threadIdx.x; i < m; i += kSortThreads) {
         const uint64_t k = stage[i];
         const uint32_t d = (uint32_t)((k >> shift) & (BINS - 1));
-        dst[(uint64_t)offsets[(uint64_t)d * ntiles + blockIdx.x] + (i - binbase[d])] = k;
+        dst[(uint64_t)gbase[d] + offsets[(uint64_t)d * ntiles + blockIdx.x] + (i - binbase[d])] = k;
     }
 }
 
@@ -636,108 +680,6 @@ __global__ __launch_bounds__(kUniqThreads) void k_unique_lb(const uint64_t *__re
     __syncthreads();
     uint64_t pos = s_off + ex;
     if (count_out && (uint64_t)(tile + 1) * kUniqTile >= n && threadIdx.x == 0) *count_out = s_off + total;
-#pragma unroll
-    for (int j = 0; j < kUniqItems; ++j)
-        if ((f >> j) & 1u) out[pos++] = v[j];
-}
-
-// Exclusive scan of u32 counts in one launch: tiles are claimed in order by
-// a ticket, publish their sum, and take their prefix by decoupled look-back
-// (values stay below 2^42: the counts of one sort total at most n keys).
-__global__ __launch_bounds__(kScanThreads) void k_scan_chained(const uint32_t *__restrict__ in, uint64_t n,
-                                                               uint32_t *__restrict__ out,
-                                                               uint64_t *__restrict__ status,
-                                                               unsigned long long *__restrict__ tile_counter,
-                                                               unsigned long long tile_base, uint32_t epoch,
-                                                               uint32_t *__restrict__ err) {
-    __shared__ uint32_t tile_v[kScanTile];
-    __shared__ uint32_t scratch[kScanThreads / 64];
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_off;
-    if (threadIdx.x == 0) s_tile = (uint32_t)(atomicAdd(tile_counter, 1ull) - tile_base);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t base = (uint64_t)tile * kScanTile;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
-        tile_v[j * kScanThreads + threadIdx.x] = idx < n ? in[idx] : 0u;
-    }
-    __syncthreads();
-    uint32_t local[kScanItems];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        local[j] = sum;
-        sum += tile_v[threadIdx.x * kScanItems + j];
-    }
-    uint32_t total;
-    const uint32_t ex = block_exclusive_sum(sum, scratch, total);
-    if (threadIdx.x == 0) {
-        if (tile == 0) {
-            lb_store(status, lb_pack(epoch, 2, total));
-            s_off = 0;
-        } else {
-            lb_store(status + tile, lb_pack(epoch, 1, total));
-            const uint64_t pre = lb_lookback(status, 1, tile, epoch, err);
-            lb_store(status + tile, lb_pack(epoch, 2, pre + total));
-            s_off = (uint32_t)pre;
-        }
-    }
-    __syncthreads();
-    const uint32_t off = s_off + ex;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) tile_v[threadIdx.x * kScanItems + j] = off + local[j];
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
-        if (idx < n) out[idx] = tile_v[j * kScanThreads + threadIdx.x];
-    }
-}
-
-// Classic unique: tile counts, scan, write.
-__global__ __launch_bounds__(kUniqThreads) void k_unique_count(const uint64_t *__restrict__ a, uint64_t n,
-                                                               uint64_t *__restrict__ counts) {
-    __shared__ uint64_t scratch[kUniqThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kUniqTile;
-    uint64_t c = 0;
-#pragma unroll
-    for (int j = 0; j < kUniqItems; ++j) {
-        const uint64_t idx = base + (uint64_t)j * kUniqThreads + threadIdx.x;
-        if (idx < n) c += (idx == 0 || a[idx] != a[idx - 1]) ? 1u : 0u;
-    }
-    c = wave_reduce_sum(c);
-    if (lane_id() == 0) scratch[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int w = 0; w < kUniqThreads / 64; ++w) t += scratch[w];
-        counts[blockIdx.x] = t;
-    }
-}
-
-// offsets: exclusive scan of the tile counts; out_n (optional) gets the total.
-__global__ __launch_bounds__(kUniqThreads) void k_unique_write(const uint64_t *__restrict__ a, uint64_t n,
-                                                               const uint64_t *__restrict__ offsets,
-                                                               uint64_t *__restrict__ out,
-                                                               uint64_t *__restrict__ count_out) {
-    __shared__ uint64_t scratch[kUniqThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kUniqTile + (uint64_t)threadIdx.x * kUniqItems;
-    uint64_t v[kUniqItems];
-    uint32_t f = 0;
-    uint64_t cnt = 0;
-#pragma unroll
-    for (int j = 0; j < kUniqItems; ++j) {
-        const uint64_t idx = base + j;
-        const bool keep = idx < n && (idx == 0 || a[idx] != a[idx - 1]);
-        v[j] = idx < n ? a[idx] : 0;
-        f |= (keep ? 1u : 0u) << j;
-        cnt += keep;
-    }
-    uint64_t total;
-    uint64_t pos = block_exclusive_sum(cnt, scratch, total) + offsets[blockIdx.x];
-    if (count_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count_out = offsets[blockIdx.x] + total;
 #pragma unroll
     for (int j = 0; j < kUniqItems; ++j)
         if ((f >> j) & 1u) out[pos++] = v[j];
@@ -1129,53 +1071,45 @@ static uint64_t *onesweep_impl(hipStream_t s, uint64_t *keys, uint64_t *tmp, uin
     return src;
 }
 
-uint64_t scan_chained_tiles(uint64_t n) { return blocks_for(n ? n : 1, kScanTile); }
-
-static void launch_scan_chained(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *out, LookbackWork &lw) {
-    const uint32_t tiles = (uint32_t)scan_chained_tiles(n);
-    const uint32_t epoch = lw.next_epoch();
-    hipLaunchKernelGGL(k_scan_chained, dim3(tiles), dim3(kScanThreads), 0, s, in, n, out, lw.status_scan,
-                       lw.counters + 3, lw.claimed_scan, epoch, lw.err);
-    lw.claimed_scan += tiles;
-}
-
 template <int BITS, int ITEMS>
 static void classic_pass_impl(hipStream_t s, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
-                              uint32_t *counts, uint32_t *offsets, LookbackWork &lw) {
+                              uint32_t *counts, uint32_t *offsets) {
     const uint32_t ntiles = blocks_for(n, kSortThreads * ITEMS);
+    uint32_t *totals = offsets + ((uint64_t)ntiles << BITS);
     {
         Timer t(s, "radix_hist", 8.0 * (double)n);  // reads every key once
         hipLaunchKernelGGL((k_radix_hist<BITS, ITEMS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, n, shift,
                            counts, ntiles);
     }
-    launch_scan_chained(s, counts, (uint64_t)ntiles << BITS, offsets, lw);
+    hipLaunchKernelGGL((k_radix_binscan<BITS>), dim3(1u << BITS), dim3(kSortThreads), 0, s, counts, ntiles, offsets,
+                       totals);
     {
         Timer t(s, "radix_scatter", 16.0 * (double)n);  // reads and writes every key once
         hipLaunchKernelGGL((k_radix_scatter<BITS, ITEMS>), dim3(ntiles), dim3(kSortThreads), 0, s, src, dst, n,
-                           shift, offsets, ntiles);
+                           shift, offsets, totals, ntiles);
     }
 }
 
 template <int BITS>
 static void classic_pass(hipStream_t s, const uint64_t *src, uint64_t *dst, uint64_t n, int shift,
-                         uint32_t *counts, uint32_t *offsets, LookbackWork &lw) {
+                         uint32_t *counts, uint32_t *offsets) {
     switch (classic_items(BITS, n)) {
-        case 4: classic_pass_impl<BITS, 4>(s, src, dst, n, shift, counts, offsets, lw); break;
-        case 8: classic_pass_impl<BITS, 8>(s, src, dst, n, shift, counts, offsets, lw); break;
-        default: classic_pass_impl<BITS, 16>(s, src, dst, n, shift, counts, offsets, lw); break;
+        case 4: classic_pass_impl<BITS, 4>(s, src, dst, n, shift, counts, offsets); break;
+        case 8: classic_pass_impl<BITS, 8>(s, src, dst, n, shift, counts, offsets); break;
+        default: classic_pass_impl<BITS, 16>(s, src, dst, n, shift, counts, offsets); break;
     }
 }
 
 uint64_t *launch_classic_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t *tmp, uint64_t n, int lo, int hi,
-                              uint32_t *counts, uint32_t *offsets, LookbackWork &lw) {
+                              uint32_t *counts, uint32_t *offsets) {
     if (n < 2 || hi <= lo) return keys;
     uint64_t *src = keys, *dst = tmp;
     for (int shift = lo; shift < hi; shift += bits) {
         switch (bits) {
-            case 4: classic_pass<4>(s, src, dst, n, shift, counts, offsets, lw); break;
-            case 6: classic_pass<6>(s, src, dst, n, shift, counts, offsets, lw); break;
-            case 11: classic_pass_impl<11, 16>(s, src, dst, n, shift, counts, offsets, lw); break;
-            default: classic_pass<8>(s, src, dst, n, shift, counts, offsets, lw); break;
+            case 4: classic_pass<4>(s, src, dst, n, shift, counts, offsets); break;
+            case 6: classic_pass<6>(s, src, dst, n, shift, counts, offsets); break;
+            case 11: classic_pass_impl<11, 16>(s, src, dst, n, shift, counts, offsets); break;
+            default: classic_pass<8>(s, src, dst, n, shift, counts, offsets); break;
         }
         std::swap(src, dst);
     }
@@ -1192,18 +1126,6 @@ uint64_t *launch_onesweep_sort(hipStream_t s, int bits, uint64_t *keys, uint64_t
         case 11: return onesweep_impl<11>(s, keys, tmp, n, lo, passes, lw);
         default: return onesweep_impl<8>(s, keys, tmp, n, lo, passes, lw);
     }
-}
-
-void launch_unique_classic(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *tile_counts,
-                           uint64_t *tile_offs, uint64_t *partial, uint64_t *out, uint64_t *count_out) {
-    if (n == 0) {
-        if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
-        return;
-    }
-    const uint32_t tiles = (uint32_t)unique_tiles(n);
-    hipLaunchKernelGGL(k_unique_count, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_counts);
-    launch_scan_u64(s, tile_counts, tiles, nullptr, tile_offs, partial, tile_offs + tiles);
-    hipLaunchKernelGGL(k_unique_write, dim3(tiles), dim3(kUniqThreads), 0, s, a, n, tile_offs, out, count_out);
 }
 
 void launch_unique(hipStream_t s, const uint64_t *a, uint64_t n, uint64_t *out, uint64_t *count_out,

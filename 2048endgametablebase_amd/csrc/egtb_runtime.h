@@ -76,11 +76,11 @@ struct HostStage {
 struct SortWork {
     DBuf counts, offsets, partial, tile_counts, tile_offs, maxbits;
     DBuf bk_u64, bk_u32, bk_scratch, bk_ids;  // sample-sort dedup
-    DBuf lb_status_sort, lb_status_uniq, lb_status_scan, lb_misc;
+    DBuf lb_status_sort, lb_status_uniq, lb_misc;
     LookbackWork lw;
     // grow the look-back state for a sort of `sort_tiles` tiles of `bins`
     // digits and a unique of `uniq_tiles` tiles (0 = no change)
-    void ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles, uint64_t scan_tiles = 0);
+    void ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles);
     // stall flag of the look-back kernels (synchronises)
     uint32_t read_error(hipStream_t s);
 };

@@ -215,32 +215,24 @@ int choose_radix_bits(int span, int requested) {
     return p11 < p8 ? 11 : 8;
 }
 
-void SortWork::ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles,
-                               uint64_t scan_tiles) {
+void SortWork::ensure_lookback(hipStream_t s, uint64_t sort_tiles, int bins, uint64_t uniq_tiles) {
     if (!lb_misc.p) {
         // counters (2 x u64), error flag, double histogram
         const size_t bytes = 64 + (size_t)2 * kHistWords * 4;
         lb_misc.alloc(bytes, s);
         HIPCHK(hipMemsetAsync(lb_misc.p, 0, bytes, s));
-        // counters[0], [1] at bytes 0-15, error flag at 16, counters[3] at 24
+        // counters[0], [1] at bytes 0-15, error flag at 16
         lw.counters = (unsigned long long *)lb_misc.p;
         lw.err = (uint32_t *)((char *)lb_misc.p + 16);
         lw.hist = (uint32_t *)((char *)lb_misc.p + 64);
-        lw.claimed = lw.claimed_uniq = lw.claimed_scan = 0;
+        lw.claimed = lw.claimed_uniq = 0;
     }
     if (lw.epoch > (1u << 20) - 64) {
         // status words are told apart by a 20-bit epoch: clear them before
         // an epoch value could come round again
         if (lb_status_sort.p) HIPCHK(hipMemsetAsync(lb_status_sort.p, 0, lb_status_sort.bytes, s));
         if (lb_status_uniq.p) HIPCHK(hipMemsetAsync(lb_status_uniq.p, 0, lb_status_uniq.bytes, s));
-        if (lb_status_scan.p) HIPCHK(hipMemsetAsync(lb_status_scan.p, 0, lb_status_scan.bytes, s));
         lw.epoch = 0;
-    }
-    const size_t want_scan = (size_t)scan_tiles * 8;
-    if (want_scan > lb_status_scan.bytes) {
-        lb_status_scan.alloc(want_scan + want_scan / 2 + 64, s);
-        HIPCHK(hipMemsetAsync(lb_status_scan.p, 0, lb_status_scan.bytes, s));
-        lw.status_scan = (uint64_t *)lb_status_scan.p;
     }
     const size_t want_sort = (size_t)sort_tiles * (size_t)bins * 8;
     if (want_sort > lb_status_sort.bytes) {
@@ -308,28 +300,21 @@ uint64_t *radix_sort(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, u
     } else {
         const uint64_t nc = ntiles << bits;
         w.counts.ensure(nc * 4, s);
-        w.offsets.ensure(nc * 4, s);
-        w.ensure_lookback(s, 0, 0, 0, scan_chained_tiles(nc));
-        out = launch_classic_sort(s, bits, keys, tmp, n, lo, hi, (uint32_t *)w.counts.p, (uint32_t *)w.offsets.p,
-                                  w.lw);
+        w.offsets.ensure((nc + (1ull << bits)) * 4, s);  // + the digit totals
+        out = launch_classic_sort(s, bits, keys, tmp, n, lo, hi, (uint32_t *)w.counts.p, (uint32_t *)w.offsets.p);
     }
     HIPCHK(hipGetLastError());
     return out;
 }
 
+// One launch with decoupled look-back for every schedule: the three-launch
+// form (tile counts, scan, write) cost ~25 us a layer at 1-4 Mi keys
+// against ~5 us, all of it latency.
 void unique_into(hipStream_t s, const uint64_t *sorted, uint64_t n, uint64_t *out, uint64_t *count_out,
                  SortWork &w, int schedule) {
-    if (radix_schedule(n, schedule) == 2) {
-        w.ensure_lookback(s, 0, 0, unique_tiles(n));
-        launch_unique(s, sorted, n, out, count_out, w.lw);
-    } else {
-        const uint64_t tiles = unique_tiles(n);
-        w.tile_counts.ensure(tiles * 8, s);
-        w.tile_offs.ensure((tiles + 1) * 8, s);
-        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
-        launch_unique_classic(s, sorted, n, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                              (uint64_t *)w.partial.p, out, count_out);
-    }
+    (void)schedule;
+    w.ensure_lookback(s, 0, 0, unique_tiles(n));
+    launch_unique(s, sorted, n, out, count_out, w.lw);
 }
 
 void dedup_into(hipStream_t s, uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all,
