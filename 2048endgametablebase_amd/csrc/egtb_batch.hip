@@ -450,15 +450,9 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
         bk.n_dev = &M[i].n_live;
         bk.keys = (uint64_t *)res.book_arena.alloc(n * 8);
         bk.rates = res.book_arena.alloc(n * 4);
-        if (n) {
-            const uint64_t tiles = compact_tiles(n);
-            w.tile_counts.ensure(tiles * 8, s);
-            w.tile_offs.ensure((tiles + 1) * 8, s);
-            w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
-        }
-        launch_compact<float>(s, res.raw[i], (const float *)rates.p, n, (uint64_t *)w.tile_counts.p,
-                              (uint64_t *)w.tile_offs.p, (uint64_t *)w.partial.p, bk.keys, (float *)bk.rates,
-                              (unsigned long long *)&M[i].maxbits, &M[i].n_live);
+        if (n) w.ensure_lookback(s, 0, 0, compact_tiles(n));
+        launch_compact<float>(s, res.raw[i], (const float *)rates.p, n, bk.keys, (float *)bk.rates,
+                              (unsigned long long *)&M[i].maxbits, &M[i].n_live, w.lw);
         bk.bits = bindex_bits(n);
         bk.idx = (uint32_t *)res.book_arena.alloc(((1ull << bk.bits) + 1) * 4);
         launch_bucket_index(s, bk.keys, n, bk.n_dev, bk.bits, bk.idx);

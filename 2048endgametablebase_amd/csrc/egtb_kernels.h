@@ -120,7 +120,7 @@ void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *ou
 constexpr int kHistWords = 16 * 2048;
 struct LookbackWork {
     uint64_t *status_sort = nullptr;  // [tiles * 2048]
-    uint64_t *status_uniq = nullptr;  // [tiles]
+    uint64_t *status_uniq = nullptr;  // [tiles] of unique and compaction launches
     unsigned long long *counters = nullptr;  // [0] sort tiles, [1] unique tiles
     uint32_t *hist = nullptr;         // [2 * kHistWords]
     uint32_t *err = nullptr;
@@ -164,11 +164,14 @@ template <typename T>
 void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,
                         T *rates);
 
+// Records with rate > threshold, in order, in one look-back launch; it
+// shares the unique kernel's status words (ensure_lookback with
+// compact_tiles(n) unique tiles first).
 uint64_t compact_tiles(uint64_t n);
 template <typename T>
-void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
-                    uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
-                    unsigned long long *max_bits, uint64_t *count_out, T threshold = T(0));
+void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
+                    T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw,
+                    T threshold = T(0));
 
 template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);

@@ -875,37 +875,28 @@ constexpr int kCompThreads = 256;
 constexpr int kCompItems = 8;
 constexpr int kCompTile = kCompThreads * kCompItems;
 
+// Keep the records with rate > thr, in order, in one launch: tiles are
+// claimed in order by a ticket, count their kept records, publish the count
+// and take their output offset by decoupled look-back. max_bits gets the
+// bits of the largest rate; count_out the number kept.
 template <typename T>
-__global__ __launch_bounds__(kCompThreads) void k_compact_count(const T *__restrict__ rates, uint64_t n, T thr,
-                                                                uint64_t *__restrict__ counts) {
+__global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__restrict__ keys,
+                                                             const T *__restrict__ rates, uint64_t n, T thr,
+                                                             uint64_t *__restrict__ out_keys,
+                                                             T *__restrict__ out_rates,
+                                                             unsigned long long *__restrict__ max_bits,
+                                                             uint64_t *__restrict__ count_out,
+                                                             uint64_t *__restrict__ status,
+                                                             unsigned long long *__restrict__ tile_counter,
+                                                             unsigned long long tile_base, uint32_t epoch,
+                                                             uint32_t *__restrict__ err) {
     __shared__ uint64_t scratch[kCompThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kCompTile;
-    uint64_t c = 0;
-#pragma unroll
-    for (int j = 0; j < kCompItems; ++j) {
-        const uint64_t idx = base + (uint64_t)j * kCompThreads + threadIdx.x;
-        if (idx < n) c += rates[idx] > thr ? 1u : 0u;
-    }
-    c = wave_reduce_sum(c);
-    if (lane_id() == 0) scratch[threadIdx.x >> 6] = c;
+    __shared__ uint32_t s_tile;
+    __shared__ uint64_t s_off;
+    if (threadIdx.x == 0) s_tile = (uint32_t)(atomicAdd(tile_counter, 1ull) - tile_base);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int w = 0; w < kCompThreads / 64; ++w) t += scratch[w];
-        counts[blockIdx.x] = t;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *__restrict__ keys,
-                                                                const T *__restrict__ rates, uint64_t n, T thr,
-                                                                const uint64_t *__restrict__ offsets,
-                                                                uint64_t *__restrict__ out_keys,
-                                                                T *__restrict__ out_rates,
-                                                                unsigned long long *__restrict__ max_bits,
-                                                                uint64_t *__restrict__ count_out) {
-    __shared__ uint64_t scratch[kCompThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kCompTile + (uint64_t)threadIdx.x * kCompItems;
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * kCompTile + (uint64_t)threadIdx.x * kCompItems;
     T r[kCompItems];
     uint64_t cnt = 0;
     T mx = (T)0;
@@ -917,8 +908,21 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_write(const uint64_t *
         mx = mx < r[j] ? r[j] : mx;
     }
     uint64_t total;
-    uint64_t pos = block_exclusive_sum(cnt, scratch, total) + offsets[blockIdx.x];
-    if (count_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count_out = offsets[blockIdx.x] + total;
+    const uint64_t ex = block_exclusive_sum(cnt, scratch, total);
+    if (threadIdx.x == 0) {
+        if (tile == 0) {
+            lb_store(status, lb_pack(epoch, 2, total));
+            s_off = 0;
+        } else {
+            lb_store(status + tile, lb_pack(epoch, 1, total));
+            const uint64_t pre = lb_lookback(status, 1, tile, epoch, err);
+            lb_store(status + tile, lb_pack(epoch, 2, pre + total));
+            s_off = pre;
+        }
+    }
+    __syncthreads();
+    uint64_t pos = s_off + ex;
+    if (count_out && (uint64_t)(tile + 1) * kCompTile >= n && threadIdx.x == 0) *count_out = s_off + total;
 #pragma unroll
     for (int j = 0; j < kCompItems; ++j) {
         if (r[j] > thr) {
@@ -1185,19 +1189,18 @@ void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const P
 uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
 
 template <typename T>
-void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *tile_counts,
-                    uint64_t *tile_offs, uint64_t *partial, uint64_t *out_keys, T *out_rates,
-                    unsigned long long *max_bits, uint64_t *count_out, T threshold) {
+void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
+                    T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw, T threshold) {
     if (n == 0) {
         if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
         return;
     }
     const uint32_t tiles = (uint32_t)compact_tiles(n);
-    hipLaunchKernelGGL((k_compact_count<T>), dim3(tiles), dim3(kCompThreads), 0, s, rates, n, threshold,
-                       tile_counts);
-    launch_scan_u64(s, tile_counts, tiles, nullptr, tile_offs, partial, tile_offs + tiles);
-    hipLaunchKernelGGL((k_compact_write<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, threshold,
-                       tile_offs, out_keys, out_rates, max_bits, count_out);
+    const uint32_t epoch = lw.next_epoch();
+    hipLaunchKernelGGL((k_compact_lb<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, threshold,
+                       out_keys, out_rates, max_bits, count_out, lw.status_uniq, lw.counters + 1, lw.claimed_uniq,
+                       epoch, lw.err);
+    lw.claimed_uniq += tiles;
 }
 
 template <typename T>
@@ -1229,9 +1232,8 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
     template void launch_hash_build<T>(hipStream_t, const uint64_t *, const T *, uint64_t, const uint64_t *,  \
                                        ulonglong2 *, uint64_t);                                               \
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
-    template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *,           \
-                                    uint64_t *, uint64_t *, uint64_t *, T *, unsigned long long *,            \
-                                    uint64_t *, T);                                                           \
+    template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *, T *,      \
+                                    unsigned long long *, uint64_t *, LookbackWork &, T);                     \
     template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
 
 EGTB_INSTANTIATE(float)

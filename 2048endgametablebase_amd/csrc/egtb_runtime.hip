@@ -658,17 +658,11 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     };
     bk.keys = (uint64_t *)carve(bound * 8, 0);
     bk.rates = carve(bound * sizeof(T), 1);
-    if (bound) {
-        const uint64_t tiles = compact_tiles(bound);
-        w.tile_counts.ensure(tiles * 8, s);
-        w.tile_offs.ensure((tiles + 1) * 8, s);
-        w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
-    }
+    if (bound) w.ensure_lookback(s, 0, 0, compact_tiles(bound));
     {
         Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2);
-        launch_compact<T>(s, keys, rates, bound, (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p,
-                          (uint64_t *)w.partial.p, bk.keys, (T *)bk.rates, (unsigned long long *)&md->maxbits,
-                          &md->n_live);
+        launch_compact<T>(s, keys, rates, bound, bk.keys, (T *)bk.rates, (unsigned long long *)&md->maxbits,
+                          &md->n_live, w.lw);
     }
     if (index) {
         bk.bits = index_bits_for(bound);
@@ -1731,14 +1725,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 thr_keys.ensure(nj * 8 + 8, s);
                 thr_rates.ensure(nj * sizeof(T) + 8, s);
                 thr_cnt.ensure(16, s);
-                const uint64_t tiles = compact_tiles(nj ? nj : 1);
-                w.tile_counts.ensure(tiles * 8, s);
-                w.tile_offs.ensure((tiles + 1) * 8, s);
-                w.partial.ensure(scan_partials_needed(tiles) * 8 + 64, s);
-                launch_compact<T>(s, res.books[j].keys, (const T *)res.books[j].rates, nj,
-                                  (uint64_t *)w.tile_counts.p, (uint64_t *)w.tile_offs.p, (uint64_t *)w.partial.p,
-                                  (uint64_t *)thr_keys.p, (T *)thr_rates.p, (unsigned long long *)thr_cnt.p + 1,
-                                  (uint64_t *)thr_cnt.p, thr);
+                w.ensure_lookback(s, 0, 0, compact_tiles(nj ? nj : 1));
+                launch_compact<T>(s, res.books[j].keys, (const T *)res.books[j].rates, nj, (uint64_t *)thr_keys.p,
+                                  (T *)thr_rates.p, (unsigned long long *)thr_cnt.p + 1, (uint64_t *)thr_cnt.p, w.lw,
+                                  thr);
                 uint64_t kept = 0;
                 HIPCHK(hipMemcpyAsync(&H[steps].n_live, thr_cnt.p, 8, hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
