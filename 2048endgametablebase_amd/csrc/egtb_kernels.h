@@ -105,6 +105,13 @@ void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint6
                    const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
                    uint64_t *or_and4, uint64_t offs_base = 0, uint64_t cap2 = ~0ull, uint64_t cap4 = ~0ull);
+// One-launch forward step (any candidate order): *tot (zero before the
+// launch) gets c2 | c4 << 32; children at or past cap2 / cap4 are counted,
+// not written.
+void launch_expand_alloc(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
+                         const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
+                         uint64_t *tot, uint64_t *out2, uint64_t *out4, uint64_t *or_and2, uint64_t *or_and4,
+                         uint64_t cap2, uint64_t cap4);
 void launch_gather_u64(hipStream_t s, const uint64_t *src, const uint64_t *idx, uint32_t m, uint64_t *out);
 void launch_copy_u64(hipStream_t s, const uint64_t *src, uint64_t *dst);
 

@@ -157,6 +157,126 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
     }
 }
 
+// A forward step in one launch, for candidates whose order does not matter
+// (a sort follows). Per block iteration (blockDim parents) each thread
+// counts its children, one atomic on *tot (c2 | c4 << 32) reserves the
+// block's run in out2 and in out4, and the children are generated again and
+// written. Writes at or past cap2 / cap4 are dropped while the counter still
+// counts them, so the totals are exact either way. The key spans of both
+// child streams are folded into or_and2 / or_and4.
+__global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restrict__ par, uint64_t n_bound,
+                                                       const uint64_t *__restrict__ n_ptr,
+                                                       const uint16_t *__restrict__ glut, PatternDev p,
+                                                       int do_check, unsigned long long *__restrict__ tot,
+                                                       uint64_t *__restrict__ out2, uint64_t *__restrict__ out4,
+                                                       uint64_t *__restrict__ or_and2,
+                                                       uint64_t *__restrict__ or_and4, uint64_t cap2,
+                                                       uint64_t cap4) {
+    extern __shared__ uint16_t lut[];
+    __shared__ uint64_t red[4][16];
+    __shared__ uint64_t scratch[16];
+    __shared__ uint64_t s_base;
+    stage_lut(lut, glut);
+    const uint64_t n = n_ptr ? (*n_ptr < n_bound ? *n_ptr : n_bound) : n_bound;
+    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const uint64_t i = i0 + threadIdx.x;
+        const uint64_t b = i < n ? par[i] : 0;
+        const bool live = i < n && !(do_check && is_success(b, p));
+        uint32_t c2 = 0, c4 = 0;
+        if (live) {
+            uint32_t em = empty_cells(b);
+            while (em) {
+                const int k = __builtin_ctz(em);
+                em &= em - 1;
+#pragma unroll
+                for (int s = 1; s <= 2; ++s) {
+                    const uint64_t sp = b | ((uint64_t)s << (4 * k));
+                    uint64_t mv[4];
+                    move4(sp, lut, mv);
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        if (mv[d] != sp && is_pattern(mv[d], p)) {
+                            const uint64_t c = canonical(mv[d], p.symm);
+                            if (s == 1) {
+                                or2 |= c;
+                                and2 &= c;
+                                ++c2;
+                            } else {
+                                or4 |= c;
+                                and4 &= c;
+                                ++c4;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        uint64_t block_total;
+        const uint64_t ex = block_exclusive_sum((uint64_t)c2 | ((uint64_t)c4 << 32), scratch, block_total);
+        if (threadIdx.x == 0) s_base = atomicAdd(tot, (unsigned long long)block_total);
+        __syncthreads();
+        const uint64_t at = s_base + ex;  // both halves: c2 and c4 of a step stay below 2^32
+        uint64_t w2 = at & 0xFFFFFFFFull, w4 = at >> 32;
+        __syncthreads();  // s_base is rewritten by the next iteration
+        if (live && (c2 | c4)) {
+            uint32_t em = empty_cells(b);
+            while (em) {
+                const int k = __builtin_ctz(em);
+                em &= em - 1;
+#pragma unroll
+                for (int s = 1; s <= 2; ++s) {
+                    const uint64_t sp = b | ((uint64_t)s << (4 * k));
+                    uint64_t mv[4];
+                    move4(sp, lut, mv);
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        if (mv[d] != sp && is_pattern(mv[d], p)) {
+                            const uint64_t c = canonical(mv[d], p.symm);
+                            if (s == 1) {
+                                if (w2 < cap2) out2[w2] = c;
+                                ++w2;
+                            } else {
+                                if (w4 < cap4) out4[w4] = c;
+                                ++w4;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // fold the span accumulators: wave, then one atomic per block
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        or2 |= __shfl_xor(or2, off, 64);
+        and2 &= __shfl_xor(and2, off, 64);
+        or4 |= __shfl_xor(or4, off, 64);
+        and4 &= __shfl_xor(and4, off, 64);
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        red[0][wave] = or2;
+        red[1][wave] = and2;
+        red[2][wave] = or4;
+        red[3][wave] = and4;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+            or2 |= red[0][w];
+            and2 &= red[1][w];
+            or4 |= red[2][w];
+            and4 &= red[3][w];
+        }
+        atomicOr((unsigned long long *)&or_and2[0], (unsigned long long)or2);
+        atomicAnd((unsigned long long *)&or_and2[1], (unsigned long long)and2);
+        atomicOr((unsigned long long *)&or_and4[0], (unsigned long long)or4);
+        atomicAnd((unsigned long long *)&or_and4[1], (unsigned long long)and4);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive scan (out[0..n) exclusive prefix, out[n] = total)
 // ---------------------------------------------------------------------------
@@ -1009,6 +1129,18 @@ void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint6
         hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
                            do_check, counts, offs, (uint64_t)0, out2, out4, or_and2, or_and4, (uint64_t)0,
                            (uint64_t)0);
+}
+
+void launch_expand_alloc(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
+                         const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
+                         uint64_t *tot, uint64_t *out2, uint64_t *out4, uint64_t *or_and2, uint64_t *or_and4,
+                         uint64_t cap2, uint64_t cap4) {
+    if (n_bound == 0) return;
+    const uint32_t want = blocks_for(n_bound, ls.threads);
+    const uint32_t grid = want < ls.blocks ? want : ls.blocks;
+    const size_t lds = kLutEntries * sizeof(uint16_t);
+    hipLaunchKernelGGL(k_expand_alloc, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
+                       (unsigned long long *)tot, out2, out4, or_and2, or_and4, cap2, cap4);
 }
 
 template <typename TI, typename TO>

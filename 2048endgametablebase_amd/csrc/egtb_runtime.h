@@ -74,7 +74,7 @@ struct HostStage {
 
 // Scratch of the dedup / compaction pipelines of one stream.
 struct SortWork {
-    DBuf counts, offsets, partial, tile_counts, tile_offs, maxbits;
+    DBuf counts, offsets, partial;
     DBuf bk_u64, bk_u32, bk_scratch, bk_ids;  // sample-sort dedup
     DBuf lb_status_sort, lb_status_uniq, lb_misc;
     LookbackWork lw;

@@ -1402,6 +1402,20 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         launch_scan_u64(s, (const uint64_t *)counts.p, bound, &M[step - 1].n_raw, (uint64_t *)offs.p,
                         (uint64_t *)w.partial.p, &M[step].tot);
     };
+    // one-pass expand of forward step `step` over layer step-1 (size on the
+    // device) into out2 / out4, up to cap2 / cap4 candidates; the totals it
+    // leaves in M[step].tot are exact even where writes were dropped
+    uint64_t spec_cap2 = 0, spec_cap4 = 0;  // capacities the last one-pass expand ran with
+    auto launch_alloc = [&](int step, uint64_t bound, uint64_t *out2, uint64_t *out4, uint64_t cap2,
+                            uint64_t cap4) {
+        HIPCHK(hipMemsetAsync(&M[step].tot, 0, 8, s));
+        spec_cap2 = cap2;
+        spec_cap4 = cap4;
+        Timer t(s, "expand", 0.0);  // its bytes are counted once per step, below
+        launch_expand_alloc(s, C.shape, res.raw[step - 1], bound, &M[step - 1].n_raw, C.lut, p,
+                            step > o.docheck_step ? 1 : 0, &M[step].tot, out2, out4, &M[step].or_all,
+                            &M[step + 1].or_all, cap2, cap4);
+    };
 
     // ---- chunked steps: runs (sorted unique pieces) of the layer being
     // assembled and of the next one; used once a step's candidates exceed
@@ -1457,10 +1471,11 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         return tot;
     };
 
-    // The sizes of step i (copied to H after its count pass, marked by
-    // meta_ev) are read while the device already runs the step's write,
-    // queued speculatively into the candidate buffers as they are; a step
-    // whose totals outgrow them grows them and writes again.
+    // Step i's one-pass expand is queued at the end of step i-1 into the
+    // candidate buffers as they are; its totals (copied to H, marked by
+    // meta_ev) tell whether every write landed. A step whose totals outgrew
+    // the buffers grows them and runs again. Chunked steps use the count
+    // pass and per-parent offsets instead.
     hipEvent_t meta_ev;
     HIPCHK(hipEventCreateWithFlags(&meta_ev, hipEventDisableTiming));
     struct EventGuard {
@@ -1472,10 +1487,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipEventRecord(meta_ev, s));
     };
-    bool spec_written = false;
     uint64_t c4_prev = 0;
     if (steps >= 3) {
-        launch_count(1, n_seeds);
+        launch_alloc(1, n_seeds, nullptr, nullptr, 0, 0);  // no buffers yet: totals and spans only
         queue_meta(0, 2);
     }
     for (int i = 1; i <= steps - 2; ++i) {
@@ -1505,11 +1519,12 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             cand[0].release();
             cand[1].release();
             sort_tmp.release();
+            launch_count(i, n_par);  // per-parent offsets of this step for its chunk ranges
         }
         uint64_t bound_next;
         if (!chunked) {
             DBuf &cur = cand[i & 1], &nxt = cand[(i + 1) & 1];
-            bool write = !spec_written;
+            bool write = c2 > spec_cap2 || c4 > spec_cap4;  // the one-pass expand dropped writes
             // candidate buffers keep a quarter of headroom over this step so
             // the next speculative write usually fits
             const uint64_t room = n_c + n_c / 4 + 64;
@@ -1530,13 +1545,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 nxt = std::move(g);
             }
             sort_tmp.ensure(n_c * 8, s);
-            if (spec_written && !write) stats_add_bytes("expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
-            if (write && n_par && (c2 || c4)) {
-                Timer t(s, "expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
-                launch_expand(s, true, C.shape, res.raw[i - 1], n_par, nullptr, C.lut, p, do_check, nullptr,
-                              (const uint64_t *)offs.p, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, nullptr,
-                              nullptr);
-            }
+            stats_add_bytes("expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
+            if (write && n_par && (c2 || c4))
+                launch_alloc(i, n_par, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, c2, c4);  // fits now
             res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
             dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all, res.raw[i],
                        &M[i].n_raw, o.radix_bits, w, o.sort_schedule);
@@ -1588,21 +1599,18 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             runs_next.clear();
             runs_next_n.clear();
         }
-        spec_written = false;
         if (i + 1 <= steps - 2) {
-            launch_count(i + 1, bound_next);
-            queue_meta(i, i + 2);
-            if (!chunked && bound_next) {
-                // step i+1's write, sized on the device; writes past the
-                // buffers are dropped and redone once the totals are known
+            if (!chunked) {
+                // step i+1 into the buffers as they are; it runs again if
+                // its totals outgrow them
                 const DBuf &cur1 = cand[(i + 1) & 1], &nxt1 = cand[i & 1];
                 const uint64_t cap2 = cur1.bytes / 8 > c4 ? cur1.bytes / 8 - c4 : 0, cap4 = nxt1.bytes / 8;
-                Timer t(s, "expand", 0.0);
-                launch_expand(s, true, C.shape, res.raw[i], bound_next, &M[i].n_raw, C.lut, p,
-                              i + 1 > o.docheck_step ? 1 : 0, nullptr, (const uint64_t *)offs.p,
-                              (uint64_t *)cur1.p + c4, (uint64_t *)nxt1.p, nullptr, nullptr, 0, cap2, cap4);
-                spec_written = true;
+                launch_alloc(i + 1, bound_next, cap2 ? (uint64_t *)cur1.p + c4 : nullptr, (uint64_t *)nxt1.p,
+                             cap2, cap4);
+            } else {
+                launch_count(i + 1, bound_next);
             }
+            queue_meta(i, i + 2);
         }
         HIPCHK(hipGetLastError());
         c4_prev = c4;
