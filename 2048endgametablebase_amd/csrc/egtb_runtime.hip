@@ -1406,9 +1406,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // device) into out2 / out4, up to cap2 / cap4 candidates; the totals it
     // leaves in M[step].tot are exact even where writes were dropped
     uint64_t spec_cap2 = 0, spec_cap4 = 0;  // capacities the last one-pass expand ran with
+    // (the per-layer records start zeroed, so only a rerun clears the counter)
     auto launch_alloc = [&](int step, uint64_t bound, uint64_t *out2, uint64_t *out4, uint64_t cap2,
-                            uint64_t cap4) {
-        HIPCHK(hipMemsetAsync(&M[step].tot, 0, 8, s));
+                            uint64_t cap4, bool rerun) {
+        if (rerun) HIPCHK(hipMemsetAsync(&M[step].tot, 0, 8, s));
         spec_cap2 = cap2;
         spec_cap4 = cap4;
         Timer t(s, "expand", 0.0);  // its bytes are counted once per step, below
@@ -1425,12 +1426,18 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     std::vector<uint64_t> runs_cur_n, runs_next_n;
     DBuf chunk2, chunk4, run_tmp, run_out, cnt_dev, gidx, gval;
     cnt_dev.alloc(64, s);
-    auto budget_keys = [&]() -> uint64_t {
+    // free HBM is read (a driver call) at the first step and again only
+    // when a step's candidates come within half of the last budget
+    uint64_t budget_cache = 0;
+    auto budget_keys = [&](uint64_t need) -> uint64_t {
         if (o.chunk_mkeys > 0) return (uint64_t)o.chunk_mkeys << 20;
-        size_t fr = 0, tot = 0;
-        HIPCHK(hipMemGetInfo(&fr, &tot));
-        // a chunk's two streams, the sort scratch and the unique output
-        return std::max<uint64_t>(1ull << 20, (uint64_t)(fr / 8 / 6));
+        if (budget_cache == 0 || need > budget_cache / 2) {
+            size_t fr = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fr, &tot));
+            // a chunk's two streams, the sort scratch and the unique output
+            budget_cache = std::max<uint64_t>(1ull << 20, (uint64_t)(fr / 8 / 6));
+        }
+        return budget_cache;
     };
     // sort + unique keys[0..n) into an exact-size run (synchronises)
     auto to_run = [&](uint64_t *keys, uint64_t n, uint64_t or_all, uint64_t and_all, std::vector<DBuf> &runs,
@@ -1489,7 +1496,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     };
     uint64_t c4_prev = 0;
     if (steps >= 3) {
-        launch_alloc(1, n_seeds, nullptr, nullptr, 0, 0);  // no buffers yet: totals and spans only
+        launch_alloc(1, n_seeds, nullptr, nullptr, 0, 0, false);  // no buffers yet: totals and spans only
         queue_meta(0, 2);
     }
     for (int i = 1; i <= steps - 2; ++i) {
@@ -1510,7 +1517,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         res.info[i].c4 = c4;
         const uint64_t n_c = c4_prev + c2;
         const int do_check = i > o.docheck_step ? 1 : 0;
-        const uint64_t budget = budget_keys();
+        const uint64_t budget = budget_keys(std::max(n_c, c4));
         if (!chunked && (n_c > budget || c4 > budget)) {
             // switch to runs: the spawn-4 prefix already in `cur` becomes a run
             chunked = true;
@@ -1547,7 +1554,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             sort_tmp.ensure(n_c * 8, s);
             stats_add_bytes("expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
             if (write && n_par && (c2 || c4))
-                launch_alloc(i, n_par, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, c2, c4);  // fits now
+                launch_alloc(i, n_par, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, c2, c4, true);  // fits now
             res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
             dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all, res.raw[i],
                        &M[i].n_raw, o.radix_bits, w, o.sort_schedule);
@@ -1606,7 +1613,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 const DBuf &cur1 = cand[(i + 1) & 1], &nxt1 = cand[i & 1];
                 const uint64_t cap2 = cur1.bytes / 8 > c4 ? cur1.bytes / 8 - c4 : 0, cap4 = nxt1.bytes / 8;
                 launch_alloc(i + 1, bound_next, cap2 ? (uint64_t *)cur1.p + c4 : nullptr, (uint64_t *)nxt1.p,
-                             cap2, cap4);
+                             cap2, cap4, false);
             } else {
                 launch_count(i + 1, bound_next);
             }

@@ -26,9 +26,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "solved states/s for 442@512 build (batched vs plain); query QPS at p99 latency"
-# single-kernel stats families and the kernels they time
-KERNEL_OF = {"radix_scatter": "egtb::k_radix_scatter<8, 16>", "radix_hist": "egtb::k_radix_hist<8, 16>",
-             "solve": "egtb::k_solve<float>", "expand": "egtb::k_expand<true>"}
+# single-kernel stats families and the kernel each times on this workload
+# (the radix families also hold the 4-key-per-thread tiles of the few small
+# layers; 8 keys per thread is what layers of 0.5-4 Mi candidates use)
+KERNEL_OF = {"radix_scatter": "egtb::k_radix_scatter<8, 8>", "radix_hist": "egtb::k_radix_hist<8, 8>",
+             "solve": "egtb::k_solve<float>", "expand": "egtb::k_expand_alloc"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_442_512_plain_build.md")
 
 
