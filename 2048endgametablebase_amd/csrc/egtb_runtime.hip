@@ -112,9 +112,12 @@ void DBuf::release() {
     bytes = 0;
 }
 
+// Grows by at least half: the forward pass's scratch follows layers that
+// grow a little every step, and each reallocation is host work between a
+// step's sizes and its first launch.
 void DBuf::ensure(size_t nbytes, hipStream_t s) {
     if (nbytes <= bytes && p) return;
-    alloc(nbytes + nbytes / 4 + 256, s);
+    alloc(std::max(nbytes + nbytes / 4, bytes + bytes / 2) + 256, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1533,21 +1536,25 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             DBuf &cur = cand[i & 1], &nxt = cand[(i + 1) & 1];
             bool write = c2 > spec_cap2 || c4 > spec_cap4;  // the one-pass expand dropped writes
             // candidate buffers keep a quarter of headroom over this step so
-            // the next speculative write usually fits
+            // the next speculative write usually fits, and grow by at least
+            // half their size: a reallocation (with its copy) sits between
+            // this step's sizes and its sort, so it must stay rare while the
+            // layers grow a little every step
             const uint64_t room = n_c + n_c / 4 + 64;
+            auto grown = [&](const DBuf &b) { return std::max<uint64_t>(room * 8 + 256, b.bytes + b.bytes / 2); };
             if (cur.bytes < n_c * 8) {  // grow, keeping the spawn-4 prefix
                 DBuf g;
-                g.alloc(room * 8 + 256, s);
+                g.alloc(grown(cur), s);
                 if (c4_prev) HIPCHK(hipMemcpyAsync(g.p, cur.p, c4_prev * 8, hipMemcpyDeviceToDevice, s));
                 cur = std::move(g);
                 write = true;
             }
             if (nxt.bytes < c4 * 8) {
-                nxt.alloc(std::max(c4, room) * 8 + 256, s);
+                nxt.alloc(std::max<uint64_t>(c4 * 8 + 256, grown(nxt)), s);
                 write = true;
             } else if (nxt.bytes < room * 8) {  // headroom for the next step, keeping what is written
                 DBuf g;
-                g.alloc(room * 8 + 256, s);
+                g.alloc(grown(nxt), s);
                 if (!write && c4) HIPCHK(hipMemcpyAsync(g.p, nxt.p, c4 * 8, hipMemcpyDeviceToDevice, s));
                 nxt = std::move(g);
             }
