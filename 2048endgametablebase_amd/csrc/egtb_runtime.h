@@ -233,6 +233,7 @@ struct Result {
     BookRing book_ring;           // books' device memory until they are on the host
     std::vector<DBuf> hash_bufs;  // per layer, freed after the layer's last solve
     double hash_max_bytes = 0;    // larger layers are searched, not hashed
+    bool book_index = true;       // books get the bucket index of the solve's search path
     std::vector<HostSlab> host_slabs;            // own the memory of host_keys / host_rates
     std::vector<HostView> host_keys, host_rates;  // per layer
     uint64_t streamed_bytes = 0, chunked_steps = 0, chunks = 0, peak_device_bytes = 0;

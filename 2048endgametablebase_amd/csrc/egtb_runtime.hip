@@ -602,8 +602,8 @@ static uint32_t index_bits_for(uint64_t n) {
 // The solve-side hash of book `layer` (at most `bound` live records, count
 // in device memory).
 // Layers whose hash would exceed this many MiB are searched instead: a
-// hash is built with one random atomic per record into a table twice the
-// layer's size, which outgrows the caches on the largest layers.
+// hash is built with one random atomic per record into a table four times
+// the layer's size, which outgrows the caches on the largest layers.
 constexpr int kDefaultSolveHashMB = 1024;
 
 template <typename T>
@@ -668,9 +668,11 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
                           &md->n_live, w.lw);
     }
     if (index) {
-        bk.bits = index_bits_for(bound);
-        bk.idx = (uint32_t *)carve(((1ull << bk.bits) + 1) * 4, 2);
-        launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
+        if (res.book_index) {
+            bk.bits = index_bits_for(bound);
+            bk.idx = (uint32_t *)carve(((1ull << bk.bits) + 1) * 4, 2);
+            launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
+        }
         build_hash<T>(s, res, layer, bound);
     }
 }
@@ -1662,6 +1664,16 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     counts.release();
     offs.release();
     const auto t_fwd = std::chrono::steady_clock::now();
+
+    // when every layer gets a solve hash the solve never searches, and the
+    // books go without the bucket index (a table set over the result makes
+    // its own)
+    {
+        bool all_hashed = res.hash_max_bytes > 0;
+        for (int i = 1; i < steps && all_hashed; ++i)
+            all_hashed = (double)hash_capacity(res.info[i].n_raw) * sizeof(ulonglong2) <= res.hash_max_bytes;
+        res.book_index = !all_hashed;
+    }
 
     // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
 
