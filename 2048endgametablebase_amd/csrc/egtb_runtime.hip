@@ -112,12 +112,15 @@ void DBuf::release() {
     bytes = 0;
 }
 
-// Grows by at least half: the forward pass's scratch follows layers that
-// grow a little every step, and each reallocation is host work between a
-// step's sizes and its first launch.
+// Grows by at least half, from 4 MiB: the forward pass's scratch follows
+// layers that grow a little every step from a few keys, and each
+// reallocation (tens of microseconds of host time) sits between a step's
+// sizes and its first launch.
+constexpr size_t kEnsureMin = (size_t)4 << 20;
+
 void DBuf::ensure(size_t nbytes, hipStream_t s) {
     if (nbytes <= bytes && p) return;
-    alloc(std::max(nbytes + nbytes / 4, bytes + bytes / 2) + 256, s);
+    alloc(std::max({nbytes + nbytes / 4, bytes + bytes / 2, kEnsureMin}) + 256, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1355,7 +1358,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                          : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
     SortWork w;
     HostStage stage, meta_host_buf;
-    DBuf cand[2], sort_tmp, counts, offs, rates, packed;
+    DBuf cand[2], counts, offs, rates, packed;
 
     const auto t_begin = std::chrono::steady_clock::now();
 
@@ -1530,7 +1533,6 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 to_run((uint64_t *)cand[i & 1].p, c4_prev, H[i].or_all, H[i].and_all, runs_cur, runs_cur_n);
             cand[0].release();
             cand[1].release();
-            sort_tmp.release();
             launch_count(i, n_par);  // per-parent offsets of this step for its chunk ranges
         }
         uint64_t bound_next;
@@ -1543,7 +1545,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             // this step's sizes and its sort, so it must stay rare while the
             // layers grow a little every step
             const uint64_t room = n_c + n_c / 4 + 64;
-            auto grown = [&](const DBuf &b) { return std::max<uint64_t>(room * 8 + 256, b.bytes + b.bytes / 2); };
+            auto grown = [&](const DBuf &b) {
+                return std::max<uint64_t>({room * 8 + 256, b.bytes + b.bytes / 2, kEnsureMin});
+            };
             if (cur.bytes < n_c * 8) {  // grow, keeping the spawn-4 prefix
                 DBuf g;
                 g.alloc(grown(cur), s);
@@ -1560,13 +1564,17 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 if (!write && c4) HIPCHK(hipMemcpyAsync(g.p, nxt.p, c4 * 8, hipMemcpyDeviceToDevice, s));
                 nxt = std::move(g);
             }
-            sort_tmp.ensure(n_c * 8, s);
             stats_add_bytes("expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
             if (write && n_par && (c2 || c4))
                 launch_alloc(i, n_par, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, c2, c4, true);  // fits now
             res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all, res.raw[i],
-                       &M[i].n_raw, o.radix_bits, w, o.sort_schedule);
+            // the sort's second buffer is the arena's next n_c keys, handed
+            // back (in stream order) once the layer is unique: no pool
+            // allocation as the layers grow
+            uint64_t *tmp = (uint64_t *)res.raw_arena.alloc(n_c * 8);
+            dedup_into(s, (uint64_t *)cur.p, tmp, n_c, H[i].or_all, H[i].and_all, res.raw[i], &M[i].n_raw,
+                       o.radix_bits, w, o.sort_schedule);
+            res.raw_arena.trim_last(tmp, 0);
             bound_next = n_c;
         } else {
             // parent ranges whose candidates fit the budget, from the prefix
@@ -1642,10 +1650,11 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         if (!chunked) {
             const uint64_t n_c = c4_prev;
             DBuf &cur = cand[L & 1];
-            sort_tmp.ensure(n_c * 8, s);
             res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all, res.raw[L],
-                       &M[L].n_raw, o.radix_bits, w, o.sort_schedule);
+            uint64_t *tmp = (uint64_t *)res.raw_arena.alloc(n_c * 8);
+            dedup_into(s, (uint64_t *)cur.p, tmp, n_c, H[L].or_all, H[L].and_all, res.raw[L], &M[L].n_raw,
+                       o.radix_bits, w, o.sort_schedule);
+            res.raw_arena.trim_last(tmp, 0);
         } else {
             merge_runs(L, runs_cur, runs_cur_n);
         }
@@ -1660,7 +1669,6 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     run_out.release();
     cand[0].release();
     cand[1].release();
-    sort_tmp.release();
     counts.release();
     offs.release();
     const auto t_fwd = std::chrono::steady_clock::now();
