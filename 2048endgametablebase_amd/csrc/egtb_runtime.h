@@ -231,7 +231,7 @@ struct Result {
     // no longer needed, then in host memory
     bool host_tables = false;
     BookRing book_ring;           // books' device memory until they are on the host
-    std::vector<DBuf> hash_bufs;  // per layer, freed after the layer's last solve
+    std::vector<DBuf> hash_bufs;  // ring of 3 solve hashes: layer i's in slot i % 3
     double hash_max_bytes = 0;    // larger layers are searched, not hashed
     bool book_index = true;       // books get the bucket index of the solve's search path
     std::vector<HostSlab> host_slabs;            // own the memory of host_keys / host_rates

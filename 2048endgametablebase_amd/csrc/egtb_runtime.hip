@@ -614,8 +614,11 @@ static void build_hash(hipStream_t s, Result &res, int layer, uint64_t bound) {
     Book &bk = res.books[layer];
     const uint64_t cap = hash_capacity(bound);
     if ((double)cap * sizeof(ulonglong2) > res.hash_max_bytes) return;  // the solve searches this layer
-    DBuf &h = res.hash_bufs[layer];
-    h.alloc(cap * sizeof(ulonglong2), s);
+    // at most three hashes are alive (layers i, i+1, i+2 while layer i is
+    // solved): layer i takes ring slot i % 3, whose previous table (layer
+    // i+3) was last read by the solve of layer i+1, queued before
+    DBuf &h = res.hash_bufs[layer % 3];
+    h.ensure(cap * sizeof(ulonglong2), s);
     Timer t(s, "hash", (double)bound * (8 + sizeof(T)) + (double)cap * sizeof(ulonglong2));
     launch_hash_build<T>(s, bk.keys, (const T *)bk.rates, bound, bk.n_dev, (ulonglong2 *)h.p, cap);
     bk.hslots = (const ulonglong2 *)h.p;
@@ -643,7 +646,6 @@ static HashDev hash_of(const Book &bk) {
 }
 
 static void drop_hash(Result &res, int layer) {
-    res.hash_bufs[layer].release();
     res.books[layer].hslots = nullptr;
     res.books[layer].hmask = 0;
 }
@@ -1353,7 +1355,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T), steps);
     }
     res.hash_bufs.clear();
-    res.hash_bufs.resize(steps);
+    res.hash_bufs.resize(3);
     res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
                          : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
     SortWork w;
@@ -1783,6 +1785,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         }
     }
     if (steps > 1) drop_hash(res, 1);
+    for (DBuf &h : res.hash_bufs) h.release();  // the solve is done with them
     if (streamer) {
         streamer->retire(1, s);
         streamer->retire(0, s);
