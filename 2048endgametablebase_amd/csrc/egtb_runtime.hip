@@ -1360,7 +1360,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                          : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
     SortWork w;
     HostStage stage, meta_host_buf;
-    DBuf cand[2], counts, offs, rates, packed;
+    DBuf cand[2], sort_tmp, counts, offs, rates, packed;
 
     const auto t_begin = std::chrono::steady_clock::now();
 
@@ -1535,6 +1535,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 to_run((uint64_t *)cand[i & 1].p, c4_prev, H[i].or_all, H[i].and_all, runs_cur, runs_cur_n);
             cand[0].release();
             cand[1].release();
+            sort_tmp.release();
             launch_count(i, n_par);  // per-parent offsets of this step for its chunk ranges
         }
         uint64_t bound_next;
@@ -1569,14 +1570,14 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             stats_add_bytes("expand", 8.0 * (double)n_par + 8.0 * (double)(c2 + c4));
             if (write && n_par && (c2 || c4))
                 launch_alloc(i, n_par, (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, c2, c4, true);  // fits now
+            // (the sort's second buffer stays out of the raw arena: a layer
+            // must remain the arena's newest allocation to be trimmed to its
+            // size, and a larger block per step would leave the arena's 1 GiB
+            // chunks mostly unused on large layers)
+            sort_tmp.ensure(n_c * 8, s);
             res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            // the sort's second buffer is the arena's next n_c keys, handed
-            // back (in stream order) once the layer is unique: no pool
-            // allocation as the layers grow
-            uint64_t *tmp = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            dedup_into(s, (uint64_t *)cur.p, tmp, n_c, H[i].or_all, H[i].and_all, res.raw[i], &M[i].n_raw,
-                       o.radix_bits, w, o.sort_schedule);
-            res.raw_arena.trim_last(tmp, 0);
+            dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all, res.raw[i],
+                       &M[i].n_raw, o.radix_bits, w, o.sort_schedule);
             bound_next = n_c;
         } else {
             // parent ranges whose candidates fit the budget, from the prefix
@@ -1652,11 +1653,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         if (!chunked) {
             const uint64_t n_c = c4_prev;
             DBuf &cur = cand[L & 1];
+            sort_tmp.ensure(n_c * 8, s);
             res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            uint64_t *tmp = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-            dedup_into(s, (uint64_t *)cur.p, tmp, n_c, H[L].or_all, H[L].and_all, res.raw[L], &M[L].n_raw,
-                       o.radix_bits, w, o.sort_schedule);
-            res.raw_arena.trim_last(tmp, 0);
+            dedup_into(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all, res.raw[L],
+                       &M[L].n_raw, o.radix_bits, w, o.sort_schedule);
         } else {
             merge_runs(L, runs_cur, runs_cur_n);
         }
@@ -1667,6 +1667,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     }
     chunk2.release();
     chunk4.release();
+    sort_tmp.release();
     run_tmp.release();
     run_out.release();
     cand[0].release();
