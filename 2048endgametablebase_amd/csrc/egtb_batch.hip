@@ -73,6 +73,7 @@ __global__ __launch_bounds__(1024) void k_bexpand(const uint64_t *__restrict__ p
         }
         if (step <= J.steps - 2 && !(step > J.docheck && is_success(b, J.p))) {
             uint32_t em = empty_cells(b);
+            const SpawnMoves smv(b, lut);
             while (em) {
                 const int k = __builtin_ctz(em);
                 em &= em - 1;
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_bexpand(const uint64_t *__restrict__ p
                 for (int s = 1; s <= 2; ++s) {
                     const uint64_t sp = b | ((uint64_t)s << (4 * k));
                     uint64_t mv[4];
-                    move4(sp, lut, mv);
+                    smv.of(k, (uint64_t)s, lut, mv);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
                         if (mv[d] != sp && is_pattern(mv[d], J.p)) {
@@ -162,6 +163,7 @@ __global__ __launch_bounds__(1024) void k_bsolve(const uint64_t *__restrict__ ke
         double sp = 0.0;
         uint32_t em = empty_cells(b);
         const int e = __popc(em);
+        const SpawnMoves smv(b, lut);
         while (em) {
             const int k = __builtin_ctz(em);
             em &= em - 1;
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(1024) void k_bsolve(const uint64_t *__restrict__ ke
             for (int s = 0; s < 2; ++s) {
                 const uint64_t spb = b | ((uint64_t)(s + 1) << (4 * k));
                 uint64_t mv[4];
-                move4(spb, lut, mv);
+                smv.of(k, (uint64_t)(s + 1), lut, mv);
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     const int j = s * 4 + d;

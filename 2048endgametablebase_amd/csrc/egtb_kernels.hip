@@ -97,6 +97,7 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
         }
         if (!(do_check && is_success(b, p))) {
             uint32_t em = empty_cells(b);
+            const SpawnMoves smv(b, lut);
             while (em) {
                 const int k = __builtin_ctz(em);
                 em &= em - 1;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                 for (int s = 1; s <= 2; ++s) {
                     const uint64_t sp = b | ((uint64_t)s << (4 * k));
                     uint64_t mv[4];
-                    move4(sp, lut, mv);
+                    smv.of(k, (uint64_t)s, lut, mv);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
                         if (mv[d] != sp && is_pattern(mv[d], p)) {
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
         const uint64_t i = i0 + threadIdx.x;
         const uint64_t b = i < n ? par[i] : 0;
         const bool live = i < n && !(do_check && is_success(b, p));
+        const SpawnMoves smv(b, lut);  // for both passes
         uint32_t c2 = 0, c4 = 0;
         if (live) {
             uint32_t em = empty_cells(b);
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
                 for (int s = 1; s <= 2; ++s) {
                     const uint64_t sp = b | ((uint64_t)s << (4 * k));
                     uint64_t mv[4];
-                    move4(sp, lut, mv);
+                    smv.of(k, (uint64_t)s, lut, mv);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
                         if (mv[d] != sp && is_pattern(mv[d], p)) {
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
                 for (int s = 1; s <= 2; ++s) {
                     const uint64_t sp = b | ((uint64_t)s << (4 * k));
                     uint64_t mv[4];
-                    move4(sp, lut, mv);
+                    smv.of(k, (uint64_t)s, lut, mv);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
                         if (mv[d] != sp && is_pattern(mv[d], p)) {
@@ -876,6 +878,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
         double sp = 0.0;
         uint32_t em = empty_cells(b);
         const int e = __popc(em);
+        const SpawnMoves smv(b, lut);
         while (em) {
             const int k = __builtin_ctz(em);
             em &= em - 1;
@@ -888,7 +891,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
             for (int s = 0; s < 2; ++s) {
                 const uint64_t sp_board = b | ((uint64_t)(s + 1) << (4 * k));
                 uint64_t mv[4];
-                move4(sp_board, lut, mv);
+                smv.of(k, (uint64_t)(s + 1), lut, mv);
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     const int j = s * 4 + d;

@@ -67,6 +67,7 @@ __device__ double fallback_value(uint64_t moved, const uint16_t *lut, double p4)
     const int e = __popc(em);
     if (e == 0) return fallback_h(moved);
     double acc = 0.0;
+    const SpawnMoves smv(moved, lut);
     while (em) {
         const int k = __builtin_ctz(em);
         em &= em - 1;
@@ -74,7 +75,7 @@ __device__ double fallback_value(uint64_t moved, const uint16_t *lut, double p4)
         for (int s = 1; s <= 2; ++s) {
             const uint64_t sp = moved | ((uint64_t)s << (4 * k));
             uint64_t mv[4];
-            move4(sp, lut, mv);
+            smv.of(k, (uint64_t)s, lut, mv);
             double best = 0.0;
 #pragma unroll
             for (int d = 0; d < 4; ++d)
