@@ -44,8 +44,8 @@ __device__ __forceinline__ T table_lookup_dev(const TableDev<T> &t, uint64_t key
 // Open-addressing hash of one solved layer, the structure the backward solve
 // looks successors up in: 16-byte slots {key, rate bits}, key 0 marks an
 // empty slot (every stored board holds tiles, so no key is 0), linear
-// probing from hash_mix(key).  The capacity is a power of two at least twice
-// the layer's size bound, so every probe run ends at an empty slot; a
+// probing from hash_mix(key).  The capacity is a power of two at least four
+// times the layer's size bound, so every probe run ends at an empty slot; a
 // lookup usually reads one cache line.
 struct HashDev {
     const ulonglong2 *slots;
@@ -173,12 +173,13 @@ void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const P
 
 // Records with rate > threshold, in order, in one look-back launch; it
 // shares the unique kernel's status words (ensure_lookback with
-// compact_tiles(n) unique tiles first).
+// compact_tiles(n) unique tiles first). With hslots (hcap slots, a power
+// of two), the table is cleared and the kept records hashed into it too.
 uint64_t compact_tiles(uint64_t n);
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
                     T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw,
-                    T threshold = T(0));
+                    T threshold = T(0), ulonglong2 *hslots = nullptr, uint64_t hcap = 0);
 
 template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);

@@ -1006,7 +1006,10 @@ constexpr int kCompTile = kCompThreads * kCompItems;
 // Keep the records with rate > thr, in order, in one launch: tiles are
 // claimed in order by a ticket, count their kept records, publish the count
 // and take their output offset by decoupled look-back. max_bits gets the
-// bits of the largest rate; count_out the number kept.
+// bits of the largest rate; count_out the number kept. With hslots, every
+// kept record also goes into that cleared open-addressing table (the solve
+// hash of the layer: keys are unique, so a claim ends at the first empty
+// slot of its run).
 template <typename T>
 __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__restrict__ keys,
                                                              const T *__restrict__ rates, uint64_t n, T thr,
@@ -1017,7 +1020,8 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
                                                              uint64_t *__restrict__ status,
                                                              unsigned long long *__restrict__ tile_counter,
                                                              unsigned long long tile_base, uint32_t epoch,
-                                                             uint32_t *__restrict__ err) {
+                                                             uint32_t *__restrict__ err,
+                                                             ulonglong2 *__restrict__ hslots, uint64_t hmask) {
     __shared__ uint64_t scratch[kCompThreads / 64];
     __shared__ uint32_t s_tile;
     __shared__ uint64_t s_off;
@@ -1054,9 +1058,16 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
 #pragma unroll
     for (int j = 0; j < kCompItems; ++j) {
         if (r[j] > thr) {
-            out_keys[pos] = keys[base + j];
+            const uint64_t k = keys[base + j];
+            out_keys[pos] = k;
             out_rates[pos] = r[j];
             ++pos;
+            if (hslots) {
+                uint64_t sl = hash_mix(k) & hmask;
+                while (atomicCAS((unsigned long long *)&hslots[sl].x, 0ull, (unsigned long long)k) != 0ull)
+                    sl = (sl + 1) & hmask;
+                hslots[sl].y = rate_bits(r[j]);
+            }
         }
     }
     // non-negative IEEE floats and unsigned ints both order as their bits
@@ -1335,7 +1346,9 @@ uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
 
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
-                    T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw, T threshold) {
+                    T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw, T threshold,
+                    ulonglong2 *hslots, uint64_t hcap) {
+    if (hslots) (void)hipMemsetAsync(hslots, 0, hcap * sizeof(ulonglong2), s);
     if (n == 0) {
         if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
         return;
@@ -1344,7 +1357,7 @@ void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_
     const uint32_t epoch = lw.next_epoch();
     hipLaunchKernelGGL((k_compact_lb<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, threshold,
                        out_keys, out_rates, max_bits, count_out, lw.status_uniq, lw.counters + 1, lw.claimed_uniq,
-                       epoch, lw.err);
+                       epoch, lw.err, hslots, hslots ? hcap - 1 : 0ull);
     lw.claimed_uniq += tiles;
 }
 
@@ -1378,7 +1391,8 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
                                        ulonglong2 *, uint64_t);                                               \
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *, T *,      \
-                                    unsigned long long *, uint64_t *, LookbackWork &, T);                     \
+                                    unsigned long long *, uint64_t *, LookbackWork &, T, ulonglong2 *,        \
+                                    uint64_t);                                                                \
     template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
 
 EGTB_INSTANTIATE(float)

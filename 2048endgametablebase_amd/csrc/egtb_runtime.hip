@@ -609,20 +609,33 @@ static uint32_t index_bits_for(uint64_t n) {
 // the layer's size, which outgrows the caches on the largest layers.
 constexpr int kDefaultSolveHashMB = 1024;
 
-template <typename T>
-static void build_hash(hipStream_t s, Result &res, int layer, uint64_t bound) {
+// The table of book `layer`'s solve hash (its slots and mask go on the
+// book; `cap` gets its capacity), or nullptr when the layer is too large to
+// hash and the solve searches it instead.
+static ulonglong2 *hash_table_for(hipStream_t s, Result &res, int layer, uint64_t bound, uint64_t &cap) {
     Book &bk = res.books[layer];
-    const uint64_t cap = hash_capacity(bound);
-    if ((double)cap * sizeof(ulonglong2) > res.hash_max_bytes) return;  // the solve searches this layer
+    cap = hash_capacity(bound);
+    if ((double)cap * sizeof(ulonglong2) > res.hash_max_bytes) return nullptr;
     // at most three hashes are alive (layers i, i+1, i+2 while layer i is
     // solved): layer i takes ring slot i % 3, whose previous table (layer
     // i+3) was last read by the solve of layer i+1, queued before
     DBuf &h = res.hash_bufs[layer % 3];
     h.ensure(cap * sizeof(ulonglong2), s);
-    Timer t(s, "hash", (double)bound * (8 + sizeof(T)) + (double)cap * sizeof(ulonglong2));
-    launch_hash_build<T>(s, bk.keys, (const T *)bk.rates, bound, bk.n_dev, (ulonglong2 *)h.p, cap);
     bk.hslots = (const ulonglong2 *)h.p;
     bk.hmask = cap - 1;
+    return (ulonglong2 *)h.p;
+}
+
+// The solve hash of a book that is already in place (one loaded from its
+// file); solved layers get theirs from the compaction that makes them.
+template <typename T>
+static void build_hash(hipStream_t s, Result &res, int layer, uint64_t bound) {
+    uint64_t cap = 0;
+    ulonglong2 *slots = hash_table_for(s, res, layer, bound, cap);
+    if (!slots) return;
+    const Book &bk = res.books[layer];
+    Timer t(s, "hash", (double)bound * (8 + sizeof(T)) + (double)cap * sizeof(ulonglong2));
+    launch_hash_build<T>(s, bk.keys, (const T *)bk.rates, bound, bk.n_dev, slots, cap);
 }
 
 template <typename T>
@@ -667,18 +680,19 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     bk.keys = (uint64_t *)carve(bound * 8, 0);
     bk.rates = carve(bound * sizeof(T), 1);
     if (bound) w.ensure_lookback(s, 0, 0, compact_tiles(bound));
+    // the compaction also fills the layer's solve hash (read by the solves
+    // of layers i-1 and i-2; layer 0 is read by none)
+    uint64_t hcap = 0;
+    ulonglong2 *hslots = index ? hash_table_for(s, res, layer, bound, hcap) : nullptr;
     {
-        Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2);
+        Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2 + (double)hcap * sizeof(ulonglong2));
         launch_compact<T>(s, keys, rates, bound, bk.keys, (T *)bk.rates, (unsigned long long *)&md->maxbits,
-                          &md->n_live, w.lw);
+                          &md->n_live, w.lw, T(0), hslots, hcap);
     }
-    if (index) {
-        if (res.book_index) {
-            bk.bits = index_bits_for(bound);
-            bk.idx = (uint32_t *)carve(((1ull << bk.bits) + 1) * 4, 2);
-            launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
-        }
-        build_hash<T>(s, res, layer, bound);
+    if (index && res.book_index) {
+        bk.bits = index_bits_for(bound);
+        bk.idx = (uint32_t *)carve(((1ull << bk.bits) + 1) * 4, 2);
+        launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
     }
 }
 
