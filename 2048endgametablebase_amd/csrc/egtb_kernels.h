@@ -112,6 +112,19 @@ void launch_expand_alloc(hipStream_t s, const LaunchShape &ls, const uint64_t *p
                          const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                          uint64_t *tot, uint64_t *out2, uint64_t *out4, uint64_t *or_and2, uint64_t *or_and4,
                          uint64_t cap2, uint64_t cap4);
+// Forward step into key sets (open addressing, 0 = empty, capacities are
+// powers of two): spawn-2 children into set2, spawn-4 children into set4
+// (either may be null: counted, not inserted). *tot gets c2 | c4 << 32;
+// meta2 / meta4 point at the LayerDev of layers step and step+1: their
+// or_all / and_all fold the children's keys and pad[0] becomes nonzero when
+// an insert was dropped (the set was too full).
+void launch_expand_hash(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
+                        const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *tot,
+                        uint64_t *set2, uint64_t cap2, uint64_t *set4, uint64_t cap4, uint64_t *meta2,
+                        uint64_t *meta4);
+// Keys of a set to out[0..*count) in any order; the set's slots are cleared.
+// *count must be zero before; out must hold cap keys.
+void launch_set_extract(hipStream_t s, uint64_t *set, uint64_t cap, uint64_t *out, uint64_t *count);
 void launch_gather_u64(hipStream_t s, const uint64_t *src, const uint64_t *idx, uint32_t m, uint64_t *out);
 void launch_copy_u64(hipStream_t s, const uint64_t *src, uint64_t *dst);
 

@@ -280,6 +280,157 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
 }
 
 // ---------------------------------------------------------------------------
+// Forward step into hash sets: the children of a step are inserted into the
+// open-addressing key sets of their layers (spawn 2: layer step, spawn 4:
+// layer step+1) instead of being written out for a sort. Slots hold the key
+// itself, 0 = empty (a spawned board is never 0); linear probing from
+// hash_mix. A slot only ever goes from 0 to a key within a launch, so a
+// plain load that sees a key is exact and one that sees 0 is settled by the
+// CAS. An insert that finds kSetProbeLimit slots in a row taken raises the
+// layer's overflow flag and drops the key: the host then regrows the set and
+// runs the step again.
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kSetProbeLimit = 256;
+
+__device__ __forceinline__ bool set_insert(uint64_t *__restrict__ tab, uint64_t mask, uint64_t k) {
+    uint64_t h = hash_mix(k) & mask;
+    for (uint32_t q = 0; q < kSetProbeLimit; ++q) {
+        const uint64_t cur = tab[h];
+        if (cur == k) return true;
+        if (cur == 0) {
+            const uint64_t old = atomicCAS((unsigned long long *)&tab[h], 0ull, (unsigned long long)k);
+            if (old == 0 || old == k) return true;
+        }
+        h = (h + 1) & mask;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict__ par, uint64_t n_bound,
+                                                      const uint64_t *__restrict__ n_ptr,
+                                                      const uint16_t *__restrict__ glut, PatternDev p, int do_check,
+                                                      unsigned long long *__restrict__ tot, uint64_t *__restrict__ set2,
+                                                      uint64_t mask2, uint64_t *__restrict__ set4, uint64_t mask4,
+                                                      uint64_t *__restrict__ meta2, uint64_t *__restrict__ meta4) {
+    extern __shared__ uint16_t lut[];
+    __shared__ uint64_t red[4][16];
+    __shared__ uint64_t wtot[16];
+    __shared__ uint32_t s_fail;
+    stage_lut(lut, glut);
+    if (threadIdx.x == 0) s_fail = 0;
+    __syncthreads();
+    const uint64_t n = n_ptr ? (*n_ptr < n_bound ? *n_ptr : n_bound) : n_bound;
+    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull, count = 0;
+    uint32_t fail = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t b = par[i];
+        if (do_check && is_success(b, p)) continue;
+        const SpawnMoves smv(b, lut);
+        uint32_t em = empty_cells(b);
+        while (em) {
+            const int k = __builtin_ctz(em);
+            em &= em - 1;
+#pragma unroll
+            for (int sv = 1; sv <= 2; ++sv) {
+                const uint64_t sp = b | ((uint64_t)sv << (4 * k));
+                uint64_t mv[4];
+                smv.of(k, (uint64_t)sv, lut, mv);
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    if (mv[d] != sp && is_pattern(mv[d], p)) {
+                        const uint64_t c = canonical(mv[d], p.symm);
+                        if (sv == 1) {
+                            or2 |= c;
+                            and2 &= c;
+                            count += 1;
+                            if (set2 && !set_insert(set2, mask2, c)) fail |= 1u;
+                        } else {
+                            or4 |= c;
+                            and4 &= c;
+                            count += 1ull << 32;
+                            if (set4 && !set_insert(set4, mask4, c)) fail |= 2u;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // fold: wave shuffles, then one set of atomics per block
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        or2 |= __shfl_xor(or2, off, 64);
+        and2 &= __shfl_xor(and2, off, 64);
+        or4 |= __shfl_xor(or4, off, 64);
+        and4 &= __shfl_xor(and4, off, 64);
+        count += __shfl_xor(count, off, 64);
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    if (fail) atomicOr(&s_fail, fail);
+    if (lane_id() == 0) {
+        red[0][wave] = or2;
+        red[1][wave] = and2;
+        red[2][wave] = or4;
+        red[3][wave] = and4;
+        wtot[wave] = count;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < (blockDim.x >> 6); ++w) {
+            or2 |= red[0][w];
+            and2 &= red[1][w];
+            or4 |= red[2][w];
+            and4 &= red[3][w];
+            count += wtot[w];
+        }
+        if (count) atomicAdd(tot, (unsigned long long)count);
+        atomicOr((unsigned long long *)&meta2[0], (unsigned long long)or2);
+        atomicAnd((unsigned long long *)&meta2[1], (unsigned long long)and2);
+        atomicOr((unsigned long long *)&meta4[0], (unsigned long long)or4);
+        atomicAnd((unsigned long long *)&meta4[1], (unsigned long long)and4);
+        if (s_fail & 1u) atomicOr((unsigned long long *)&meta2[4], 1ull);
+        if (s_fail & 2u) atomicOr((unsigned long long *)&meta4[4], 1ull);
+    }
+}
+
+// Moves the keys of a set to out[] (any order) and clears their slots, so
+// the set is empty again for its next layer; *count (zero before) gets the
+// number of keys.
+constexpr int kExtractThreads = 256;
+constexpr int kExtractItems = 8;
+constexpr int kExtractTile = kExtractThreads * kExtractItems;
+
+__global__ __launch_bounds__(kExtractThreads) void k_set_extract(uint64_t *__restrict__ set, uint64_t cap,
+                                                                 uint64_t *__restrict__ out,
+                                                                 unsigned long long *__restrict__ count) {
+    __shared__ uint64_t stage[kExtractTile];
+    __shared__ uint32_t scratch[kExtractThreads / 64];
+    __shared__ unsigned long long s_base;
+    const uint64_t base = (uint64_t)blockIdx.x * kExtractTile;
+    uint64_t v[kExtractItems];
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kExtractItems; ++j) {
+        const uint64_t idx = base + (uint64_t)j * kExtractThreads + threadIdx.x;
+        v[j] = idx < cap ? set[idx] : 0ull;
+        if (v[j]) {
+            ++c;
+            set[idx] = 0;
+        }
+    }
+    uint32_t total;
+    uint32_t at = block_exclusive_sum(c, scratch, total);
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(count, (unsigned long long)total) : 0ull;
+#pragma unroll
+    for (int j = 0; j < kExtractItems; ++j)
+        if (v[j]) stage[at++] = v[j];
+    __syncthreads();
+    const unsigned long long b0 = s_base;
+    for (uint32_t q = threadIdx.x; q < total; q += kExtractThreads) out[b0 + q] = stage[q];
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive scan (out[0..n) exclusive prefix, out[n] = total)
 // ---------------------------------------------------------------------------
 
@@ -1160,6 +1311,25 @@ void launch_expand_alloc(hipStream_t s, const LaunchShape &ls, const uint64_t *p
     const size_t lds = kLutEntries * sizeof(uint16_t);
     hipLaunchKernelGGL(k_expand_alloc, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
                        (unsigned long long *)tot, out2, out4, or_and2, or_and4, cap2, cap4);
+}
+
+void launch_expand_hash(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
+                        const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *tot,
+                        uint64_t *set2, uint64_t cap2, uint64_t *set4, uint64_t cap4, uint64_t *meta2,
+                        uint64_t *meta4) {
+    if (n_bound == 0) return;
+    const uint32_t want = blocks_for(n_bound, ls.threads);
+    const uint32_t grid = want < ls.blocks ? want : ls.blocks;
+    const size_t lds = kLutEntries * sizeof(uint16_t);
+    hipLaunchKernelGGL(k_expand_hash, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
+                       (unsigned long long *)tot, set2, set2 ? cap2 - 1 : 0ull, set4, set4 ? cap4 - 1 : 0ull, meta2,
+                       meta4);
+}
+
+void launch_set_extract(hipStream_t s, uint64_t *set, uint64_t cap, uint64_t *out, uint64_t *count) {
+    if (cap == 0) return;
+    hipLaunchKernelGGL(k_set_extract, dim3(blocks_for(cap, kExtractTile)), dim3(kExtractThreads), 0, s, set, cap, out,
+                       (unsigned long long *)count);
 }
 
 template <typename TI, typename TO>

@@ -237,6 +237,7 @@ struct Result {
     std::vector<HostSlab> host_slabs;            // own the memory of host_keys / host_rates
     std::vector<HostView> host_keys, host_rates;  // per layer
     uint64_t streamed_bytes = 0, chunked_steps = 0, chunks = 0, peak_device_bytes = 0;
+    uint64_t hash_steps = 0, hash_regrows = 0;  // forward steps deduplicated in key sets; set regrowths
     double stream_ms = 0;
     LayerDev *meta_dev() const { return (LayerDev *)meta.p; }
 };

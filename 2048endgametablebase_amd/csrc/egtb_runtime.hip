@@ -128,6 +128,9 @@ void DBuf::ensure(size_t nbytes, hipStream_t s) {
 // ---------------------------------------------------------------------------
 
 static std::atomic<int> g_stats_on{0};
+// slots per predicted key of the forward pass's key sets
+constexpr double kSetSlotsPerKey = 3.0;
+static std::atomic<double> g_set_slots_per_key{kSetSlotsPerKey};
 static std::mutex g_stats_mu;
 struct StatAcc {
     uint64_t launches = 0;
@@ -1122,7 +1125,8 @@ class BookStreamer {
 // write_binary_vector_direct, FileIOUtils.h:444-451) written by a thread of
 // their own while the forward pass goes on: device-to-host copies on a
 // private stream into pinned slots, fwrite of one slot while the next fills.
-// The layers are final (their step has synchronised) when they are queued.
+// A layer is queued with its producer's stream: the copies wait for what that
+// stream had queued (the layer's sort) at that point.
 class RawFileWriter {
   public:
     explicit RawFileWriter(int dev) : dev_(dev) {
@@ -1139,10 +1143,17 @@ class RawFileWriter {
         } catch (...) {
         }
     }
-    void write(const std::string &path, const uint64_t *keys, uint64_t n) {
+    void write(const std::string &path, const uint64_t *keys, uint64_t n, hipStream_t producer) {
+        hipEvent_t ready;
+        HIPCHK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        const hipError_t e = hipEventRecord(ready, producer);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(ready);
+            HIPCHK(e);
+        }
         {
             std::lock_guard<std::mutex> lk(mu_);
-            q_.push_back({path, keys, n});
+            q_.push_back({path, keys, n, ready});
         }
         cv_.notify_one();
     }
@@ -1170,8 +1181,10 @@ class RawFileWriter {
         std::string path;
         const uint64_t *keys;
         uint64_t n;
+        hipEvent_t ready;
     };
     void write_one(const Job &j) {
+        HIPCHK(hipStreamWaitEvent(cs_, j.ready, 0));
         const std::string tmp = j.path + ".tmp";
         FILE *f = std::fopen(tmp.c_str(), "wb");
         if (!f) throw Error(EGTB_ERR_IO, "cannot open " + tmp + " for writing");
@@ -1218,12 +1231,15 @@ class RawFileWriter {
                 j = std::move(q_.front());
                 q_.pop_front();
             }
-            if (!err_.empty()) continue;  // after a failure the remaining files are dropped
-            try {
-                write_one(j);
-            } catch (const std::exception &e) {
-                err_ = e.what();
+            if (err_.empty()) {  // after a failure the remaining files are dropped
+                try {
+                    write_one(j);
+                } catch (const std::exception &e) {
+                    err_ = e.what();
+                }
             }
+            (void)hipEventSynchronize(j.ready);
+            (void)hipEventDestroy(j.ready);
         }
     }
     int dev_;
@@ -1403,7 +1419,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         if (o.write_raw_files) raw_writer = std::make_unique<RawFileWriter>(o.device);
     }
     auto queue_raw = [&](int L, uint64_t n) {
-        if (raw_writer && !have_book[L]) raw_writer->write(prefix + std::to_string(L), res.raw[L], n);
+        if (raw_writer && !have_book[L]) raw_writer->write(prefix + std::to_string(L), res.raw[L], n, s);
     };
     LayerDev *M = res.meta_dev();
 
@@ -1519,11 +1535,225 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         HIPCHK(hipEventRecord(meta_ev, s));
     };
     uint64_t c4_prev = 0;
-    if (steps >= 3) {
+
+    // ---- hash-set forward: step i inserts its spawn-2 children into layer
+    // i's key set and its spawn-4 children into layer i+1's; once step i has
+    // run, layer i's set is extracted (unsorted, unique) and feeds step i+1
+    // with its size on the device, and is sorted into the raw arena after
+    // step i+1 is queued, so the host's wait for a layer's size overlaps the
+    // next step on the device. Sets are sized ahead from the growth of the
+    // known layers; a set that overflows is regrown and the steps feeding it
+    // run again from the sorted layers. When a set would not fit free HBM
+    // (or forward_hash_steps says so) the sort loop below takes over.
+    struct KeySet {
+        DBuf buf;
+        uint64_t cap = 0;  // slots in use (a power of two); the rest of buf is zero
+    };
+    KeySet sets[3];
+    DBuf ubuf[2];  // unsorted unique layers, layer L in ubuf[L & 1]
+    hipEvent_t hev[2];
+    HIPCHK(hipEventCreateWithFlags(&hev[0], hipEventDisableTiming));
+    EventGuard hev0_guard{hev[0]};
+    HIPCHK(hipEventCreateWithFlags(&hev[1], hipEventDisableTiming));
+    EventGuard hev1_guard{hev[1]};
+    static_assert(offsetof(LayerDev, pad) - offsetof(LayerDev, or_all) == 4 * sizeof(uint64_t),
+                  "k_expand_hash raises the overflow flag at or_all + 4 words");
+    auto set_of = [&](int L) -> KeySet & { return sets[L % 3]; };
+    auto set_prepare = [&](KeySet &k, uint64_t cap) {
+        if (k.buf.bytes < cap * 8) {
+            k.buf.alloc(cap * 8, s);
+            HIPCHK(hipMemsetAsync(k.buf.p, 0, k.buf.bytes, s));
+        }
+        k.cap = cap;
+    };
+    // slots for layer L from the sizes of layers up to `known`: the larger
+    // of the last two growth ratios, compounded, times the slots per key
+    const double slots_per_key = g_set_slots_per_key.load();
+    auto cap_for = [&](int L, int known) -> uint64_t {
+        const double n_k = (double)res.info[known].n_raw;
+        double g = 2.0;
+        if (known >= 1 && res.info[known - 1].n_raw) g = n_k / (double)res.info[known - 1].n_raw;
+        if (known >= 2 && res.info[known - 2].n_raw)
+            g = std::max(g, (double)res.info[known - 1].n_raw / (double)res.info[known - 2].n_raw);
+        g = std::min(std::max(g, 1.0), 8.0);
+        const double pred = n_k * std::pow(g, (double)(L - known)) * slots_per_key + 1024.0;
+        uint64_t c = (uint64_t)1 << 15;
+        while ((double)c < pred) c <<= 1;
+        return c;
+    };
+    auto hash_step = [&](int step, const uint64_t *parents, uint64_t bound, KeySet *s2, KeySet *s4, uint64_t *tot) {
+        Timer t(s, "expand", 0.0);  // its bytes are counted once per step, from its totals
+        launch_expand_hash(s, C.shape, parents, bound, &M[step - 1].n_raw, C.lut, p, step > o.docheck_step ? 1 : 0,
+                           tot, s2 ? (uint64_t *)s2->buf.p : nullptr, s2 ? s2->cap : 0,
+                           s4 ? (uint64_t *)s4->buf.p : nullptr, s4 ? s4->cap : 0, &M[step].or_all,
+                           &M[step + 1].or_all);
+    };
+    auto extract = [&](int L) {
+        KeySet &k = set_of(L);
+        ubuf[L & 1].ensure(k.cap * 8, s);
+        {
+            Timer t(s, "extract", 16.0 * (double)k.cap);
+            launch_set_extract(s, (uint64_t *)k.buf.p, k.cap, (uint64_t *)ubuf[L & 1].p, &M[L].n_raw);
+        }
+        HIPCHK(hipMemcpyAsync(H + L, M + L, sizeof(LayerDev), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(hev[L & 1], s));
+    };
+    auto wait_meta = [&](int L) {
+        for (hipError_t q; (q = hipEventQuery(hev[L & 1])) != hipSuccess;)
+            if (q != hipErrorNotReady) HIPCHK(q);
+    };
+    auto check_layer = [&](int L, uint64_t n) {
+        if (n * 64ull >= (1ull << 32))
+            throw Error(EGTB_ERR_ARG, "layer " + std::to_string(L) + " has too many states for one step");
+    };
+    // layer L (size known, sets drained) sorted into the raw arena
+    auto sort_layer = [&](int L, uint64_t n) {
+        res.raw[L] = (uint64_t *)res.raw_arena.alloc(n * 8);
+        uint64_t *u = (uint64_t *)ubuf[L & 1].p;
+        if (o.sort_schedule == 0) {
+            Timer t(s, "sort", 16.0 * (double)n);
+            uint64_t *out = radix_sort(s, u, res.raw[L], n, H[L].or_all, H[L].and_all, o.radix_bits, w, 0);
+            if (out != res.raw[L] && n)
+                HIPCHK(hipMemcpyAsync(res.raw[L], out, n * 8, hipMemcpyDeviceToDevice, s));
+        } else {  // an explicit schedule sorts (and uniques) as it would the candidates
+            sort_tmp.ensure(n * 8, s);
+            dedup_into(s, u, (uint64_t *)sort_tmp.p, n, H[L].or_all, H[L].and_all, res.raw[L], &M[L].n_raw,
+                       o.radix_bits, w, o.sort_schedule);
+        }
+    };
+    // layer L's set dropped keys: regrow it, empty every set, run the steps
+    // that feed it again from the sorted layers below, extract it again
+    // (synchronous; rare). Leaves the state of a step whose set L+1 holds
+    // exactly the spawn-4 children of step L.
+    auto recover = [&](int L) {
+        uint64_t cap = set_of(L).cap;
+        while (H[L].pad[0]) {
+            HIPCHK(hipStreamSynchronize(s));
+            cap *= 4;
+            res.hash_regrows++;
+            for (KeySet &k : sets)
+                if (k.buf.p) HIPCHK(hipMemsetAsync(k.buf.p, 0, k.buf.bytes, s));
+            set_prepare(set_of(L), cap);
+            if (L + 1 <= steps - 1 && !set_of(L + 1).buf.p) set_prepare(set_of(L + 1), cap);
+            for (int q = L; q <= std::min(L + 2, steps); ++q) {
+                HIPCHK(hipMemsetAsync(&M[q].pad[0], 0, 8, s));
+                if (q > L) HIPCHK(hipMemsetAsync(&M[q].tot, 0, 8, s));
+            }
+            HIPCHK(hipMemsetAsync(&M[L].n_raw, 0, 8, s));
+            if (L + 1 <= steps) HIPCHK(hipMemsetAsync(&M[L + 1].n_raw, 0, 8, s));
+            uint64_t *sink = (uint64_t *)cnt_dev.p;  // the totals of these steps are already known
+            if (L >= 2) hash_step(L - 1, res.raw[L - 2], res.info[L - 2].n_raw, nullptr, &set_of(L), sink);
+            if (L <= steps - 2)
+                hash_step(L, res.raw[L - 1], res.info[L - 1].n_raw, &set_of(L), &set_of(L + 1), sink);
+            extract(L);
+            HIPCHK(hipEventSynchronize(hev[L & 1]));
+        }
+    };
+
+    int first_sort_step = 1;
+    bool hash_done = false;
+    const bool hash_fwd = steps >= 3 && o.chunk_mkeys == 0 && o.forward_hash_steps >= 0 &&
+                          o.forward_hash_steps != 1;
+    if (hash_fwd) {
+        res.info[0].n_raw = n_seeds;
+        check_layer(0, n_seeds);
+        const uint64_t c1 = cap_for(1, 0), c2 = cap_for(2, 0);
+        if (c1 + c2 <= budget_keys(c1 + c2)) {
+            queue_raw(0, n_seeds);
+            if (streamer) streamer->reserve(n_seeds);
+            set_prepare(set_of(1), c1);
+            set_prepare(set_of(2), c2);
+            hash_step(1, res.raw[0], n_seeds, &set_of(1), &set_of(2), &M[1].tot);
+            extract(1);
+            first_sort_step = 0;
+            for (int i = 1; i <= steps - 2; ++i) {
+                progress_set((uint32_t)i, prog_total);
+                bool handover = false;
+                bool queued = false;  // step i+1 (or the last layer's extraction) is queued
+                if (i + 1 <= steps - 2) {
+                    const uint64_t c = cap_for(i + 2, i - 1);
+                    if ((o.forward_hash_steps > 0 && i + 1 >= o.forward_hash_steps) || 5 * c > budget_keys(5 * c)) {
+                        handover = true;
+                    } else {
+                        set_prepare(set_of(i + 2), c);
+                        hash_step(i + 1, (const uint64_t *)ubuf[i & 1].p, ubuf[i & 1].bytes / 8, &set_of(i + 1),
+                                  &set_of(i + 2), &M[i + 1].tot);
+                        extract(i + 1);
+                        queued = true;
+                    }
+                } else {
+                    extract(steps - 1);  // the last layer: spawn-4 children of step steps-2 only
+                    queued = true;
+                }
+                wait_meta(i);
+                if (H[i].pad[0]) {
+                    recover(i);
+                    if (queued) {  // queue again what the recovery undid
+                        if (i + 1 <= steps - 2) {
+                            hash_step(i + 1, (const uint64_t *)ubuf[i & 1].p, ubuf[i & 1].bytes / 8, &set_of(i + 1),
+                                      &set_of(i + 2), &M[i + 1].tot);
+                            extract(i + 1);
+                        } else {
+                            extract(steps - 1);
+                        }
+                    }
+                }
+                const uint64_t n = H[i].n_raw;
+                res.hash_steps++;
+                res.info[i].n_raw = n;
+                res.info[i].c2 = H[i].tot & 0xFFFFFFFFull;
+                res.info[i].c4 = H[i].tot >> 32;
+                stats_add_bytes("expand", 8.0 * (double)res.info[i - 1].n_raw +
+                                              8.0 * (double)(res.info[i].c2 + res.info[i].c4));
+                check_layer(i, n);
+                sort_layer(i, n);
+                if (handover) {
+                    // the sort loop continues at step i+1: set i+1 holds the
+                    // spawn-4 children of step i, which become the prefix of
+                    // its candidate buffer
+                    KeySet &k = set_of(i + 1);
+                    DBuf &cur = cand[(i + 1) & 1], &nxt = cand[i & 1];
+                    const uint64_t room = std::max<uint64_t>(k.cap + k.cap / 2, kEnsureMin / 8);
+                    cur.alloc(room * 8 + 256, s);
+                    nxt.alloc(room * 8 + 256, s);
+                    HIPCHK(hipMemsetAsync(cnt_dev.p, 0, 8, s));
+                    launch_set_extract(s, (uint64_t *)k.buf.p, k.cap, (uint64_t *)cur.p, (uint64_t *)cnt_dev.p);
+                    uint64_t c4x = 0;
+                    HIPCHK(hipMemcpyAsync(&c4x, cnt_dev.p, 8, hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                    for (KeySet &q : sets) q.buf.release();
+                    ubuf[0].release();
+                    ubuf[1].release();
+                    c4_prev = c4x;
+                    const uint64_t cap2 = cur.bytes / 8 - c4x, cap4 = nxt.bytes / 8;
+                    launch_alloc(i + 1, n, (uint64_t *)cur.p + c4x, (uint64_t *)nxt.p, cap2, cap4, false);
+                    queue_meta(i, i + 2);
+                    first_sort_step = i + 1;
+                    break;
+                }
+                queue_raw(i, n);
+                if (streamer) streamer->reserve(n);
+                HIPCHK(hipGetLastError());
+            }
+            if (first_sort_step == 0) {  // the last layer
+                const int L = steps - 1;
+                wait_meta(L);
+                if (H[L].pad[0]) recover(L);
+                res.info[L].n_raw = H[L].n_raw;
+                sort_layer(L, H[L].n_raw);
+                if (streamer) streamer->reserve(H[L].n_raw);
+                for (KeySet &q : sets) q.buf.release();
+                ubuf[0].release();
+                ubuf[1].release();
+                hash_done = true;
+            }
+        }
+    }
+    if (first_sort_step == 1 && steps >= 3) {
         launch_alloc(1, n_seeds, nullptr, nullptr, 0, 0, false);  // no buffers yet: totals and spans only
         queue_meta(0, 2);
     }
-    for (int i = 1; i <= steps - 2; ++i) {
+    for (int i = std::max(first_sort_step, 1); !hash_done && i <= steps - 2; ++i) {
         progress_set((uint32_t)i, prog_total);
         // poll: the host must wake as soon as the sizes land, while the
         // device is still busy with the speculative write
@@ -1657,7 +1887,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         c4_prev = c4;
     }
     // layer steps-1: only the spawn-4 children of layer steps-3
-    {
+    if (!hash_done) {
         const int L = steps - 1;
         read_meta(s, res, H, L - 1, L);
         res.raw_arena.trim_last(res.raw[L - 1], H[L - 1].n_raw * 8);
@@ -1972,6 +2202,13 @@ int egtb_result_memory(const egtb_result *r, uint64_t *peak_device_bytes, uint64
     return EGTB_OK;
 }
 
+int egtb_result_forward(const egtb_result *r, uint64_t *hash_steps, uint64_t *set_regrows) {
+    if (!r) return EGTB_ERR_ARG;
+    if (hash_steps) *hash_steps = r->hash_steps;
+    if (set_regrows) *set_regrows = r->hash_regrows;
+    return EGTB_OK;
+}
+
 int egtb_result_timing(const egtb_result *r, double *fwd_ms, double *bwd_ms, double *total_ms) {
     if (!r) return EGTB_ERR_ARG;
     if (fwd_ms) *fwd_ms = r->fwd_ms;
@@ -2127,6 +2364,15 @@ int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune = std::move(t);
+    return EGTB_OK;
+}
+
+int egtb_set_key_set_sizing(double slots_per_key) {
+    if (!(slots_per_key >= 0.0 && slots_per_key <= 64.0)) {
+        g_last_error = "slots per key must lie in (0, 64], or 0 for the default";
+        return EGTB_ERR_ARG;
+    }
+    g_set_slots_per_key.store(slots_per_key > 0.0 ? slots_per_key : kSetSlotsPerKey);
     return EGTB_OK;
 }
 

@@ -57,7 +57,7 @@ class Options(ctypes.Structure):
         ("write_raw_files", ctypes.c_int32),
         ("deletion_threshold", ctypes.c_double),
         ("solve_hash_mb", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("forward_hash_steps", ctypes.c_int32),
     ]
 
 
@@ -129,6 +129,7 @@ SIGNATURES = {
     "egtb_result_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "egtb_result_memory": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, ctypes.POINTER(ctypes.c_double), u64p, u64p]),
+    "egtb_result_forward": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p]),
     "egtb_result_free": (None, [ctypes.c_void_p]),
     "egtb_batch_build": (ctypes.c_int, [ctypes.POINTER(BatchJob), ctypes.c_int, ctypes.POINTER(BatchOptions),
                                         ctypes.POINTER(ctypes.c_void_p)]),
@@ -166,6 +167,7 @@ SIGNATURES = {
                                            ctypes.POINTER(LoadgenReport)]),
     "sort_uint64": (None, [u64p, ctypes.c_size_t, ctypes.c_bool]),
     "unique_sorted_u64_inplace": (ctypes.c_size_t, [u64p, ctypes.c_size_t]),
+    "egtb_set_key_set_sizing": (ctypes.c_int, [ctypes.c_double]),
     "egtb_set_sort_tuning": (ctypes.c_int, [u64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                             ctypes.c_int]),
     "egtb_bench_sort": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -71,7 +71,9 @@ typedef struct egtb_options {
     int32_t solve_hash_mb;           /* successor lookups of the solve: hash tables up to this many MiB
                                         per layer, bucket-indexed binary search above; 0 = default,
                                         -1 = always search */
-    int32_t reserved[1];
+    int32_t forward_hash_steps;      /* forward dedup: 0 = key sets (hashing) while they fit free HBM, then
+                                        sort + unique; -1 = sort + unique at every step; k > 0 = key sets
+                                        for steps below k only */
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file
@@ -125,6 +127,10 @@ EGTB_EXPORT int egtb_result_timing(const egtb_result *r, double *fwd_ms, double 
  * steps that ran chunked and their chunk count. */
 EGTB_EXPORT int egtb_result_memory(const egtb_result *r, uint64_t *peak_device_bytes, uint64_t *streamed_bytes,
                                    double *stream_ms, uint64_t *chunked_steps, uint64_t *chunks);
+/* Forward dedup of the last build: steps whose candidates went into key
+ * sets (the rest were sorted and uniqued) and how often a set was regrown
+ * and its steps run again. */
+EGTB_EXPORT int egtb_result_forward(const egtb_result *r, uint64_t *hash_steps, uint64_t *set_regrows);
 EGTB_EXPORT void egtb_result_free(egtb_result *r);
 
 /* ---- batched builds: many (pattern, target) jobs through one pipeline ------ */
@@ -302,6 +308,13 @@ EGTB_EXPORT int egtb_sort_unique_u64(const uint64_t *keys, size_t n, int schedul
  * (its sort has no tunables); written by the package's autotuner. */
 EGTB_EXPORT int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32_t *schedule,
                                      int rows);
+
+/* Slots per predicted key of the forward pass's key sets (default 3: a set
+ * is at most about a third full when the growth estimate holds; valid
+ * 0 < v <= 64; 0 restores the default).  A set that fills up is regrown and
+ * its steps run again, so small values only cost time.  No counterpart in
+ * the reference (its forward pass sorts every step). */
+EGTB_EXPORT int egtb_set_key_set_sizing(double slots_per_key);
 
 /* Time sort + unique of n host keys on device 0 with a given digit width
  * (4, 6, 8, 11) and schedule (1 per-digit kernels, 2 look-back); average
