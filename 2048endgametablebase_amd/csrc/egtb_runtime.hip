@@ -131,6 +131,13 @@ static std::atomic<int> g_stats_on{0};
 // slots per predicted key of the forward pass's key sets
 constexpr double kSetSlotsPerKey = 3.0;
 static std::atomic<double> g_set_slots_per_key{kSetSlotsPerKey};
+// largest forward key set (128 MiB); steps that need more sort their
+// candidates: LL@64 (sets up to 2^24 slots) builds in 120 ms with sets
+// against 161 ms sorting, while 444@2048's 2^26-slot sets (inserts going
+// to HBM rather than the last-level cache) made its forward pass 5.1 s
+// against 3.0 s
+constexpr uint64_t kSetMaxSlots = (uint64_t)1 << 24;
+static std::atomic<uint64_t> g_set_max_slots{kSetMaxSlots};
 static std::mutex g_stats_mu;
 struct StatAcc {
     uint64_t launches = 0;
@@ -1569,6 +1576,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // slots for layer L from the sizes of layers up to `known`: the larger
     // of the last two growth ratios, compounded, times the slots per key
     const double slots_per_key = g_set_slots_per_key.load();
+    const uint64_t max_slots = g_set_max_slots.load();
     auto cap_for = [&](int L, int known) -> uint64_t {
         const double n_k = (double)res.info[known].n_raw;
         double g = 2.0;
@@ -1658,7 +1666,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         res.info[0].n_raw = n_seeds;
         check_layer(0, n_seeds);
         const uint64_t c1 = cap_for(1, 0), c2 = cap_for(2, 0);
-        if (c1 + c2 <= budget_keys(c1 + c2)) {
+        if (c1 + c2 <= budget_keys(c1 + c2) && std::max(c1, c2) <= max_slots) {
             queue_raw(0, n_seeds);
             if (streamer) streamer->reserve(n_seeds);
             set_prepare(set_of(1), c1);
@@ -1672,7 +1680,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 bool queued = false;  // step i+1 (or the last layer's extraction) is queued
                 if (i + 1 <= steps - 2) {
                     const uint64_t c = cap_for(i + 2, i - 1);
-                    if ((o.forward_hash_steps > 0 && i + 1 >= o.forward_hash_steps) || 5 * c > budget_keys(5 * c)) {
+                    if ((o.forward_hash_steps > 0 && i + 1 >= o.forward_hash_steps) || c > max_slots ||
+                        5 * c > budget_keys(5 * c)) {
                         handover = true;
                     } else {
                         set_prepare(set_of(i + 2), c);
@@ -1708,25 +1717,29 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 check_layer(i, n);
                 sort_layer(i, n);
                 if (handover) {
-                    // the sort loop continues at step i+1: set i+1 holds the
-                    // spawn-4 children of step i, which become the prefix of
-                    // its candidate buffer
-                    KeySet &k = set_of(i + 1);
+                    // the sort loop continues at step i+1. Its candidate
+                    // buffer starts with the spawn-4 children of step i:
+                    // written here again by the one-pass expand from the
+                    // sorted layer i-1 (set i+1 may have dropped some; its
+                    // overflow is only looked at when that layer is due)
+                    const uint64_t c4 = H[i].tot >> 32, c2 = H[i].tot & 0xFFFFFFFFull;
                     DBuf &cur = cand[(i + 1) & 1], &nxt = cand[i & 1];
-                    const uint64_t room = std::max<uint64_t>(k.cap + k.cap / 2, kEnsureMin / 8);
+                    const uint64_t room = std::max<uint64_t>(c4 + 2 * (c2 + c4) + 64, kEnsureMin / 8);
                     cur.alloc(room * 8 + 256, s);
                     nxt.alloc(room * 8 + 256, s);
-                    HIPCHK(hipMemsetAsync(cnt_dev.p, 0, 8, s));
-                    launch_set_extract(s, (uint64_t *)k.buf.p, k.cap, (uint64_t *)cur.p, (uint64_t *)cnt_dev.p);
-                    uint64_t c4x = 0;
-                    HIPCHK(hipMemcpyAsync(&c4x, cnt_dev.p, 8, hipMemcpyDeviceToHost, s));
-                    HIPCHK(hipStreamSynchronize(s));
                     for (KeySet &q : sets) q.buf.release();
                     ubuf[0].release();
                     ubuf[1].release();
-                    c4_prev = c4x;
-                    const uint64_t cap2 = cur.bytes / 8 - c4x, cap4 = nxt.bytes / 8;
-                    launch_alloc(i + 1, n, (uint64_t *)cur.p + c4x, (uint64_t *)nxt.p, cap2, cap4, false);
+                    HIPCHK(hipMemsetAsync(cnt_dev.p, 0, 8, s));  // its write positions come from this total
+                    {
+                        Timer t(s, "expand", 0.0);
+                        launch_expand_alloc(s, C.shape, res.raw[i - 1], res.info[i - 1].n_raw, &M[i - 1].n_raw,
+                                            C.lut, p, i > o.docheck_step ? 1 : 0, (uint64_t *)cnt_dev.p, nullptr,
+                                            (uint64_t *)cur.p, &M[i].or_all, &M[i + 1].or_all, 0, c4);
+                    }
+                    c4_prev = c4;
+                    const uint64_t cap2 = cur.bytes / 8 - c4, cap4 = nxt.bytes / 8;
+                    launch_alloc(i + 1, n, (uint64_t *)cur.p + c4, (uint64_t *)nxt.p, cap2, cap4, false);
                     queue_meta(i, i + 2);
                     first_sort_step = i + 1;
                     break;
@@ -2367,12 +2380,17 @@ int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32
     return EGTB_OK;
 }
 
-int egtb_set_key_set_sizing(double slots_per_key) {
+int egtb_set_key_set_sizing(double slots_per_key, uint64_t max_slots) {
     if (!(slots_per_key >= 0.0 && slots_per_key <= 64.0)) {
         g_last_error = "slots per key must lie in (0, 64], or 0 for the default";
         return EGTB_ERR_ARG;
     }
+    if (max_slots > ((uint64_t)1 << 34)) {
+        g_last_error = "max_slots must be at most 2^34, or 0 for the default";
+        return EGTB_ERR_ARG;
+    }
     g_set_slots_per_key.store(slots_per_key > 0.0 ? slots_per_key : kSetSlotsPerKey);
+    g_set_max_slots.store(max_slots ? max_slots : kSetMaxSlots);
     return EGTB_OK;
 }
 

@@ -167,7 +167,7 @@ SIGNATURES = {
                                            ctypes.POINTER(LoadgenReport)]),
     "sort_uint64": (None, [u64p, ctypes.c_size_t, ctypes.c_bool]),
     "unique_sorted_u64_inplace": (ctypes.c_size_t, [u64p, ctypes.c_size_t]),
-    "egtb_set_key_set_sizing": (ctypes.c_int, [ctypes.c_double]),
+    "egtb_set_key_set_sizing": (ctypes.c_int, [ctypes.c_double, ctypes.c_uint64]),
     "egtb_set_sort_tuning": (ctypes.c_int, [u64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                             ctypes.c_int]),
     "egtb_bench_sort": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -27,10 +27,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "solved states/s for 442@512 build (batched vs plain); query QPS at p99 latency"
 # single-kernel stats families and the kernel each times on this workload
-# (the radix families also hold the 4-key-per-thread tiles of the few small
-# layers; 8 keys per thread is what layers of 0.5-4 Mi candidates use)
-KERNEL_OF = {"radix_scatter": "egtb::k_radix_scatter<8, 8>", "radix_hist": "egtb::k_radix_hist<8, 8>",
-             "solve": "egtb::k_solve<float>", "expand": "egtb::k_expand_alloc"}
+# (the forward pass sorts unique layers: 4 keys per thread up to 512 Ki keys,
+# 8 above, so each radix family holds both tile instances)
+KERNEL_OF = {"radix_scatter": "egtb::k_radix_scatter<8, 4>", "radix_hist": "egtb::k_radix_hist<8, 4>",
+             "solve": "egtb::k_solve<float>", "expand": "egtb::k_expand_hash", "extract": "egtb::k_set_extract"}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_442_512_plain_build.md")
 
 
@@ -218,8 +218,8 @@ def main():
         r.close()
     torch.cuda.synchronize()
     L.egtb_stats_enable(0)
-    stats = kernel_stats(native, ["expand_count", "expand", "sort", "unique", "solve", "compact", "hash",
-                                  "radix_hist", "radix_scatter"])
+    stats = kernel_stats(native, ["expand_count", "expand", "extract", "sort", "unique", "solve", "compact",
+                                  "hash", "radix_hist", "radix_scatter"])
     single = {k: v for k, v in stats.items() if k in KERNEL_OF and v["launches"]}
     roofline = None
     if single:

@@ -309,12 +309,16 @@ EGTB_EXPORT int egtb_sort_unique_u64(const uint64_t *keys, size_t n, int schedul
 EGTB_EXPORT int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32_t *schedule,
                                      int rows);
 
-/* Slots per predicted key of the forward pass's key sets (default 3: a set
- * is at most about a third full when the growth estimate holds; valid
- * 0 < v <= 64; 0 restores the default).  A set that fills up is regrown and
- * its steps run again, so small values only cost time.  No counterpart in
+/* Sizing of the forward pass's key sets.  slots_per_key: slots per
+ * predicted key (default 3: a set is at most about a third full when the
+ * growth estimate holds; valid 0 < v <= 64).  max_slots: the largest set
+ * (default 2^24 slots, 128 MiB: random inserts pay while the two sets being
+ * filled stay about within the 256 MiB last-level cache); a step that would need a larger one
+ * and the rest of the forward pass sort and unique their candidates instead.
+ * 0 restores a default.  A set that fills up is regrown and its steps run
+ * again, so small slots_per_key values only cost time.  No counterpart in
  * the reference (its forward pass sorts every step). */
-EGTB_EXPORT int egtb_set_key_set_sizing(double slots_per_key);
+EGTB_EXPORT int egtb_set_key_set_sizing(double slots_per_key, uint64_t max_slots);
 
 /* Time sort + unique of n host keys on device 0 with a given digit width
  * (4, 6, 8, 11) and schedule (1 per-digit kernels, 2 look-back); average
