@@ -328,6 +328,9 @@ __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict
         const uint64_t b = par[i];
         if (do_check && is_success(b, p)) continue;
         const SpawnMoves smv(b, lut);
+        // the last child per (spawn value, move): spawns in neighbouring
+        // cells often slide to the same board, which needs no second insert
+        uint64_t last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t em = empty_cells(b);
         while (em) {
             const int k = __builtin_ctz(em);
@@ -341,16 +344,19 @@ __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict
                 for (int d = 0; d < 4; ++d) {
                     if (mv[d] != sp && is_pattern(mv[d], p)) {
                         const uint64_t c = canonical(mv[d], p.symm);
+                        const int j = (sv - 1) * 4 + d;
+                        const bool again = c == last[j];
+                        last[j] = c;
                         if (sv == 1) {
                             or2 |= c;
                             and2 &= c;
                             count += 1;
-                            if (set2 && !set_insert(set2, mask2, c)) fail |= 1u;
+                            if (set2 && !again && !set_insert(set2, mask2, c)) fail |= 1u;
                         } else {
                             or4 |= c;
                             and4 &= c;
                             count += 1ull << 32;
-                            if (set4 && !set_insert(set4, mask4, c)) fail |= 2u;
+                            if (set4 && !again && !set_insert(set4, mask4, c)) fail |= 2u;
                         }
                     }
                 }

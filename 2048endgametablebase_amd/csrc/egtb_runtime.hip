@@ -128,8 +128,9 @@ void DBuf::ensure(size_t nbytes, hipStream_t s) {
 // ---------------------------------------------------------------------------
 
 static std::atomic<int> g_stats_on{0};
-// slots per predicted key of the forward pass's key sets
-constexpr double kSetSlotsPerKey = 3.0;
+// slots per predicted key of the forward pass's key sets (2: 442@512's sets
+// of 2-4 MiB then stay about within an XCD's L2; 3 was 1-3% slower)
+constexpr double kSetSlotsPerKey = 2.0;
 static std::atomic<double> g_set_slots_per_key{kSetSlotsPerKey};
 // largest forward key set (128 MiB); steps that need more sort their
 // candidates: LL@64 (sets up to 2^24 slots) builds in 120 ms with sets

@@ -310,7 +310,7 @@ EGTB_EXPORT int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits,
                                      int rows);
 
 /* Sizing of the forward pass's key sets.  slots_per_key: slots per
- * predicted key (default 3: a set is at most about a third full when the
+ * predicted key (default 2: a set is at most about half full when the
  * growth estimate holds; valid 0 < v <= 64).  max_slots: the largest set
  * (default 2^24 slots, 128 MiB: random inserts pay while the two sets being
  * filled stay about within the 256 MiB last-level cache); a step that would need a larger one
