@@ -176,9 +176,12 @@ void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, 
                          uint32_t bits, uint32_t *idx);
 
 template <typename T>
+// clear / clear_n: a hash table the same launch zeroes (the next layer's, whose
+// slots no probe of this launch reads), so its fill needs no launch of its own.
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
-                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates);
+                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
+                  ulonglong2 *clear = nullptr, uint64_t clear_n = 0);
 
 template <typename T>
 void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,
@@ -192,7 +195,8 @@ uint64_t compact_tiles(uint64_t n);
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
                     T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw,
-                    T threshold = T(0), ulonglong2 *hslots = nullptr, uint64_t hcap = 0);
+                    T threshold = T(0), ulonglong2 *hslots = nullptr, uint64_t hcap = 0,
+                    bool hslots_zeroed = false);
 
 template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);

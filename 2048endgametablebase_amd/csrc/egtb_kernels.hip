@@ -1018,8 +1018,13 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                                                 const uint16_t *__restrict__ glut, PatternDev p,
                                                 int do_check, HashDev h1, HashDev h2, TableDev<T> t1,
                                                 TableDev<T> t2, double w2, double w4, T max_scale,
-                                                T *__restrict__ rates) {
+                                                T *__restrict__ rates, ulonglong2 *__restrict__ clear,
+                                                uint64_t clear_n) {
     extern __shared__ uint16_t lut[];
+    // the table filled after this launch: disjoint from h1 / h2
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < clear_n;
+         j += (uint64_t)gridDim.x * blockDim.x)
+        clear[j] = make_ulonglong2(0ull, 0ull);
     stage_lut(lut, glut);
     // hash probes when both successor layers have a hash, else binary
     // searches narrowed by the bucket index (uniform over the launch)
@@ -1481,12 +1486,16 @@ void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, 
 template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
-                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates) {
-    if (n == 0) return;
-    const uint32_t want = blocks_for(n, ls.threads);
+                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
+                  ulonglong2 *clear, uint64_t clear_n) {
+    if (n == 0) {
+        if (clear_n) (void)hipMemsetAsync(clear, 0, clear_n * sizeof(ulonglong2), s);
+        return;
+    }
+    const uint32_t want = blocks_for(n > clear_n ? n : clear_n, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     hipLaunchKernelGGL((k_solve<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n,
-                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates);
+                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates, clear, clear_n);
 }
 
 // Slots for a layer of at most `bound` records: a power of two, at most a
@@ -1523,8 +1532,8 @@ uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
                     T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw, T threshold,
-                    ulonglong2 *hslots, uint64_t hcap) {
-    if (hslots) (void)hipMemsetAsync(hslots, 0, hcap * sizeof(ulonglong2), s);
+                    ulonglong2 *hslots, uint64_t hcap, bool hslots_zeroed) {
+    if (hslots && !hslots_zeroed) (void)hipMemsetAsync(hslots, 0, hcap * sizeof(ulonglong2), s);
     if (n == 0) {
         if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
         return;
@@ -1562,13 +1571,13 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
     template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
                                   const uint16_t *, const PatternDev &, int, const HashDev &,                 \
                                   const HashDev &, const TableDev<T> &, const TableDev<T> &, double, double,  \
-                                  T, T *);                                                                    \
+                                  T, T *, ulonglong2 *, uint64_t);                                            \
     template void launch_hash_build<T>(hipStream_t, const uint64_t *, const T *, uint64_t, const uint64_t *,  \
                                        ulonglong2 *, uint64_t);                                               \
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *, T *,      \
                                     unsigned long long *, uint64_t *, LookbackWork &, T, ulonglong2 *,        \
-                                    uint64_t);                                                                \
+                                    uint64_t, bool);                                                                \
     template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
 
 EGTB_INSTANTIATE(float)

@@ -679,7 +679,7 @@ static void drop_hash(Result &res, int layer) {
 // device; nothing is synchronised.
 template <typename T>
 static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *keys, const T *rates, uint64_t bound,
-                      SortWork &w, bool index) {
+                      SortWork &w, bool index, bool hash_zeroed = false) {
     Book &bk = res.books[layer];
     LayerDev *md = res.meta_dev() + layer;
     bk.n_dev = &md->n_live;
@@ -693,12 +693,15 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     if (bound) w.ensure_lookback(s, 0, 0, compact_tiles(bound));
     // the compaction also fills the layer's solve hash (read by the solves
     // of layers i-1 and i-2; layer 0 is read by none)
-    uint64_t hcap = 0;
-    ulonglong2 *hslots = index ? hash_table_for(s, res, layer, bound, hcap) : nullptr;
+    // (with hash_zeroed the table is in place already, cleared by the solve)
+    uint64_t hcap = bk.hmask ? bk.hmask + 1 : 0;
+    ulonglong2 *hslots = !index             ? nullptr
+                         : hash_zeroed      ? (ulonglong2 *)bk.hslots
+                                            : hash_table_for(s, res, layer, bound, hcap);
     {
         Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2 + (double)hcap * sizeof(ulonglong2));
         launch_compact<T>(s, keys, rates, bound, bk.keys, (T *)bk.rates, (unsigned long long *)&md->maxbits,
-                          &md->n_live, w.lw, T(0), hslots, hcap);
+                          &md->n_live, w.lw, T(0), hslots, hslots ? hcap : 0, hash_zeroed);
     }
     if (index && res.book_index) {
         bk.bits = index_bits_for(bound);
@@ -1985,12 +1988,18 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         const auto l0 = std::chrono::steady_clock::now();
         const uint64_t n = res.info[i].n_raw;
         const Book &b1 = res.books[i + 1], &b2 = res.books[i + 2];
+        // layer i's solve hash (ring slot i % 3; its previous table, layer
+        // i+3's, was last read by the solve of layer i+1) is cleared by this
+        // solve and filled by the compaction that makes book i
+        uint64_t hcap = 0;
+        ulonglong2 *hs = i > 0 ? hash_table_for(s, res, i, n, hcap) : nullptr;
         {
             Timer t(s, "solve", (double)n * (8 + sizeof(T)));
             launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
-                            hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p);
+                            hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p, hs,
+                            hs ? hcap : 0);
         }
-        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0);
+        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0, hs != nullptr);
         drop_hash(res, i + 2);  // its last reader was this layer's solve
         HIPCHK(hipGetLastError());
         if (streamer) {
