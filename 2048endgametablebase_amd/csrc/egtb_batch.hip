@@ -37,6 +37,11 @@ struct JobDev {
 
 constexpr int kMaxJobs = 256;
 
+// Largest key set of a batched forward step (128 MiB, as for plain builds).
+constexpr uint64_t kBatchSetMaxSlots = (uint64_t)1 << 24;
+// Largest solve hash of a batched book (as the plain builds' default).
+constexpr double kBatchHashMaxBytes = 1024.0 * 1024.0 * 1024.0;
+
 __device__ __forceinline__ uint64_t rot_key(uint64_t k, uint32_t job) { return (k >> 8) | ((uint64_t)job << 56); }
 __device__ __forceinline__ uint64_t unrot_key(uint64_t r) { return (r << 8) | 0xFFull; }
 
@@ -57,7 +62,7 @@ __global__ __launch_bounds__(1024) void k_bexpand(const uint64_t *__restrict__ p
     stage_lut(lut, glut);
     const uint64_t n = n_ptr ? (*n_ptr < n_bound ? *n_ptr : n_bound) : n_bound;
     uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull;
-    bool bad = false;
+    bool bad = false, zero = false;  // zero: a rotated key equal to the solve hashes' empty slot
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t r = par[i];
@@ -88,6 +93,7 @@ __global__ __launch_bounds__(1024) void k_bexpand(const uint64_t *__restrict__ p
                             const uint64_t c = canonical(mv[d], J.p.symm);
                             bad |= (c & 0xFF) != 0xFF;
                             const uint64_t rc = rot_key(c, job);
+                            zero |= rc == 0;
                             if (s == 1) {
                                 if (WRITE) out2[w2 + c2] = rc;
                                 else { or2 |= rc; and2 &= rc; }
@@ -105,6 +111,7 @@ __global__ __launch_bounds__(1024) void k_bexpand(const uint64_t *__restrict__ p
         if (!WRITE) counts[i] = (uint64_t)c2 | ((uint64_t)c4 << 32);
     }
     if (bad) atomicOr(err, 2u);
+    if (zero) atomicOr(err, 4u);
     if (!WRITE) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -141,10 +148,13 @@ __global__ __launch_bounds__(1024) void k_bexpand(const uint64_t *__restrict__ p
 // otherwise, looking up rotated keys of the same job in the global books.
 __global__ __launch_bounds__(1024) void k_bsolve(const uint64_t *__restrict__ keys, uint64_t n,
                                                  const uint16_t *__restrict__ glut, const JobDev *__restrict__ jobs,
-                                                 int layer, TableDev<float> t1, TableDev<float> t2,
-                                                 float *__restrict__ rates) {
+                                                 int layer, TableDev<float> t1, TableDev<float> t2, HashDev h1,
+                                                 HashDev h2, float *__restrict__ rates) {
     extern __shared__ uint16_t lut[];
     stage_lut(lut, glut);
+    // hash probes when both successor layers have a hash (rotated keys; the
+    // forward pass rejects a rotated key of 0), else bucket-index searches
+    const bool use_hash = h1.slots != nullptr && h2.slots != nullptr;
     const uint64_t n1 = t1.n_ptr ? *t1.n_ptr : t1.n, n2 = t2.n_ptr ? *t2.n_ptr : t2.n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -182,37 +192,72 @@ __global__ __launch_bounds__(1024) void k_bsolve(const uint64_t *__restrict__ ke
                     lo[j] = hi[j] = 0;
                     const uint64_t nn = s == 0 ? n1 : n2;
                     if (mv[d] != spb && is_pattern(mv[d], J.p) && nn) {
-                        const TableDev<float> &t = s == 0 ? t1 : t2;
-                        const uint64_t bk = key[j] >> (64 - t.bits);
-                        lo[j] = t.idx[bk];
-                        hi[j] = t.idx[bk + 1];
+                        if (!use_hash) {
+                            const TableDev<float> &t = s == 0 ? t1 : t2;
+                            const uint64_t bk = key[j] >> (64 - t.bits);
+                            lo[j] = t.idx[bk];
+                            hi[j] = t.idx[bk + 1];
+                        }
                         live |= 1u << j;
                     }
                 }
             }
-            uint32_t active = live;
-            while (active) {
+            float best[2] = {0.0f, 0.0f};
+            if (use_hash) {
+                uint64_t slot[8];
+                float r[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    if (lo[j] < hi[j]) {
-                        const uint32_t mid = (lo[j] + hi[j]) >> 1;
-                        const uint64_t km = (j < 4 ? t1.keys : t2.keys)[mid];
-                        if (km < key[j]) lo[j] = mid + 1;
-                        else hi[j] = mid;
+                    slot[j] = hash_mix(key[j]) & (j < 4 ? h1.mask : h2.mask);
+                    r[j] = 0.0f;
+                }
+                uint32_t active = live;
+                while (active) {
+                    ulonglong2 ent[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if ((active >> j) & 1u) ent[j] = (j < 4 ? h1.slots : h2.slots)[slot[j]];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if ((active >> j) & 1u) {
+                            if (ent[j].x == key[j]) {
+                                r[j] = rate_from_bits<float>(ent[j].y);
+                                active &= ~(1u << j);
+                            } else if (ent[j].x == 0) {
+                                active &= ~(1u << j);  // absent: rate 0
+                            } else {
+                                slot[j] = (slot[j] + 1) & (j < 4 ? h1.mask : h2.mask);
+                            }
+                        }
                     }
                 }
-                active = 0;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) active |= (lo[j] < hi[j] ? 1u : 0u) << j;
-            }
-            float best[2] = {0.0f, 0.0f};
+                for (int j = 0; j < 8; ++j)
+                    if ((live >> j) & 1u) best[j >> 2] = best[j >> 2] < r[j] ? r[j] : best[j >> 2];
+            } else {
+                uint32_t active = live;
+                while (active) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if ((live >> j) & 1u) {
-                    const TableDev<float> &t = j < 4 ? t1 : t2;
-                    const uint64_t nn = j < 4 ? n1 : n2;
-                    const float v = (lo[j] < nn && t.keys[lo[j]] == key[j]) ? t.rates[lo[j]] : 0.0f;
-                    best[j >> 2] = best[j >> 2] < v ? v : best[j >> 2];
+                    for (int j = 0; j < 8; ++j) {
+                        if (lo[j] < hi[j]) {
+                            const uint32_t mid = (lo[j] + hi[j]) >> 1;
+                            const uint64_t km = (j < 4 ? t1.keys : t2.keys)[mid];
+                            if (km < key[j]) lo[j] = mid + 1;
+                            else hi[j] = mid;
+                        }
+                    }
+                    active = 0;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) active |= (lo[j] < hi[j] ? 1u : 0u) << j;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if ((live >> j) & 1u) {
+                        const TableDev<float> &t = j < 4 ? t1 : t2;
+                        const uint64_t nn = j < 4 ? n1 : n2;
+                        const float v = (lo[j] < nn && t.keys[lo[j]] == key[j]) ? t.rates[lo[j]] : 0.0f;
+                        best[j >> 2] = best[j >> 2] < v ? v : best[j >> 2];
+                    }
                 }
             }
             sp = __fma_rn((double)best[0], J.w2, sp);
@@ -302,6 +347,7 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
         for (size_t q = 0; q < jb.n_seeds; ++q) {
             const uint64_t k = jb.seeds[q];
             if ((k & 0xFF) != 0xFF) throw Error(EGTB_ERR_ARG, "job " + std::to_string(j) + ": seed not in pattern");
+            if (j == 0 && k == 0xFF) throw Error(EGTB_ERR_ARG, "job 0: a seed rotates to key 0, the empty slot");
             br.seeds[j].push_back(k);
             layer0.push_back((k >> 8) | ((uint64_t)j << 56));
         }
@@ -359,6 +405,50 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
                         (uint64_t *)w.partial.p, &M[step].tot);
     };
 
+    // Layer L from its candidates keys[0..n_c): into a key set of at least
+    // 2 n_c slots (it cannot fill), extracted unsorted, then only the
+    // unique keys sorted into the raw arena; sets above kBatchSetMaxSlots
+    // sort and unique the candidates instead. The raw layer and M[L].n_raw
+    // are the same either way.
+    DBuf kset, uniq;
+    auto dedup_layer = [&](int L, uint64_t *keys, uint64_t n_c) {
+        uint64_t cap = 1024;
+        while (cap < 2 * n_c) cap <<= 1;
+        if (n_c >= 2 && cap <= kBatchSetMaxSlots && bo.sort_schedule == 0) {
+            if (kset.bytes < cap * 8) {
+                kset.alloc(cap * 8, s);
+                HIPCHK(hipMemsetAsync(kset.p, 0, kset.bytes, s));
+            }
+            uniq.ensure(n_c * 8, s);
+            {
+                Timer t(s, "batch_set", 8.0 * (double)n_c + 16.0 * (double)cap);
+                launch_set_insert(s, keys, n_c, (uint64_t *)kset.p, cap);
+                HIPCHK(hipMemsetAsync(&M[L].n_raw, 0, 8, s));
+                launch_set_extract(s, (uint64_t *)kset.p, cap, (uint64_t *)uniq.p, &M[L].n_raw);
+            }
+            HIPCHK(hipMemcpyAsync(&H[L].n_raw, &M[L].n_raw, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            const uint64_t n = H[L].n_raw;
+            res.raw[L] = (uint64_t *)res.raw_arena.alloc(n * 8);
+            Timer t(s, "batch_sort", 16.0 * (double)n);
+            uint64_t *out = radix_sort(s, (uint64_t *)uniq.p, res.raw[L], n, H[L].or_all, H[L].and_all,
+                                       bo.radix_bits, w, 0);
+            if (out != res.raw[L] && n) HIPCHK(hipMemcpyAsync(res.raw[L], out, n * 8, hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        uint64_t *sorted;
+        {
+            Timer t(s, "batch_sort", 16.0 * (double)n_c);
+            sorted = radix_sort(s, keys, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all, bo.radix_bits, w,
+                                bo.sort_schedule);
+        }
+        res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
+        {
+            Timer t(s, "batch_unique", 16.0 * (double)n_c);
+            unique_into(s, sorted, n_c, res.raw[L], &M[L].n_raw, w, bo.sort_schedule);
+        }
+    };
+
     uint64_t c4_prev = 0;
     launch_count(1, layer0.size());
     for (int i = 1; i <= max_steps - 2; ++i) {
@@ -386,17 +476,7 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
                                n_par, nullptr, C.lut, (const JobDev *)d_jobs.p, i, nullptr, (const uint64_t *)offs.p,
                                (uint64_t *)cur.p + c4_prev, (uint64_t *)nxt.p, nullptr, nullptr, (uint32_t *)errb.p);
         }
-        uint64_t *sorted;
-        {
-            Timer t(s, "batch_sort", 16.0 * (double)n_c);
-            sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[i].or_all, H[i].and_all,
-                                bo.radix_bits, w, bo.sort_schedule);
-        }
-        res.raw[i] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-        {
-            Timer t(s, "batch_unique", 16.0 * (double)n_c);
-            unique_into(s, sorted, n_c, res.raw[i], &M[i].n_raw, w, bo.sort_schedule);
-        }
+        dedup_layer(i, (uint64_t *)cur.p, n_c);
         if (i + 1 <= max_steps - 2) launch_count(i + 1, n_c);
         HIPCHK(hipGetLastError());
         c4_prev = c4;
@@ -410,10 +490,7 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
         const uint64_t n_c = c4_prev;
         DBuf &cur = cand[L & 1];
         sort_tmp.ensure(n_c * 8, s);
-        uint64_t *sorted = radix_sort(s, (uint64_t *)cur.p, (uint64_t *)sort_tmp.p, n_c, H[L].or_all, H[L].and_all,
-                                      bo.radix_bits, w, bo.sort_schedule);
-        res.raw[L] = (uint64_t *)res.raw_arena.alloc(n_c * 8);
-        unique_into(s, sorted, n_c, res.raw[L], &M[L].n_raw, w, bo.sort_schedule);
+        dedup_layer(L, (uint64_t *)cur.p, n_c);
         HIPCHK(hipMemcpyAsync(H + L, M + L, sizeof(LayerDev), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         res.raw_arena.trim_last(res.raw[L], H[L].n_raw * 8);
@@ -422,11 +499,14 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
     cand[0].release();
     cand[1].release();
     sort_tmp.release();
+    kset.release();
+    uniq.release();
     counts.release();
     offs.release();
     const auto t_fwd = std::chrono::steady_clock::now();
 
     // ---- backward, all jobs per layer; sizes stay on the device
+    DBuf hring[3];  // solve hashes of the books, layer i in slot i % 3
     uint64_t max_n = 0;
     for (int i = 0; i < max_steps; ++i) max_n = std::max(max_n, res.info[i].n_raw);
     rates.ensure(max_n * 4 + 8, s);
@@ -442,10 +522,15 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
             const Book &b2 = res.books[i + 2];
             t2 = TableDev<float>{b2.keys, (const float *)b2.rates, b2.idx, b2.n_dev, 0, b2.bits, 1};
         }
+        HashDev h1{nullptr, 0}, h2{nullptr, 0};
+        if (i + 2 < max_steps) {
+            h1 = HashDev{res.books[i + 1].hslots, res.books[i + 1].hmask};
+            h2 = HashDev{res.books[i + 2].hslots, res.books[i + 2].hmask};
+        }
         if (n) {
             Timer t(s, "batch_solve", (double)n * 12.0);
             hipLaunchKernelGGL(k_bsolve, dim3(grid_for(n)), dim3(C.shape.threads), lds, s, res.raw[i], n, C.lut,
-                               (const JobDev *)d_jobs.p, i, t1, t2, (float *)rates.p);
+                               (const JobDev *)d_jobs.p, i, t1, t2, h1, h2, (float *)rates.p);
         }
         // compact + index (the same kernels as the plain build)
         Book &bk = res.books[i];
@@ -453,8 +538,22 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
         bk.keys = (uint64_t *)res.book_arena.alloc(n * 8);
         bk.rates = res.book_arena.alloc(n * 4);
         if (n) w.ensure_lookback(s, 0, 0, compact_tiles(n));
+        // the compaction fills layer i's solve hash (read by the solves of
+        // layers i-1 and i-2): ring slot i % 3, whose previous table (layer
+        // i+3) was last read by the solve of layer i+1
+        ulonglong2 *hs = nullptr;
+        uint64_t hcap = 0;
+        if (i > 0) {
+            hcap = hash_capacity(n);
+            if ((double)hcap * sizeof(ulonglong2) <= kBatchHashMaxBytes) {
+                hring[i % 3].ensure(hcap * sizeof(ulonglong2), s);
+                hs = (ulonglong2 *)hring[i % 3].p;
+            }
+        }
+        bk.hslots = hs;
+        bk.hmask = hs ? hcap - 1 : 0;
         launch_compact<float>(s, res.raw[i], (const float *)rates.p, n, bk.keys, (float *)bk.rates,
-                              (unsigned long long *)&M[i].maxbits, &M[i].n_live, w.lw);
+                              (unsigned long long *)&M[i].maxbits, &M[i].n_live, w.lw, 0.0f, hs, hs ? hcap : 0);
         bk.bits = bindex_bits(n);
         bk.idx = (uint32_t *)res.book_arena.alloc(((1ull << bk.bits) + 1) * 4);
         launch_bucket_index(s, bk.keys, n, bk.n_dev, bk.bits, bk.idx);
@@ -463,6 +562,10 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
                                n_jobs, (uint64_t *)br.bounds.p + (size_t)i * (n_jobs + 1));
         HIPCHK(hipGetLastError());
     }
+    for (int i = 0; i < max_steps; ++i) {  // the ring is freed with this scope
+        res.books[i].hslots = nullptr;
+        res.books[i].hmask = 0;
+    }
     uint32_t err = 0;
     HIPCHK(hipMemcpyAsync(&err, errb.p, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(H, M, (size_t)max_steps * sizeof(LayerDev), hipMemcpyDeviceToHost, s));
@@ -470,6 +573,7 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
     HIPCHK(hipMemcpyAsync(hb.data(), br.bounds.p, hb.size() * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (err & 2u) throw Error(EGTB_ERR_INTERNAL, "a canonical child left the pinned low byte");
+    if (err & 4u) throw Error(EGTB_ERR_INTERNAL, "a rotated key is 0, which the solve hashes cannot hold");
     if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     const auto t_end = std::chrono::steady_clock::now();
     for (int i = 0; i < max_steps; ++i) {

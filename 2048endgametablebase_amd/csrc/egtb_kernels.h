@@ -122,6 +122,9 @@ void launch_expand_hash(hipStream_t s, const LaunchShape &ls, const uint64_t *pa
                         const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *tot,
                         uint64_t *set2, uint64_t cap2, uint64_t *set4, uint64_t cap4, uint64_t *meta2,
                         uint64_t *meta4);
+// keys[0..n) (nonzero) into a set of cap > n slots (a power of two); cannot
+// overflow.
+void launch_set_insert(hipStream_t s, const uint64_t *keys, uint64_t n, uint64_t *set, uint64_t cap);
 // Keys of a set to out[0..*count) in any order; the set's slots are cleared.
 // *count must be zero before; out must hold cap keys.
 void launch_set_extract(hipStream_t s, uint64_t *set, uint64_t cap, uint64_t *out, uint64_t *count);

@@ -400,6 +400,26 @@ __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict
     }
 }
 
+// Inserts keys[0..n) (nonzero) into a set of more than n slots: probing
+// ends at the first empty slot, which a set that cannot fill always has.
+__global__ __launch_bounds__(256) void k_set_insert(const uint64_t *__restrict__ keys, uint64_t n,
+                                                    uint64_t *__restrict__ set, uint64_t mask) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        uint64_t h = hash_mix(k) & mask;
+        for (uint64_t q = 0; q <= mask; ++q) {
+            const uint64_t cur = set[h];
+            if (cur == k) break;
+            if (cur == 0) {
+                const uint64_t old = atomicCAS((unsigned long long *)&set[h], 0ull, (unsigned long long)k);
+                if (old == 0 || old == k) break;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+}
+
 // Moves the keys of a set to out[] (any order) and clears their slots, so
 // the set is empty again for its next layer; *count (zero before) gets the
 // number of keys.
@@ -1335,6 +1355,12 @@ void launch_expand_hash(hipStream_t s, const LaunchShape &ls, const uint64_t *pa
     hipLaunchKernelGGL(k_expand_hash, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
                        (unsigned long long *)tot, set2, set2 ? cap2 - 1 : 0ull, set4, set4 ? cap4 - 1 : 0ull, meta2,
                        meta4);
+}
+
+void launch_set_insert(hipStream_t s, const uint64_t *keys, uint64_t n, uint64_t *set, uint64_t cap) {
+    if (n == 0) return;
+    const uint64_t want = blocks_for(n, 256);
+    hipLaunchKernelGGL(k_set_insert, dim3(want < 65536 ? want : 65536), dim3(256), 0, s, keys, n, set, cap - 1);
 }
 
 void launch_set_extract(hipStream_t s, uint64_t *set, uint64_t cap, uint64_t *out, uint64_t *count) {

@@ -50,3 +50,20 @@ def test_batch_rejects_unpinned_patterns(pkg):
     h = ctypes.c_void_p()
     rc = native.lib().egtb_batch_build(ctypes.byref(job), 1, ctypes.byref(o), ctypes.byref(h))
     assert rc != 0 and "pin nibbles" in native.last_error()
+
+
+def test_batch_key_sets_match_sorted_dedup(pkg):
+    """The batched forward pass deduplicates in key sets by default; with an
+    explicit sort schedule it sorts and uniques every step. Both give every
+    job the same books."""
+    from importlib import import_module
+
+    batched = import_module("2048endgametablebase_amd.batched")
+    jobs = [("442", 7), ("LL", 6), ("t", 6)]
+    print("case batched key sets vs sort schedule 1", flush=True)
+    with batched.build_batch(jobs) as a, batched.build_batch(jobs, sort_schedule=1) as b:
+        for j, (pat, t) in enumerate(jobs):
+            for i in range(pkg.patterns.build_params(pat, t)["steps"]):
+                ka, ra = a.book(j, i)
+                kb, rb = b.book(j, i)
+                assert np.array_equal(ka, kb) and ra.tobytes() == rb.tobytes(), (pat, t, i)
