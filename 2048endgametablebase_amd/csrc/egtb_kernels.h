@@ -179,12 +179,15 @@ void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, 
                          uint32_t bits, uint32_t *idx);
 
 template <typename T>
-// clear / clear_n: a hash table the same launch zeroes (the next layer's, whose
+// clear / clear_n: a hash table the same launch zeroes (a later layer's, whose
 // slots no probe of this launch reads), so its fill needs no launch of its own.
+// own / own_cap: this layer's own solve hash (cleared before the launch),
+// which gets every state with a rate above zero.
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
                   const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
-                  ulonglong2 *clear = nullptr, uint64_t clear_n = 0);
+                  ulonglong2 *clear = nullptr, uint64_t clear_n = 0, ulonglong2 *own = nullptr,
+                  uint64_t own_cap = 0);
 
 template <typename T>
 void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,

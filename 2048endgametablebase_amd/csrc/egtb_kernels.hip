@@ -1033,13 +1033,22 @@ __device__ __forceinline__ uint64_t rate_bits(T v) {
     else return __builtin_bit_cast(uint64_t, v);
 }
 
+// Record (k, r) in a layer's solve hash (keys of a layer are unique and the
+// table is at most a quarter full, so the claim ends at an empty slot).
+template <typename T>
+__device__ __forceinline__ void hash_put(ulonglong2 *__restrict__ slots, uint64_t mask, uint64_t k, T r) {
+    uint64_t sl = hash_mix(k) & mask;
+    while (atomicCAS((unsigned long long *)&slots[sl].x, 0ull, (unsigned long long)k) != 0ull) sl = (sl + 1) & mask;
+    slots[sl].y = rate_bits(r);
+}
+
 template <typename T>
 __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ keys, uint64_t n,
                                                 const uint16_t *__restrict__ glut, PatternDev p,
                                                 int do_check, HashDev h1, HashDev h2, TableDev<T> t1,
                                                 TableDev<T> t2, double w2, double w4, T max_scale,
                                                 T *__restrict__ rates, ulonglong2 *__restrict__ clear,
-                                                uint64_t clear_n) {
+                                                uint64_t clear_n, ulonglong2 *__restrict__ own, uint64_t own_mask) {
     extern __shared__ uint16_t lut[];
     // the table filled after this launch: disjoint from h1 / h2
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < clear_n;
@@ -1049,12 +1058,16 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
     // hash probes when both successor layers have a hash, else binary
     // searches narrowed by the bucket index (uniform over the launch)
     const bool use_hash = h1.slots != nullptr && h2.slots != nullptr;
-    const uint64_t n1 = t1.n_ptr ? *t1.n_ptr : t1.n, n2 = t2.n_ptr ? *t2.n_ptr : t2.n;
+    // (the sizes of the successor books are read only by the searches: with
+    // hashes their compaction may still be running on another stream)
+    const uint64_t n1 = use_hash ? 0 : (t1.n_ptr ? *t1.n_ptr : t1.n);
+    const uint64_t n2 = use_hash ? 0 : (t2.n_ptr ? *t2.n_ptr : t2.n);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t b = keys[i];
         if (do_check && is_success(b, p)) {
             rates[i] = max_scale;
+            if (own) hash_put(own, own_mask, b, max_scale);
             continue;
         }
         double sp = 0.0;
@@ -1146,7 +1159,9 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
             sp = __fma_rn((double)best[0], w2, sp);
             sp = __fma_rn((double)best[1], w4, sp);
         }
-        rates[i] = e > 0 ? (T)(sp / (double)e) : (T)0;
+        const T rv = e > 0 ? (T)(sp / (double)e) : (T)0;
+        rates[i] = rv;
+        if (own && rv > (T)0) hash_put(own, own_mask, b, rv);
     }
 }
 
@@ -1513,7 +1528,7 @@ template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
                   const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
-                  ulonglong2 *clear, uint64_t clear_n) {
+                  ulonglong2 *clear, uint64_t clear_n, ulonglong2 *own, uint64_t own_cap) {
     if (n == 0) {
         if (clear_n) (void)hipMemsetAsync(clear, 0, clear_n * sizeof(ulonglong2), s);
         return;
@@ -1521,7 +1536,8 @@ void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, ui
     const uint32_t want = blocks_for(n > clear_n ? n : clear_n, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     hipLaunchKernelGGL((k_solve<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n,
-                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates, clear, clear_n);
+                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates, clear, clear_n, own,
+                       own ? own_cap - 1 : 0ull);
 }
 
 // Slots for a layer of at most `bound` records: a power of two, at most a
@@ -1597,7 +1613,7 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
     template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
                                   const uint16_t *, const PatternDev &, int, const HashDev &,                 \
                                   const HashDev &, const TableDev<T> &, const TableDev<T> &, double, double,  \
-                                  T, T *, ulonglong2 *, uint64_t);                                            \
+                                  T, T *, ulonglong2 *, uint64_t, ulonglong2 *, uint64_t);                    \
     template void launch_hash_build<T>(hipStream_t, const uint64_t *, const T *, uint64_t, const uint64_t *,  \
                                        ulonglong2 *, uint64_t);                                               \
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \

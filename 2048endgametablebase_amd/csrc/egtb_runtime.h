@@ -28,6 +28,7 @@ struct DeviceCtx {
     int cus = 0;
     uint16_t *lut = nullptr;  // 65536-entry left-slide row table
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;  // second stream: the backward pass's compactions
     LaunchShape shape{};
 };
 
@@ -231,7 +232,8 @@ struct Result {
     // no longer needed, then in host memory
     bool host_tables = false;
     BookRing book_ring;           // books' device memory until they are on the host
-    std::vector<DBuf> hash_bufs;  // ring of 3 solve hashes: layer i's in slot i % 3
+    std::vector<DBuf> hash_bufs;  // ring of solve hashes: layer i's in slot i % hash_ring
+    int hash_ring = 3;
     double hash_max_bytes = 0;    // larger layers are searched, not hashed
     bool book_index = true;       // books get the bucket index of the solve's search path
     std::vector<HostSlab> host_slabs;            // own the memory of host_keys / host_rates

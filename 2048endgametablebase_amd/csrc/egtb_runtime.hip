@@ -71,6 +71,7 @@ DeviceCtx &device_ctx(int dev) {
     configure_kernels();
     configure_bucket_kernels();
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&c->lut, kLutEntries * sizeof(uint16_t)));
     launch_build_lut(c->stream, c->lut);
     HIPCHK(hipGetLastError());
@@ -618,6 +619,8 @@ static uint32_t index_bits_for(uint64_t n) {
 // Layers whose hash would exceed this many MiB are searched instead: a
 // hash is built with one random atomic per record into a table four times
 // the layer's size, which outgrows the caches on the largest layers.
+// per-layer rates kept for the second stream's compactions, at most
+constexpr double kTwoStreamRatesMax = 16.0 * 1024 * 1024 * 1024;
 constexpr int kDefaultSolveHashMB = 1024;
 
 // The table of book `layer`'s solve hash (its slots and mask go on the
@@ -627,10 +630,11 @@ static ulonglong2 *hash_table_for(hipStream_t s, Result &res, int layer, uint64_
     Book &bk = res.books[layer];
     cap = hash_capacity(bound);
     if ((double)cap * sizeof(ulonglong2) > res.hash_max_bytes) return nullptr;
-    // at most three hashes are alive (layers i, i+1, i+2 while layer i is
-    // solved): layer i takes ring slot i % 3, whose previous table (layer
-    // i+3) was last read by the solve of layer i+1, queued before
-    DBuf &h = res.hash_bufs[layer % 3];
+    // with a ring of 3, at most three hashes are alive (layers i, i+1, i+2
+    // while layer i is solved): layer i takes slot i % 3, whose previous
+    // table (layer i+3) was last read by the solve of layer i+1, queued
+    // before. A ring of 4 lets the solve of layer i+1 clear layer i's slot.
+    DBuf &h = res.hash_bufs[layer % res.hash_ring];
     h.ensure(cap * sizeof(ulonglong2), s);
     bk.hslots = (const ulonglong2 *)h.p;
     bk.hmask = cap - 1;
@@ -677,19 +681,28 @@ static void drop_hash(Result &res, int layer) {
 // Carve book storage for a layer of at most `bound` live records, compact
 // (keys, rates) into it (stable, rate > 0) and index it.  Sizes stay on the
 // device; nothing is synchronised.
+// Book storage for a layer of at most `bound` live records (host-side
+// carving; a new arena chunk is allocated on the build stream).
+static void *book_bytes(Result &res, int layer, size_t bytes) {
+    if (!res.host_tables) return res.book_arena.alloc(bytes);
+    return res.book_ring.alloc(layer, bytes + 256);
+}
+
 template <typename T>
-static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *keys, const T *rates, uint64_t bound,
-                      SortWork &w, bool index, bool hash_zeroed = false) {
+static void carve_book(Result &res, int layer, uint64_t bound) {
+    Book &bk = res.books[layer];
+    bk.n_dev = &res.meta_dev()[layer].n_live;
+    bk.keys = (uint64_t *)book_bytes(res, layer, bound * 8);
+    bk.rates = book_bytes(res, layer, bound * sizeof(T));
+}
+
+// Compact a carved book: (keys, rates) -> book storage (stable, rate > 0)
+// and index it. Sizes stay on the device; nothing is synchronised.
+template <typename T>
+static void compact_book(hipStream_t s, Result &res, int layer, const uint64_t *keys, const T *rates, uint64_t bound,
+                         SortWork &w, bool index, bool hash_zeroed = false) {
     Book &bk = res.books[layer];
     LayerDev *md = res.meta_dev() + layer;
-    bk.n_dev = &md->n_live;
-    auto carve = [&](size_t bytes, int which) -> void * {
-        if (!res.host_tables) return res.book_arena.alloc(bytes);
-        (void)which;
-        return res.book_ring.alloc(layer, bytes + 256);
-    };
-    bk.keys = (uint64_t *)carve(bound * 8, 0);
-    bk.rates = carve(bound * sizeof(T), 1);
     if (bound) w.ensure_lookback(s, 0, 0, compact_tiles(bound));
     // the compaction also fills the layer's solve hash (read by the solves
     // of layers i-1 and i-2; layer 0 is read by none)
@@ -705,9 +718,17 @@ static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *key
     }
     if (index && res.book_index) {
         bk.bits = index_bits_for(bound);
-        bk.idx = (uint32_t *)carve(((1ull << bk.bits) + 1) * 4, 2);
+        bk.idx = (uint32_t *)book_bytes(res, layer, ((1ull << bk.bits) + 1) * 4);
         launch_bucket_index(s, bk.keys, bound, bk.n_dev, bk.bits, bk.idx);
     }
+}
+
+// Carve and compact (the single-stream path).
+template <typename T>
+static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *keys, const T *rates, uint64_t bound,
+                      SortWork &w, bool index, bool hash_zeroed = false) {
+    carve_book<T>(res, layer, bound);
+    compact_book<T>(s, res, layer, keys, rates, bound, w, index, hash_zeroed);
 }
 
 void HostSlab::map(size_t n) {
@@ -1396,7 +1417,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T), steps);
     }
     res.hash_bufs.clear();
-    res.hash_bufs.resize(3);
+    res.hash_bufs.resize(4);
+    res.hash_ring = 3;
     res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
                          : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
     SortWork w;
@@ -1947,6 +1969,16 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         res.book_index = !all_hashed;
     }
 
+    // Two streams for the backward pass of an in-HBM build: each solve
+    // fills its own layer's hash, so the compaction that makes the book
+    // leaves the chain of solves and runs on the second stream; every layer
+    // keeps its own rates until its compaction has read them.
+    uint64_t rates_all_bytes = 0;
+    for (int i = 0; i < steps; ++i) rates_all_bytes += (res.info[i].n_raw * sizeof(T) + 255) & ~(uint64_t)255;
+    const bool two_stream = steps >= 3 && !files && !streamer && !res.book_index && o.deletion_threshold <= 0.0 &&
+                            (double)rates_all_bytes <= kTwoStreamRatesMax;
+    if (two_stream) res.hash_ring = 4;
+
     // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
 
     uint64_t max_n = 0;
@@ -1974,9 +2006,58 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     }
 
     // ---- backward (reference recalculate_process_impl, BookSolver.cpp:513-605)
+    if (two_stream) {
+        // layer i: the solve (build stream) reads the hashes of layers i+1
+        // and i+2, fills layer i's (cleared by the solve of layer i+1) and
+        // clears layer i-1's; then the compaction that makes book i runs on
+        // the second stream, from this layer's own rates
+        hipStream_t s2 = C.aux;
+        std::vector<uint64_t> roff(steps + 1, 0);
+        for (int i = 0; i < steps; ++i)
+            roff[i + 1] = roff[i] + ((res.info[i].n_raw * sizeof(T) + 255) & ~(uint64_t)255);
+        DBuf rates_all;
+        rates_all.alloc(roff[steps] + 256, s);
+        // the look-back words at their largest size now: the second stream
+        // must never see them reallocated
+        w.ensure_lookback(s, 0, 0, compact_tiles(max_n ? max_n : 1));
+        hipEvent_t fork_ev, join_ev;
+        HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        EventGuard fork_guard{fork_ev};
+        HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+        EventGuard join_guard{join_ev};
+        const int i0 = steps - 3;
+        uint64_t own_cap = 0;
+        ulonglong2 *own = i0 > 0 ? hash_table_for(s, res, i0, res.info[i0].n_raw, own_cap) : nullptr;
+        if (own) HIPCHK(hipMemsetAsync(own, 0, own_cap * sizeof(ulonglong2), s));
+        for (int i = i0; i >= 0; --i) {
+            progress_set(prog_total - (uint32_t)i - 2u, prog_total);
+            const uint64_t n = res.info[i].n_raw;
+            const Book &b1 = res.books[i + 1], &b2 = res.books[i + 2];
+            uint64_t next_cap = 0;
+            ulonglong2 *next = i - 1 > 0 ? hash_table_for(s, res, i - 1, res.info[i - 1].n_raw, next_cap) : nullptr;
+            T *ri = (T *)((char *)rates_all.p + roff[i]);
+            {
+                Timer t(s, "solve", (double)n * (8 + sizeof(T)));
+                launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
+                                hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, ri, next,
+                                next ? next_cap : 0, own, own ? own_cap : 0);
+            }
+            carve_book<T>(res, i, n);  // before the fork: a new arena chunk is allocated on the build stream
+            HIPCHK(hipEventRecord(fork_ev, s));
+            HIPCHK(hipStreamWaitEvent(s2, fork_ev, 0));
+            compact_book<T>(s2, res, i, res.raw[i], ri, n, w, false);
+            drop_hash(res, i + 2);
+            HIPCHK(hipGetLastError());
+            own = next;
+            own_cap = next_cap;
+        }
+        HIPCHK(hipEventRecord(join_ev, s2));
+        HIPCHK(hipStreamWaitEvent(s, join_ev, 0));
+        rates_all.release();
+    }
     bool started = false;
     DBuf thr_keys, thr_rates, thr_cnt;
-    for (int i = steps - 3; i >= 0; --i) {
+    for (int i = two_stream ? -1 : steps - 3; i >= 0; --i) {
         progress_set(prog_total - (uint32_t)i - 2u, prog_total);
         if (files && !started && have_book[i]) continue;  // solved by an earlier run
         if (!started) {
