@@ -1451,8 +1451,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         }
         if (o.write_raw_files) raw_writer = std::make_unique<RawFileWriter>(o.device);
     }
-    auto queue_raw = [&](int L, uint64_t n) {
-        if (raw_writer && !have_book[L]) raw_writer->write(prefix + std::to_string(L), res.raw[L], n, s);
+    auto queue_raw = [&](int L, uint64_t n, hipStream_t producer = nullptr) {
+        if (raw_writer && !have_book[L])
+            raw_writer->write(prefix + std::to_string(L), res.raw[L], n, producer ? producer : s);
     };
     LayerDev *M = res.meta_dev();
 
@@ -1583,7 +1584,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         uint64_t cap = 0;  // slots in use (a power of two); the rest of buf is zero
     };
     KeySet sets[3];
-    DBuf ubuf[2];  // unsorted unique layers, layer L in ubuf[L & 1]
+    // unsorted unique layers, layer L in ubuf[L % kURing]: a layer's sort
+    // runs on the second stream a few steps after its extraction
+    constexpr int kURing = 8;
+    DBuf ubuf[kURing];
     hipEvent_t hev[2];
     HIPCHK(hipEventCreateWithFlags(&hev[0], hipEventDisableTiming));
     EventGuard hev0_guard{hev[0]};
@@ -1622,12 +1626,76 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                            s4 ? (uint64_t *)s4->buf.p : nullptr, s4 ? s4->cap : 0, &M[step].or_all,
                            &M[step + 1].or_all);
     };
+    // Sorts of unique layers on the second stream, in batches of kSortBatch
+    // layers: one fork (the second stream waits for everything queued on
+    // the build stream, which includes the expand that reads each layer) per
+    // batch, and the build stream waits for a batch only before reusing its
+    // layers' buffers. Explicit sort schedules sort in line instead.
+    constexpr size_t kSortBatch = 4;
+    const bool defer = o.sort_schedule == 0;
+    hipStream_t s2 = C.aux;
+    SortWork w_side;  // the second stream's sort state
+    std::vector<int> pending;
+    struct Batch {
+        int last;
+        hipEvent_t ev;
+    };
+    std::deque<Batch> batches;
+    int s_waited = 0;  // layers up to this one are sorted before anything the build stream runs now
+    hipEvent_t fork2;
+    HIPCHK(hipEventCreateWithFlags(&fork2, hipEventDisableTiming));
+    EventGuard fork2_guard{fork2};
+    struct BatchGuard {
+        std::deque<Batch> &b;
+        ~BatchGuard() {
+            for (Batch &x : b) (void)hipEventDestroy(x.ev);
+        }
+    } batch_guard{batches};
+    auto submit = [&]() {
+        if (pending.empty()) return;
+        HIPCHK(hipEventRecord(fork2, s));
+        HIPCHK(hipStreamWaitEvent(s2, fork2, 0));
+        for (int L : pending) {
+            const uint64_t n = res.info[L].n_raw;
+            uint64_t *u = (uint64_t *)ubuf[L % kURing].p;
+            {
+                Timer t(s2, "sort", 16.0 * (double)n);
+                uint64_t *out = radix_sort(s2, u, res.raw[L], n, H[L].or_all, H[L].and_all, o.radix_bits, w_side, 0);
+                if (out != res.raw[L] && n)
+                    HIPCHK(hipMemcpyAsync(res.raw[L], out, n * 8, hipMemcpyDeviceToDevice, s2));
+            }
+            if (L <= steps - 2) queue_raw(L, n, s2);
+        }
+        hipEvent_t ev;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        batches.push_back(Batch{pending.back(), ev});
+        HIPCHK(hipEventRecord(ev, s2));
+        pending.clear();
+    };
+    // the build stream waits for the sort of layer L (already submitted)
+    auto wait_sorted = [&](int L) {
+        if (L <= s_waited) return;
+        while (batches.size() > 1 && batches.front().last < L) {  // a later batch implies this one
+            (void)hipEventDestroy(batches.front().ev);
+            batches.pop_front();
+        }
+        if (batches.empty() || batches.front().last < L)
+            throw Error(EGTB_ERR_INTERNAL, "layer " + std::to_string(L) + " was not queued for its sort");
+        HIPCHK(hipStreamWaitEvent(s, batches.front().ev, 0));
+        s_waited = batches.front().last;
+    };
+    // every sort queued so far finishes before the build stream goes on
+    auto join_sorts = [&]() {
+        submit();
+        if (!batches.empty()) wait_sorted(batches.back().last);
+    };
     auto extract = [&](int L) {
         KeySet &k = set_of(L);
-        ubuf[L & 1].ensure(k.cap * 8, s);
+        if (defer && L - kURing >= 1) wait_sorted(L - kURing);  // the slot's previous layer
+        ubuf[L % kURing].ensure(k.cap * 8, s);
         {
             Timer t(s, "extract", 16.0 * (double)k.cap);
-            launch_set_extract(s, (uint64_t *)k.buf.p, k.cap, (uint64_t *)ubuf[L & 1].p, &M[L].n_raw);
+            launch_set_extract(s, (uint64_t *)k.buf.p, k.cap, (uint64_t *)ubuf[L % kURing].p, &M[L].n_raw);
         }
         HIPCHK(hipMemcpyAsync(H + L, M + L, sizeof(LayerDev), hipMemcpyDeviceToHost, s));
         HIPCHK(hipEventRecord(hev[L & 1], s));
@@ -1642,8 +1710,12 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     };
     // layer L (size known, sets drained) sorted into the raw arena
     auto sort_layer = [&](int L, uint64_t n) {
-        res.raw[L] = (uint64_t *)res.raw_arena.alloc(n * 8);
-        uint64_t *u = (uint64_t *)ubuf[L & 1].p;
+        res.raw[L] = (uint64_t *)res.raw_arena.alloc(n * 8);  // before the batch's fork
+        if (defer) {
+            pending.push_back(L);
+            return;
+        }
+        uint64_t *u = (uint64_t *)ubuf[L % kURing].p;
         if (o.sort_schedule == 0) {
             Timer t(s, "sort", 16.0 * (double)n);
             uint64_t *out = radix_sort(s, u, res.raw[L], n, H[L].or_all, H[L].and_all, o.radix_bits, w, 0);
@@ -1661,6 +1733,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // exactly the spawn-4 children of step L.
     auto recover = [&](int L) {
         uint64_t cap = set_of(L).cap;
+        submit();  // the steps run again from the sorted layers below
+        HIPCHK(hipStreamSynchronize(s2));
         while (H[L].pad[0]) {
             HIPCHK(hipStreamSynchronize(s));
             cap *= 4;
@@ -1711,8 +1785,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                         handover = true;
                     } else {
                         set_prepare(set_of(i + 2), c);
-                        hash_step(i + 1, (const uint64_t *)ubuf[i & 1].p, ubuf[i & 1].bytes / 8, &set_of(i + 1),
-                                  &set_of(i + 2), &M[i + 1].tot);
+                        hash_step(i + 1, (const uint64_t *)ubuf[i % kURing].p, ubuf[i % kURing].bytes / 8,
+                                  &set_of(i + 1), &set_of(i + 2), &M[i + 1].tot);
                         extract(i + 1);
                         queued = true;
                     }
@@ -1725,8 +1799,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                     recover(i);
                     if (queued) {  // queue again what the recovery undid
                         if (i + 1 <= steps - 2) {
-                            hash_step(i + 1, (const uint64_t *)ubuf[i & 1].p, ubuf[i & 1].bytes / 8, &set_of(i + 1),
-                                      &set_of(i + 2), &M[i + 1].tot);
+                            hash_step(i + 1, (const uint64_t *)ubuf[i % kURing].p, ubuf[i % kURing].bytes / 8,
+                                      &set_of(i + 1), &set_of(i + 2), &M[i + 1].tot);
                             extract(i + 1);
                         } else {
                             extract(steps - 1);
@@ -1743,6 +1817,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 check_layer(i, n);
                 sort_layer(i, n);
                 if (handover) {
+                    join_sorts();  // the sort loop reads the sorted layers on the build stream
                     // the sort loop continues at step i+1. Its candidate
                     // buffer starts with the spawn-4 children of step i:
                     // written here again by the one-pass expand from the
@@ -1754,8 +1829,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                     cur.alloc(room * 8 + 256, s);
                     nxt.alloc(room * 8 + 256, s);
                     for (KeySet &q : sets) q.buf.release();
-                    ubuf[0].release();
-                    ubuf[1].release();
+                    for (DBuf &u : ubuf) u.release();
                     HIPCHK(hipMemsetAsync(cnt_dev.p, 0, 8, s));  // its write positions come from this total
                     {
                         Timer t(s, "expand", 0.0);
@@ -1770,8 +1844,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                     first_sort_step = i + 1;
                     break;
                 }
-                queue_raw(i, n);
+                if (!defer) queue_raw(i, n);  // (a deferred sort queues it after sorting)
                 if (streamer) streamer->reserve(n);
+                if (defer && pending.size() >= kSortBatch) submit();
                 HIPCHK(hipGetLastError());
             }
             if (first_sort_step == 0) {  // the last layer
@@ -1780,10 +1855,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 if (H[L].pad[0]) recover(L);
                 res.info[L].n_raw = H[L].n_raw;
                 sort_layer(L, H[L].n_raw);
+                join_sorts();
                 if (streamer) streamer->reserve(H[L].n_raw);
                 for (KeySet &q : sets) q.buf.release();
-                ubuf[0].release();
-                ubuf[1].release();
+                for (DBuf &u : ubuf) u.release();
                 hash_done = true;
             }
         }
