@@ -1631,7 +1631,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // the build stream, which includes the expand that reads each layer) per
     // batch, and the build stream waits for a batch only before reusing its
     // layers' buffers. Explicit sort schedules sort in line instead.
-    constexpr size_t kSortBatch = 4;
+    // 442@512: 65-67 ms with a fork per layer, 67-70 per 2, 72.7 per 4, 78
+    // per 8 layers: a sort started early matters more than the ~18 us of
+    // host time of an event pair.
+    constexpr size_t kSortBatch = 1;
     const bool defer = o.sort_schedule == 0;
     hipStream_t s2 = C.aux;
     SortWork w_side;  // the second stream's sort state
