@@ -1654,6 +1654,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             for (Batch &x : b) (void)hipEventDestroy(x.ev);
         }
     } batch_guard{batches};
+    struct SideSync {  // on the way out: sorts still running read buffers freed on the build stream
+        hipStream_t side;
+        ~SideSync() { (void)hipStreamSynchronize(side); }
+    } sort_sync{s2};
     auto submit = [&]() {
         if (pending.empty()) return;
         HIPCHK(hipEventRecord(fork2, s));
@@ -2095,6 +2099,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             roff[i + 1] = roff[i] + ((res.info[i].n_raw * sizeof(T) + 255) & ~(uint64_t)255);
         DBuf rates_all;
         rates_all.alloc(roff[steps] + 256, s);
+        struct SideSync {  // on the way out: the second stream's compactions read rates_all
+            hipStream_t side;
+            ~SideSync() { (void)hipStreamSynchronize(side); }
+        } side_sync{s2};
         // the look-back words at their largest size now: the second stream
         // must never see them reallocated
         w.ensure_lookback(s, 0, 0, compact_tiles(max_n ? max_n : 1));
