@@ -153,8 +153,8 @@ def main():
                     help="copy 1 GiB with torch first (a kernel of known bytes for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
-              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a layer would be sharded by "
+              "key range, with an all-to-all of candidates); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

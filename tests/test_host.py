@@ -10,7 +10,7 @@ ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 
 
 @pytest.mark.parametrize("pattern,target,steps,docheck", [
-    # SURVEY Appendix A, derived from BookBuilder.py:124-128
+    # the reference's steps / docheck formulas (engine_core/BookBuilder.py:124-128) worked out per config
     ("L3", 7, 112, 60), ("L3", 8, 176, 124), ("L3", 9, 304, 252), ("L3", 10, 560, 508),
     ("442", 9, 304, 252), ("444", 8, 188, 127), ("444", 11, 1084, 1023),
     ("4442", 11, 1072, 1020), ("LL", 9, 304, 249), ("4431", 9, 304, 249),
