@@ -133,12 +133,11 @@ static std::atomic<int> g_stats_on{0};
 // of 2-4 MiB then stay about within an XCD's L2; 3 was 1-3% slower)
 constexpr double kSetSlotsPerKey = 2.0;
 static std::atomic<double> g_set_slots_per_key{kSetSlotsPerKey};
-// largest forward key set (128 MiB); steps that need more sort their
-// candidates: LL@64 (sets up to 2^24 slots) builds in 120 ms with sets
-// against 161 ms sorting, while 444@2048's 2^26-slot sets (inserts going
-// to HBM rather than the last-level cache) made its forward pass 5.1 s
-// against 3.0 s
-constexpr uint64_t kSetMaxSlots = (uint64_t)1 << 24;
+// largest forward key set (256 MiB); steps that need more sort their
+// candidates. 444@2048's forward pass, with the sorts on the second
+// stream: 2.78 s at a limit of 2^24 slots (176 steps on sets), 1.78 s at
+// 2^25 (every step), 4.8 s at 2^26 (inserts at HBM speed)
+constexpr uint64_t kSetMaxSlots = (uint64_t)1 << 25;
 static std::atomic<uint64_t> g_set_max_slots{kSetMaxSlots};
 static std::mutex g_stats_mu;
 struct StatAcc {

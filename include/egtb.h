@@ -312,8 +312,8 @@ EGTB_EXPORT int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits,
 /* Sizing of the forward pass's key sets.  slots_per_key: slots per
  * predicted key (default 2: a set is at most about half full when the
  * growth estimate holds; valid 0 < v <= 64).  max_slots: the largest set
- * (default 2^24 slots, 128 MiB: random inserts pay while the two sets being
- * filled stay about within the 256 MiB last-level cache); a step that would need a larger one
+ * (default 2^25 slots, 256 MiB: random inserts pay while the sets being
+ * filled stay near the 256 MiB last-level cache); a step that would need a larger one
  * and the rest of the forward pass sort and unique their candidates instead.
  * 0 restores a default.  A set that fills up is regrown and its steps run
  * again, so small slots_per_key values only cost time.  No counterpart in
