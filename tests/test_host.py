@@ -73,3 +73,18 @@ def test_save_config_to_txt(pkg, tmp_path):
 def test_unknown_pattern_raises(pkg, tmp_path):
     with pytest.raises(KeyError):
         pkg.start_build("nope", 9, str(tmp_path / "x_"))
+
+
+def test_key_set_sizing_and_forward_query_check_arguments(pkg):
+    """The forward key-set tuning and the forward-dedup query validate their
+    arguments on the host (no GPU touched)."""
+    L = pkg.native.lib()
+    assert L.egtb_set_key_set_sizing(-1.0, 0) != 0
+    assert "slots per key" in pkg.native.last_error()
+    assert L.egtb_set_key_set_sizing(64.5, 0) != 0
+    assert L.egtb_set_key_set_sizing(2.0, (1 << 34) + 1) != 0
+    assert "max_slots" in pkg.native.last_error()
+    assert L.egtb_set_key_set_sizing(2.0, 1 << 20) == 0
+    assert L.egtb_set_key_set_sizing(0.0, 0) == 0  # back to the defaults
+    a, b = ctypes.c_uint64(), ctypes.c_uint64()
+    assert L.egtb_result_forward(None, ctypes.byref(a), ctypes.byref(b)) != 0
