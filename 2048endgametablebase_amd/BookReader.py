@@ -54,10 +54,14 @@ def round_half_away(x: float) -> float:
 
 
 def maybe_round_value(value: float, dtype: str) -> float:
-    """ReaderRuntime.cpp:208-214: 32-bit dtypes are rounded to 1e-9."""
+    """ReaderRuntime.cpp:208-214: 32-bit dtypes are rounded to 1e-9.  The
+    reference's release build (-Ofast, CMakeLists.txt:79-95) compiles the
+    division by 10^9 as a multiplication by the double nearest 1e-9
+    (-freciprocal-math), so that is the value its reader returns: e.g.
+    0.23655372900000002 where a division would give 0.236553729."""
     if "32" not in dtype or abs(value) <= 1e-7:
         return value
-    return round_half_away(value * 1e9) / 1e9
+    return round_half_away(value * 1e9) * 1e-9
 
 
 def blank_entries() -> Dict[str, object]:

@@ -2056,8 +2056,13 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // keeps its own rates until its compaction has read them.
     uint64_t rates_all_bytes = 0;
     for (int i = 0; i < steps; ++i) rates_all_bytes += (res.info[i].n_raw * sizeof(T) + 255) & ~(uint64_t)255;
+    // the per-layer rates must also fit in half the free HBM, so a build that
+    // fits with one stream never fails for want of the second's buffers
+    size_t hbm_free = 0, hbm_total = 0;
+    HIPCHK(hipMemGetInfo(&hbm_free, &hbm_total));
     const bool two_stream = steps >= 3 && !files && !streamer && !res.book_index && o.deletion_threshold <= 0.0 &&
-                            (double)rates_all_bytes <= kTwoStreamRatesMax;
+                            (double)rates_all_bytes <= kTwoStreamRatesMax &&
+                            rates_all_bytes + 256 <= (uint64_t)hbm_free / 2;
     if (two_stream) res.hash_ring = 4;
 
     // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
@@ -2231,7 +2236,11 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         HIPCHK(hipStreamSynchronize(s));
         if (hipDeviceGetDefaultMemPool(&pool, res.dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
     }
-    if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
+    // both sort states carry a look-back error flag: the build stream's and
+    // the second stream's (which sorts the small unique layers with the
+    // single-pass look-back); a raised flag means a partial prefix was used
+    if (w.read_error(s) || w_side.read_error(s2))
+        throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     read_meta(s, res, H, 0, steps - 1);
     for (int i = 0; i < steps; ++i) {
         res.info[i].n_live = H[i].n_live;

@@ -22,6 +22,8 @@
 #include <map>
 #include <mutex>
 #include <random>
+#include <set>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -83,6 +85,9 @@ class Server {
     void run();
     bool launch_slot(Slot &s);
     void retire_slot(Slot &s);
+    // answer tickets with EGTB_LOOKUP_ERROR (NaN rates): a batch that could
+    // not run, or requests left queued when the worker stopped
+    void complete_with_error(const uint64_t *tickets, size_t n);
 
     int dev_;
     size_t max_batch_;
@@ -93,6 +98,9 @@ class Server {
     std::deque<Request> queue_;
     std::map<int32_t, TableSet *> tables_;
     std::map<int32_t, int> inflight_;  // batches in flight per table
+    std::set<int32_t> detaching_;      // tables that refuse new requests
+    std::string fail_msg_;             // why the worker stopped (under mu_)
+    bool failed_ = false;              // the worker stopped on an error (under mu_)
     int32_t next_table_ = 0;
     int64_t table_bytes_ = 0;
     uint64_t next_ticket_ = 1;
@@ -159,23 +167,31 @@ int32_t Server::attach(TableSet *t) {
 void Server::detach(int32_t id) {
     std::unique_lock<std::mutex> lk(mu_);
     auto it = tables_.find(id);
-    if (it == tables_.end()) throw Error(EGTB_ERR_ARG, "unknown table id");
+    if (it == tables_.end() || detaching_.count(id)) throw Error(EGTB_ERR_ARG, "unknown table id");
     for (const Request &r : queue_)
         if (r.table == id) throw Error(EGTB_ERR_ARG, "table has queued requests");
-    // wait for its batches in flight
+    // from here on submit refuses the table; wait for its batches in flight
+    // (a failed batch gives its count back, and the worker retires every
+    // busy slot when it stops)
+    detaching_.insert(id);
     while (inflight_[id] > 0) {
         lk.unlock();
         std::this_thread::sleep_for(std::chrono::microseconds(50));
         lk.lock();
     }
-    table_bytes_ -= (int64_t)it->second->bytes;
-    tables_.erase(it);
+    table_bytes_ -= (int64_t)tables_.at(id)->bytes;
+    tables_.erase(id);
+    inflight_.erase(id);
+    detaching_.erase(id);
 }
 
 void Server::submit(int32_t table, const uint64_t *boards, size_t n, uint64_t *tickets) {
     {
         std::lock_guard<std::mutex> lk(mu_);
-        if (!tables_.count(table)) throw Error(EGTB_ERR_ARG, "unknown table id");
+        if (stop_.load())
+            throw Error(failed_ ? EGTB_ERR_HIP : EGTB_ERR_ARG,
+                        "the serving worker has stopped" + (failed_ ? ": " + fail_msg_ : std::string()));
+        if (!tables_.count(table) || detaching_.count(table)) throw Error(EGTB_ERR_ARG, "unknown table id");
         for (size_t i = 0; i < n; ++i) {
             const uint64_t t = next_ticket_++;
             queue_.push_back(Request{t, boards[i], table});
@@ -185,7 +201,25 @@ void Server::submit(int32_t table, const uint64_t *boards, size_t n, uint64_t *t
     cv_.notify_one();
 }
 
+void Server::complete_with_error(const uint64_t *tickets, size_t n) {
+    if (!n) return;
+    const auto now = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(cmu_);
+        for (size_t i = 0; i < n; ++i) {
+            Completion c;
+            c.ticket = tickets[i];
+            for (float &v : c.out4) v = NAN;
+            c.status = EGTB_LOOKUP_ERROR;
+            c.done = now;
+            done_.push_back(c);
+        }
+    }
+    ccv_.notify_all();
+}
+
 bool Server::launch_slot(Slot &s) {
+    TableSet *ts = nullptr;
     {
         std::lock_guard<std::mutex> lk(mu_);
         if (queue_.empty()) return false;
@@ -203,20 +237,32 @@ bool Server::launch_slot(Slot &s) {
             }
         }
         s.table = table;
-        inflight_[table]++;
+        auto t = tables_.find(table);
+        if (t != tables_.end()) {
+            ts = t->second;
+            inflight_[table]++;
+        }
     }
-    TableSet *ts;
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        ts = tables_[s.table];
+    if (!ts) {  // the table went away while its requests waited
+        complete_with_error(s.tickets.data(), s.tickets.size());
+        return true;
     }
     const size_t m = s.tickets.size();
-    HIPCHK(hipMemcpyAsync(s.d_boards, s.h_boards, m * 8, hipMemcpyHostToDevice, s.stream));
-    launch_lookup<float>(s.stream, *ts, s.d_boards, m, s.d_out, s.d_status, nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(s.h_out, s.d_out, m * 16, hipMemcpyDeviceToHost, s.stream));
-    HIPCHK(hipMemcpyAsync(s.h_status, s.d_status, m * 4, hipMemcpyDeviceToHost, s.stream));
-    HIPCHK(hipEventRecord(s.done, s.stream));
+    try {
+        HIPCHK(hipMemcpyAsync(s.d_boards, s.h_boards, m * 8, hipMemcpyHostToDevice, s.stream));
+        launch_lookup<float>(s.stream, *ts, s.d_boards, m, s.d_out, s.d_status, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(s.h_out, s.d_out, m * 16, hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipMemcpyAsync(s.h_status, s.d_status, m * 4, hipMemcpyDeviceToHost, s.stream));
+        HIPCHK(hipEventRecord(s.done, s.stream));
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            inflight_[s.table]--;
+        }
+        complete_with_error(s.tickets.data(), s.tickets.size());
+        throw;
+    }
     s.busy = true;
     batches_++;
     queries_ += m;
@@ -264,6 +310,9 @@ void Server::run() {
                     if (launch_slot(s)) did = true;
                 } catch (const std::exception &e) {
                     std::fprintf(stderr, "egtb server: batch launch failed: %s\n", e.what());
+                    std::lock_guard<std::mutex> lk(mu_);
+                    failed_ = true;
+                    fail_msg_ = e.what();
                     stop_.store(true);
                 }
             }
@@ -284,6 +333,15 @@ void Server::run() {
             (void)hipEventSynchronize(s.done);
             retire_slot(s);
         }
+    // requests still queued get an error answer; submit refuses new ones
+    // from now on (it checks stop_ under mu_, which was set before this)
+    std::vector<uint64_t> left;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (const Request &r : queue_) left.push_back(r.ticket);
+        queue_.clear();
+    }
+    complete_with_error(left.data(), left.size());
 }
 
 size_t Server::poll(size_t max, uint64_t *tickets, float *out4, int32_t *status, int wait_us,
@@ -291,6 +349,12 @@ size_t Server::poll(size_t max, uint64_t *tickets, float *out4, int32_t *status,
     std::unique_lock<std::mutex> lk(cmu_);
     if (done_.empty() && wait_us > 0)
         ccv_.wait_for(lk, std::chrono::microseconds(wait_us), [this] { return !done_.empty(); });
+    if (done_.empty() && stop_.load()) {
+        lk.unlock();
+        std::lock_guard<std::mutex> g(mu_);
+        if (failed_) throw Error(EGTB_ERR_HIP, "the serving worker has stopped: " + fail_msg_);
+        return 0;
+    }
     size_t k = 0;
     while (k < max && !done_.empty()) {
         const Completion &c = done_.front();

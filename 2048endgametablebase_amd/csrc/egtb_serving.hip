@@ -120,8 +120,9 @@ __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ bo
                         const double val = (double)raw / scale;
                         r[c_absdir[op][d]] = (OutT)val;
                         // compared as the reader reports them: 4-byte rates
-                        // rounded to 1e-9 (maybe_round_value, ReaderRuntime.cpp:208-214)
-                        top = fmax(top, (round9 && fabs(val) > 1e-7) ? round(val * 1e9) / 1e9 : val);
+                        // rounded to 1e-9 (maybe_round_value, ReaderRuntime.cpp:208-214,
+                        // whose release build multiplies by 1e-9 rather than divides)
+                        top = fmax(top, (round9 && fabs(val) > 1e-7) ? round(val * 1e9) * 1e-9 : val);
                         any = true;
                     }
                 }

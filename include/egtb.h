@@ -180,6 +180,8 @@ typedef struct egtb_table egtb_table;
 
 #define EGTB_LOOKUP_OK 0        /* answered from the table */
 #define EGTB_LOOKUP_FALLBACK 1  /* off-table: depth-2 expectimax fallback values */
+#define EGTB_LOOKUP_ERROR (-1)  /* serving only: the query's batch could not run (its table was
+                                   detached, or a HIP call failed); rates are NaN */
 
 /* Load every <pathname><i>.book (i < steps) of a build into HBM.  A missing
  * layer file reads as off-table, an empty one as all-zero rates, as in the
@@ -252,7 +254,9 @@ EGTB_EXPORT int egtb_server_detach(egtb_server *s, int32_t table_id);
 EGTB_EXPORT int egtb_server_submit(egtb_server *s, int32_t table_id, const uint64_t *boards, size_t n,
                                    uint64_t *tickets);
 /* Take up to max finished answers (4 floats + status each, as
- * egtb_lookup_batch), waiting up to wait_us for the first. */
+ * egtb_lookup_batch, or EGTB_LOOKUP_ERROR), waiting up to wait_us for the
+ * first.  Once the worker has stopped on a failure, submit fails and poll
+ * fails when no answer is left. */
 EGTB_EXPORT int egtb_server_poll(egtb_server *s, size_t max, uint64_t *tickets, float *out4, int32_t *status,
                                  size_t *n_out, int32_t wait_us);
 EGTB_EXPORT int egtb_server_stats(const egtb_server *s, uint64_t *batches, uint64_t *queries, uint64_t *max_batch);

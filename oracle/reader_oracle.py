@@ -78,6 +78,13 @@ def std_round(x):
     return t
 
 
+def round9(x):
+    """maybe_round_value (ReaderRuntime.cpp:208-214) as the reference's
+    release flags compile it: -Ofast implies -freciprocal-math, so the
+    division by 10^9 becomes a multiplication by the double nearest 1e-9."""
+    return std_round(x * 1e9) * 1e-9
+
+
 # record layout and max scale per dtype (FormationRuntime.h:197-200,
 # ReaderRuntime.cpp:183-206: integer rates are divided by their max scale)
 REC_FORMAT = {"float32": ("<Qf", 1.0), "float64": ("<Qd", 1.0), "uint32": ("<QI", 4e9),
@@ -134,7 +141,7 @@ def move_on_dic(board, folder, pattern_full, nums_adjust, masks, symm, dtype="fl
                 continue
             val = find_in_file(path, orc.canonical(moved, symm), *REC_FORMAT[dtype])
             if "32" in dtype and abs(val) > 1e-7:
-                val = std_round(val * 1e9) / 1e9
+                val = round9(val)
             vals[{0: 2, 1: 1, 2: 3, 3: 0}[index]] = val
         if all(x is None for x in vals):
             continue

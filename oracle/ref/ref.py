@@ -1,5 +1,5 @@
 """ctypes loader for oracle/_ref/libegtb_ref.so: the reference's own classic
-builder compiled from its sources by oracle/ref/Makefile.
+builder and classic reader compiled from their sources by oracle/ref/Makefile.
 
 TEST INFRASTRUCTURE ONLY (golden-vector generation, CPU baseline timing).
 """
@@ -41,12 +41,38 @@ def lib():
                       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                       ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                       ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        f = L.egtb_ref_build_ex
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                      ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_char_p, ctypes.c_size_t]
+        res = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p,
+               ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]
+        f = L.egtb_ref_move_on_dic
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
+                      ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int] + res
+        L.egtb_ref_reader_new.restype = ctypes.c_void_p
+        L.egtb_ref_reader_new.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        f = L.egtb_ref_reader_move_on_dic
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                      ctypes.c_int64] + res
+        L.egtb_ref_reader_free.restype = None
+        L.egtb_ref_reader_free.argtypes = [ctypes.c_void_p]
+        f = L.egtb_ref_find_classic_value
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p,
+                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
 
 def run_build(seeds, masks, shifts, symm, target, steps, docheck, pathname, is_free=False,
-              p4=0.1, dtype="float32", threads=8) -> None:
+              p4=0.1, dtype="float32", threads=8, deletion_threshold=0.0,
+              optimal_branch_only=False, compress=False) -> None:
     """Run the reference's run_pattern_build_cpp; raises on its exceptions."""
     s = np.ascontiguousarray(seeds, dtype=np.uint64)
     m = np.ascontiguousarray(masks if len(masks) else [0], dtype=np.uint64)
@@ -55,11 +81,92 @@ def run_build(seeds, masks, shifts, symm, target, steps, docheck, pathname, is_f
     # resuming from the layer files already written.
     for attempt in range(2):
         err = ctypes.create_string_buffer(1024)
-        rc = lib().egtb_ref_build(s.ctypes.data, s.size, m.ctypes.data, len(masks), sh.ctypes.data,
-                                  len(shifts), symm, target, steps, docheck, pathname.encode(),
-                                  int(is_free), p4, dtype.encode(), threads, err, 1024)
+        rc = lib().egtb_ref_build_ex(s.ctypes.data, s.size, m.ctypes.data, len(masks), sh.ctypes.data,
+                                     len(shifts), symm, target, steps, docheck, pathname.encode(),
+                                     int(is_free), p4, dtype.encode(), threads,
+                                     float(deletion_threshold), int(bool(optimal_branch_only)),
+                                     int(bool(compress)), err, 1024)
         if rc == 0:
             return
         msg = err.value.decode(errors="replace")
         if attempt or not msg.startswith("The length multiplier"):
             raise RuntimeError(msg)
+
+
+def _result(keys, kinds, values, texts, n, dt):
+    out = []
+    for i in range(n.value):
+        k = keys.raw[16 * i:16 * i + 16].split(b"\0", 1)[0].decode()
+        kind = kinds[i]
+        if kind == 1:
+            v = float(values[i])
+        elif kind == 2:
+            v = texts.raw[16 * i:16 * i + 16].split(b"\0", 1)[0].decode()
+        else:
+            v = None
+        out.append((k, v))
+    return dict(out), dt.value.decode()
+
+
+def _buffers():
+    return (ctypes.create_string_buffer(64), (ctypes.c_int32 * 4)(), (ctypes.c_double * 4)(),
+            ctypes.create_string_buffer(64), ctypes.c_int32(), ctypes.create_string_buffer(32),
+            ctypes.create_string_buffer(512))
+
+
+def _board16(board):
+    b = np.ascontiguousarray(np.asarray(board, dtype=np.int32).reshape(16))
+    return b
+
+
+def move_on_dic(board, folder, dtype, pattern_full, nums_adjust, name, masks, symm):
+    """ClassicBookReader(spec).move_on_dic(board, [(folder, dtype)], pattern_full,
+    nums_adjust) with a fresh reader (symmetry ops tried 0..7).  Returns the
+    reference binding's (dict, dtype) pair: numbers as float, "" strings,
+    None for invalid directions."""
+    b = _board16(board)
+    m = np.ascontiguousarray(masks if len(masks) else [0], dtype=np.uint64)
+    keys, kinds, values, texts, n, dt, err = _buffers()
+    rc = lib().egtb_ref_move_on_dic(b.ctypes.data, folder.encode() if folder else b"", dtype.encode(),
+                                    pattern_full.encode(), int(nums_adjust), name.encode(), m.ctypes.data,
+                                    len(masks), int(symm), keys, kinds, values, texts, ctypes.byref(n), dt,
+                                    err, 512)
+    if rc:
+        raise RuntimeError(err.value.decode(errors="replace"))
+    return _result(keys, kinds, values, texts, n, dt)
+
+
+class Reader:
+    """One ClassicBookReader kept across calls (the reference's stateful
+    symmetry order: the op that answered last is tried first)."""
+
+    def __init__(self, name, masks, symm):
+        m = np.ascontiguousarray(masks if len(masks) else [0], dtype=np.uint64)
+        self._h = lib().egtb_ref_reader_new(name.encode(), m.ctypes.data, len(masks), int(symm))
+        if not self._h:
+            raise RuntimeError("reader construction failed")
+
+    def move_on_dic(self, board, folder, dtype, pattern_full, nums_adjust):
+        b = _board16(board)
+        keys, kinds, values, texts, n, dt, err = _buffers()
+        rc = lib().egtb_ref_reader_move_on_dic(self._h, b.ctypes.data, folder.encode(), dtype.encode(),
+                                               pattern_full.encode(), int(nums_adjust), keys, kinds, values,
+                                               texts, ctypes.byref(n), dt, err, 512)
+        if rc:
+            raise RuntimeError(err.value.decode(errors="replace"))
+        return _result(keys, kinds, values, texts, n, dt)
+
+    def close(self):
+        if self._h:
+            lib().egtb_ref_reader_free(self._h)
+            self._h = None
+
+
+def find_classic_value(pathname, filename, key, dtype):
+    """find_classic_value_native: (value, found)."""
+    v = ctypes.c_double()
+    f = ctypes.c_int32()
+    if lib().egtb_ref_find_classic_value(pathname.encode(), filename.encode(), int(key), dtype.encode(),
+                                         ctypes.byref(v), ctypes.byref(f)):
+        raise RuntimeError("find_classic_value_native raised")
+    return v.value, bool(f.value)

@@ -20,14 +20,20 @@
 #include "BookSolver.h"
 #include "FormationRuntime.h"
 
-extern "C" int egtb_ref_build(const uint64_t *seeds, size_t n_seeds,
-                              const uint64_t *masks, size_t n_masks,
-                              const uint8_t *shifts, size_t n_shifts,
-                              int symm_mode, int target, int steps,
-                              int docheck_step, const char *pathname,
-                              int is_free, double spawn_rate4,
-                              const char *dtype, int num_threads, char *err,
-                              size_t err_len) {
+// deletion_threshold, optimal_branch_only and compress are the RunOptions
+// fields of the same names (FormationRuntime.h:24-42): the post-solve
+// pruning of BookSolver.cpp:589-593, the optimal-branch pass :611-735 and the
+// .z compression of finished books (CompressionBridge.cpp:41-60).
+extern "C" int egtb_ref_build_ex(const uint64_t *seeds, size_t n_seeds,
+                                 const uint64_t *masks, size_t n_masks,
+                                 const uint8_t *shifts, size_t n_shifts,
+                                 int symm_mode, int target, int steps,
+                                 int docheck_step, const char *pathname,
+                                 int is_free, double spawn_rate4,
+                                 const char *dtype, int num_threads,
+                                 double deletion_threshold,
+                                 int optimal_branch_only, int compress,
+                                 char *err, size_t err_len) {
     try {
         PatternSpec spec;
         spec.name = "ref";
@@ -44,10 +50,10 @@ extern "C" int egtb_ref_build(const uint64_t *seeds, size_t n_seeds,
         opt.is_variant = false;
         opt.spawn_rate4 = spawn_rate4;
         opt.success_rate_dtype = dtype;
-        opt.deletion_threshold = 0.0;
-        opt.compress = false;
+        opt.deletion_threshold = deletion_threshold;
+        opt.compress = compress != 0;
         opt.compress_temp_files = false;
-        opt.optimal_branch_only = false;
+        opt.optimal_branch_only = optimal_branch_only != 0;
         opt.chunked_solve = false;
         opt.num_threads = num_threads;
         opt.direct_io = false;
@@ -68,4 +74,18 @@ extern "C" int egtb_ref_build(const uint64_t *seeds, size_t n_seeds,
         }
         return -2;
     }
+}
+
+extern "C" int egtb_ref_build(const uint64_t *seeds, size_t n_seeds,
+                              const uint64_t *masks, size_t n_masks,
+                              const uint8_t *shifts, size_t n_shifts,
+                              int symm_mode, int target, int steps,
+                              int docheck_step, const char *pathname,
+                              int is_free, double spawn_rate4,
+                              const char *dtype, int num_threads, char *err,
+                              size_t err_len) {
+    return egtb_ref_build_ex(seeds, n_seeds, masks, n_masks, shifts, n_shifts,
+                             symm_mode, target, steps, docheck_step, pathname,
+                             is_free, spawn_rate4, dtype, num_threads, 0.0, 0,
+                             0, err, err_len);
 }

@@ -38,7 +38,21 @@ def load_golden(tag):
 
 
 def golden_tags():
-    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))
+    """Builder goldens (per-layer book digests); reader fixtures are reader_*."""
+    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json") and not f.startswith("reader_"))
+
+
+def reader_tags():
+    return sorted(f[7:-5] for f in os.listdir(GOLDEN) if f.startswith("reader_") and f.endswith(".json"))
+
+
+def load_reader_fixture(tag):
+    """Reader fixture: cases of (4x4 tile values, expected (dict, dtype))."""
+    with open(os.path.join(GOLDEN, f"reader_{tag}.json")) as fh:
+        fx = json.load(fh)
+    fx["expected"] = [(np.array(b, dtype=np.int64).reshape(4, 4), (dict((k, v) for k, v in res), dt))
+                      for b, res, dt in fx["cases"]]
+    return fx
 
 
 def book_bytes(keys, rates):

@@ -1,6 +1,8 @@
-"""GPU reader / lookup parity: BookReader.move_on_dic over HBM tables
-against the CPU restatement of the reference reader (on-disk binary
-search), the fallback against its definition, and table sources agreeing."""
+"""GPU reader / lookup parity: BookReader.move_on_dic and batched table
+lookups over HBM tables against the reference's own compiled reader (the
+fixtures in tests/golden/reader_*.json) and against the CPU restatement of
+that reader (on-disk binary search, itself pinned to the same fixtures);
+the fallback against its definition, and table sources agreeing."""
 import importlib
 import math
 import os
@@ -8,6 +10,7 @@ import os
 import numpy as np
 import pytest
 
+from conftest import load_reader_fixture, reader_tags
 from oracle import oracle as orc
 from oracle import reader_oracle as ro
 
@@ -199,3 +202,64 @@ def test_move_on_dic_uint32_tables(pkg, built_u32):
     for k in keys[:: max(1, keys.size // 50)].tolist():
         want = ro.find_in_file(prefix + "40.book", k, *ro.REC_FORMAT["uint32"])
         assert pkg.find_classic_value(folder, os.path.basename(prefix) + "40.book", k, "uint32") == want
+
+
+# ---- against the reference's own reader (tests/golden/reader_*.json) --------
+
+PREFER_MAX_NAMES = ("4442ff", "4442f", "4tiler")
+
+
+def same_answers(fx, got):
+    bad = []
+    for (m, want), g in zip(fx["expected"], got):
+        if g != want or list(g[0].keys()) != list(want[0].keys()):
+            bad.append((m.tolist(), g, want))
+    return bad
+
+
+@pytest.mark.parametrize("tag", reader_tags())
+def test_lookup_matches_reference_reader_fixture(pkg, tag):
+    """Every fixture position through k_lookup on a table built on the GPU:
+    the same ordered dict and dtype as the reference's reader gave."""
+    fx = load_reader_fixture(tag)
+    print(f"reader fixture {tag}: {fx['pattern']}@{2 ** fx['target']} {fx['dtype']}", flush=True)
+    res = pkg.build_in_memory(fx["pattern"], fx["target"], success_rate_dtype=fx["dtype"])
+    t = pkg.Table.from_result(res, fx["pattern"], fx["target"])
+    try:
+        if fx["reader_name"] in PREFER_MAX_NAMES:
+            t.set_prefer_max(True)
+        reader_mod = importlib.import_module("2048endgametablebase_amd.BookReader")
+        got = t.move_on_dic_batch([reader_mod.encode_board(m) for m, _ in fx["expected"]])
+        bad = same_answers(fx, got)
+        assert not bad, (len(bad), bad[:3])
+    finally:
+        t.close()
+        res.close()
+
+
+def test_book_reader_matches_reference_reader_fixture(pkg, built):
+    """BookReader.move_on_dic over the files start_build wrote (float32)."""
+    fx = load_reader_fixture("442_128")
+    got = [pkg.BookReader.move_on_dic(m, PATTERN, str(2 ** TARGET), fx["pattern_full"]) for m, _ in fx["expected"]]
+    bad = same_answers(fx, got)
+    assert not bad, (len(bad), bad[:3])
+
+
+def test_book_reader_uint32_matches_reference_reader_fixture(pkg, built_u32):
+    """The same over the default configuration's uint32 files."""
+    folder, prefix = built_u32
+    fx = load_reader_fixture("442_128_uint32")
+    cfg = pkg.config.SingletonConfig().config
+    key = (fx["pattern_full"], cfg["4_spawn_rate"])
+    old = cfg["filepath_map"].get(key)
+    cfg["filepath_map"][key] = [(folder, "uint32")]
+    try:
+        got = [pkg.BookReader.move_on_dic(m, PATTERN, str(2 ** TARGET), fx["pattern_full"])
+               for m, _ in fx["expected"]]
+    finally:
+        if old is None:
+            cfg["filepath_map"].pop(key, None)
+        else:
+            cfg["filepath_map"][key] = old
+    bad = same_answers(fx, got)
+    assert not bad, (len(bad), bad[:3])

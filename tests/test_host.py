@@ -88,3 +88,20 @@ def test_key_set_sizing_and_forward_query_check_arguments(pkg):
     assert L.egtb_set_key_set_sizing(0.0, 0) == 0  # back to the defaults
     a, b = ctypes.c_uint64(), ctypes.c_uint64()
     assert L.egtb_result_forward(None, ctypes.byref(a), ctypes.byref(b)) != 0
+
+
+def test_every_sort_state_error_flag_is_read():
+    """run_build keeps one sort state per stream, each with its own
+    look-back error flag; every one of them must be read before the build
+    returns, or a sort that hit its spin limit would pass unnoticed."""
+    import re
+
+    src = open(os.path.join(os.path.dirname(__file__), "..", "2048endgametablebase_amd", "csrc",
+                            "egtb_runtime.hip")).read()
+    start = src.index("void run_build(")
+    end = src.index("\nvoid build_dispatch(", start)
+    body = src[start:end]
+    states = re.findall(r"\bSortWork\s+(\w+)\s*;", body)
+    assert len(states) >= 2, states
+    for name in states:
+        assert f"{name}.read_error(" in body, name
