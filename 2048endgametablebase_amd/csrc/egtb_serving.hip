@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -355,6 +357,46 @@ std::unique_ptr<TableSet> table_from_result(const Result &r, const egtb_pattern 
     return ts;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free lookups: staging in a file-scope device array
+// ---------------------------------------------------------------------------
+
+// The staging of egtb_lookup_batch_noalloc: boards (8 B), four float rates
+// (16 B) and a status word (4 B) per query, for at most kNoAllocMaxBoards
+// queries (28 MiB).  Every GPU the library runs on holds its own copy until
+// the process ends.  In host code the array's name is not a device address:
+// kernels and copies use the address hipGetSymbolAddress gives per device.
+constexpr size_t kNoAllocMaxBoards = (size_t)1 << 20;
+constexpr size_t kNoAllocBoardsOff = 0;
+constexpr size_t kNoAllocOutOff = kNoAllocMaxBoards * 8;
+constexpr size_t kNoAllocStatusOff = kNoAllocOutOff + kNoAllocMaxBoards * 16;
+constexpr size_t kNoAllocBytes = kNoAllocStatusOff + kNoAllocMaxBoards * 4;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+struct NoAllocStage {
+    std::mutex mu;            // calls on one device run one at a time
+    unsigned char *ptr = nullptr;  // device address of `scratch` on this device
+};
+static std::mutex g_noalloc_mu;
+static std::map<int, std::unique_ptr<NoAllocStage>> g_noalloc;
+
+// Set-up, once per device, with that device current: look up the array's
+// device address.  Allocates nothing on the device.
+static NoAllocStage &noalloc_stage(int dev) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    auto it = g_noalloc.find(dev);
+    if (it != g_noalloc.end()) return *it->second;
+    HIPCHK(hipSetDevice(dev));
+    auto st = std::make_unique<NoAllocStage>();
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(EGTB_ERR_HIP, "hipGetSymbolAddress gave no address for the lookup staging");
+    st->ptr = (unsigned char *)ptr;
+    NoAllocStage &ref = *st;
+    g_noalloc[dev] = std::move(st);
+    return ref;
+}
+
 }  // namespace egtb
 
 // ===========================================================================
@@ -429,6 +471,32 @@ int egtb_lookup_batch(egtb_table *t, const uint64_t *boards, size_t n, float *ou
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out4, o.p, n * 16, hipMemcpyDeviceToHost, s));
         if (status) HIPCHK(hipMemcpyAsync(status, st.p, n * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int egtb_lookup_batch_noalloc(egtb_table *t, const uint64_t *boards, size_t n, float *out4, int32_t *status) {
+    return guarded_s([&] {
+        if (!t) throw Error(EGTB_ERR_ARG, "table is NULL");
+        if (n == 0) return;
+        if (!boards || !out4) throw Error(EGTB_ERR_ARG, "boards and out4 must not be NULL");
+        if (n > kNoAllocMaxBoards)
+            throw Error(EGTB_ERR_ARG, "egtb_lookup_batch_noalloc takes at most " + std::to_string(kNoAllocMaxBoards) +
+                                          " boards per call (got " + std::to_string(n) + ")");
+        DeviceCtx &C = device_ctx(t->dev);  // the table made it: no allocation here
+        NoAllocStage &st = noalloc_stage(t->dev);
+        std::lock_guard<std::mutex> lk(st.mu);
+        hipStream_t s = C.stream;
+        uint64_t *d_boards = (uint64_t *)(st.ptr + kNoAllocBoardsOff);
+        float *d_out = (float *)(st.ptr + kNoAllocOutOff);
+        int32_t *d_status = (int32_t *)(st.ptr + kNoAllocStatusOff);
+        // every query's four rates and status are written by the kernel, so
+        // nothing left by the previous call needs clearing
+        HIPCHK(hipMemcpyAsync(d_boards, boards, n * 8, hipMemcpyHostToDevice, s));
+        launch_lookup<float>(s, *t, d_boards, n, d_out, d_status, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out4, d_out, n * 16, hipMemcpyDeviceToHost, s));
+        if (status) HIPCHK(hipMemcpyAsync(status, d_status, n * 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     });
 }

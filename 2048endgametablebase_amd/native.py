@@ -147,6 +147,8 @@ SIGNATURES = {
     "egtb_table_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), u64p, u64p]),
     "egtb_lookup_batch": (ctypes.c_int, [ctypes.c_void_p, u64p, ctypes.c_size_t, f32p,
                                          ctypes.POINTER(ctypes.c_int32)]),
+    "egtb_lookup_batch_noalloc": (ctypes.c_int, [ctypes.c_void_p, u64p, ctypes.c_size_t, f32p,
+                                                 ctypes.POINTER(ctypes.c_int32)]),
     "egtb_lookup_batch_f64": (ctypes.c_int, [ctypes.c_void_p, u64p, ctypes.c_size_t,
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32)]),

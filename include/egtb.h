@@ -202,6 +202,16 @@ EGTB_EXPORT int egtb_table_info(const egtb_table *t, int *steps, uint64_t *recor
  * (ReaderRuntime.cpp:739-840) with a stateless symmetry order 0..7.
  * status[i] is EGTB_LOOKUP_OK or EGTB_LOOKUP_FALLBACK.  Host buffers. */
 EGTB_EXPORT int egtb_lookup_batch(egtb_table *t, const uint64_t *boards, size_t n, float *out4, int32_t *status);
+/* As egtb_lookup_batch, but allocates nothing: no hipMalloc, hipFree or
+ * allocating library call.  No counterpart in the reference; for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid it).  The device staging of boards, rates and status lives in a
+ * file-scope device array of 28 MiB per GPU, held until the process ends:
+ * at most 2^20 boards per call (more: EGTB_ERR_ARG).  Calls on one GPU share
+ * that array and would race; the library serialises them with a per-device
+ * lock.  Same answers as egtb_lookup_batch.  Host buffers. */
+EGTB_EXPORT int egtb_lookup_batch_noalloc(egtb_table *t, const uint64_t *boards, size_t n, float *out4,
+                                          int32_t *status);
 /* Same in float64, also returning the symmetry op that answered (-1 none). */
 EGTB_EXPORT int egtb_lookup_batch_f64(egtb_table *t, const uint64_t *boards, size_t n, double *out4,
                                       int32_t *status, int32_t *op_used);
