@@ -522,10 +522,10 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
             const Book &b2 = res.books[i + 2];
             t2 = TableDev<float>{b2.keys, (const float *)b2.rates, b2.idx, b2.n_dev, 0, b2.bits, 1};
         }
-        HashDev h1{nullptr, 0}, h2{nullptr, 0};
+        HashDev h1{}, h2{};
         if (i + 2 < max_steps) {
-            h1 = HashDev{res.books[i + 1].hslots, res.books[i + 1].hmask};
-            h2 = HashDev{res.books[i + 2].hslots, res.books[i + 2].hmask};
+            h1 = res.books[i + 1].h;
+            h2 = res.books[i + 2].h;
         }
         if (n) {
             Timer t(s, "batch_solve", (double)n * 12.0);
@@ -550,10 +550,13 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
                 hs = (ulonglong2 *)hring[i % 3].p;
             }
         }
-        bk.hslots = hs;
-        bk.hmask = hs ? hcap - 1 : 0;
+        bk.h = HashDev{};
+        if (hs) {  // rotated keys span every job: the wide layout
+            bk.h.slots = hs;
+            bk.h.mask = hcap - 1;
+        }
         launch_compact<float>(s, res.raw[i], (const float *)rates.p, n, bk.keys, (float *)bk.rates,
-                              (unsigned long long *)&M[i].maxbits, &M[i].n_live, w.lw, 0.0f, hs, hs ? hcap : 0);
+                              (unsigned long long *)&M[i].maxbits, &M[i].n_live, w.lw, 0.0f, bk.h);
         bk.bits = bindex_bits(n);
         bk.idx = (uint32_t *)res.book_arena.alloc(((1ull << bk.bits) + 1) * 4);
         launch_bucket_index(s, bk.keys, n, bk.n_dev, bk.bits, bk.idx);
@@ -562,10 +565,7 @@ static void batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch
                                n_jobs, (uint64_t *)br.bounds.p + (size_t)i * (n_jobs + 1));
         HIPCHK(hipGetLastError());
     }
-    for (int i = 0; i < max_steps; ++i) {  // the ring is freed with this scope
-        res.books[i].hslots = nullptr;
-        res.books[i].hmask = 0;
-    }
+    for (int i = 0; i < max_steps; ++i) res.books[i].h = HashDev{};  // the ring is freed with this scope
     uint32_t err = 0;
     HIPCHK(hipMemcpyAsync(&err, errb.p, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(H, M, (size_t)max_steps * sizeof(LayerDev), hipMemcpyDeviceToHost, s));

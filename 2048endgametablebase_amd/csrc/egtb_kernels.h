@@ -41,16 +41,67 @@ __device__ __forceinline__ T table_lookup_dev(const TableDev<T> &t, uint64_t key
     return table_lookup_n(t, t.n, key);
 }
 
-// Open-addressing hash of one solved layer, the structure the backward solve
-// looks successors up in: 16-byte slots {key, rate bits}, key 0 marks an
-// empty slot (every stored board holds tiles, so no key is 0), linear
-// probing from hash_mix(key).  The capacity is a power of two at least four
-// times the layer's size bound, so every probe run ends at an empty slot; a
-// lookup usually reads one cache line.
+// Hash of one solved layer, the structure the backward solve looks
+// successors up in.  Two layouts:
+//
+// wide (slots): 16-byte slots {key, rate bits}, key 0 marks an empty slot
+//   (every stored board holds tiles, so no key is 0), linear probing from
+//   hash_mix(key); the capacity is a power of two at least four times the
+//   layer's size bound, so every probe run ends at an empty slot.
+//
+// compact (cslots, 4-byte rates only): 8-byte slots in buckets of 8 (64
+//   bytes).  A layer's keys agree outside one span of s bits (the varying
+//   bits of the layer, from the OR / AND of its keys), so a key is its span
+//   value x plus the constant bits; a key with other constant bits is not in
+//   the layer.  x goes through a bijective mix on s bits; the top cb bits
+//   pick the home bucket and the low s - cb <= 26 bits are stored.  A slot
+//   holds [valid:1][displacement:5][stored:26][rate bits:32], 0 = empty; a
+//   record sits in bucket home + d (d < 32) with d in its slot, so (bucket,
+//   d, stored) names the key exactly.  Buckets fill front to back and slots
+//   only go from empty to full, so a lookup stops at the first empty slot.
+//   Tables are 4-8x smaller than the wide layout, which keeps the solve's
+//   successor tables in the caches.
 struct HashDev {
-    const ulonglong2 *slots;
-    uint64_t mask;  // capacity - 1
+    const ulonglong2 *slots = nullptr;  // wide layout
+    uint64_t mask = 0;                  // wide: capacity - 1; compact: buckets - 1
+    const uint64_t *cslots = nullptr;   // compact layout
+    uint64_t cconst = 0;                // compact: the key bits outside the span
+    uint64_t cspan = 0;                 // compact: 2^s - 1
+    uint32_t clo = 0, cs = 0, cb = 0;   // compact: span shift, span bits s, bucket bits
+    uint32_t *err = nullptr;            // fills: bit 0 set when a compact insert found no slot
+    __host__ __device__ bool present() const { return slots != nullptr || cslots != nullptr; }
+    __host__ __device__ uint64_t bytes() const {
+        return cslots ? (mask + 1) * 64 : slots ? (mask + 1) * 16 : 0;
+    }
+    __host__ __device__ void *table() const { return cslots ? (void *)cslots : (void *)slots; }
 };
+
+constexpr uint32_t kCompactStoredBits = 26;
+constexpr uint32_t kCompactMaxDisp = 32;
+
+// Home bucket and stored bits of `key` in a compact hash; false when the
+// key's constant bits differ from the layer's (then it is not in the layer).
+__device__ __forceinline__ bool chash_split(const HashDev &h, uint64_t key, uint64_t &bucket, uint32_t &stored) {
+    if ((key & ~(h.cspan << h.clo)) != h.cconst) return false;
+    uint64_t x = (key >> h.clo) & h.cspan;
+    if (h.cs) {
+        // odd multipliers and right xorshifts are bijections on s bits
+        const uint32_t r = (h.cs + 1) >> 1;
+        x = (x * 0x9e3779b97f4a7c15ull) & h.cspan;
+        x ^= x >> r;
+        x = (x * 0xbf58476d1ce4e5b9ull) & h.cspan;
+        x ^= x >> r;
+    }
+    const uint32_t sb = h.cs - h.cb;
+    bucket = x >> sb;
+    stored = (uint32_t)(x & ((1ull << sb) - 1));
+    return true;
+}
+
+// the upper word of a compact slot holding (d, stored)
+__device__ __forceinline__ uint32_t chash_tag(uint32_t d, uint32_t stored) {
+    return 0x80000000u | (d << kCompactStoredBits) | stored;
+}
 
 // murmur3's 64-bit finaliser: board keys share most of their bits
 __device__ __forceinline__ uint64_t hash_mix(uint64_t x) {
@@ -69,10 +120,11 @@ __device__ __forceinline__ T rate_from_bits(uint64_t v) {
 }
 
 uint64_t hash_capacity(uint64_t bound);
-// Insert records [0, n) of a layer (n = *n_ptr when non-null, else bound).
+// Insert records [0, n) of a layer (n = *n_ptr when non-null, else bound)
+// into a cleared-here table.
 template <typename T>
 void launch_hash_build(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t bound, const uint64_t *n_ptr,
-                       ulonglong2 *slots, uint64_t capacity);
+                       const HashDev &h);
 
 // Sample-sort dedup (egtb_bucket.hip): sorted unique keys of up to
 // kBucketMaxKeys candidates in six launches.  BucketWork holds its device
@@ -179,15 +231,14 @@ void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, 
                          uint32_t bits, uint32_t *idx);
 
 template <typename T>
-// clear / clear_n: a hash table the same launch zeroes (a later layer's, whose
+// clear: a hash table the same launch zeroes (a later layer's, whose
 // slots no probe of this launch reads), so its fill needs no launch of its own.
-// own / own_cap: this layer's own solve hash (cleared before the launch),
-// which gets every state with a rate above zero.
+// own: this layer's own solve hash (cleared before the launch), which gets
+// every state with a rate above zero.
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
                   const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
-                  ulonglong2 *clear = nullptr, uint64_t clear_n = 0, ulonglong2 *own = nullptr,
-                  uint64_t own_cap = 0);
+                  const HashDev &clear = HashDev{}, const HashDev &own = HashDev{});
 
 template <typename T>
 void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,
@@ -195,14 +246,13 @@ void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const P
 
 // Records with rate > threshold, in order, in one look-back launch; it
 // shares the unique kernel's status words (ensure_lookback with
-// compact_tiles(n) unique tiles first). With hslots (hcap slots, a power
-// of two), the table is cleared and the kept records hashed into it too.
+// compact_tiles(n) unique tiles first). With a hash `fill`, the table is
+// cleared (unless `zeroed`) and the kept records hashed into it too.
 uint64_t compact_tiles(uint64_t n);
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
                     T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw,
-                    T threshold = T(0), ulonglong2 *hslots = nullptr, uint64_t hcap = 0,
-                    bool hslots_zeroed = false);
+                    T threshold = T(0), const HashDev &fill = HashDev{}, bool zeroed = false);
 
 template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);

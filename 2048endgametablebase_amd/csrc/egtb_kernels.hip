@@ -1033,12 +1033,37 @@ __device__ __forceinline__ uint64_t rate_bits(T v) {
     else return __builtin_bit_cast(uint64_t, v);
 }
 
-// Record (k, r) in a layer's solve hash (keys of a layer are unique and the
-// table is at most a quarter full, so the claim ends at an empty slot).
+// Record (k, r) in a layer's solve hash.  Wide: keys of a layer are unique
+// and the table is at most a quarter full, so the claim ends at an empty
+// slot.  Compact: the first empty slot of the home bucket or of the next 31
+// (one 64-bit CAS writes key and rate together); none free sets the error
+// flag.
 template <typename T>
-__device__ __forceinline__ void hash_put(ulonglong2 *__restrict__ slots, uint64_t mask, uint64_t k, T r) {
-    uint64_t sl = hash_mix(k) & mask;
-    while (atomicCAS((unsigned long long *)&slots[sl].x, 0ull, (unsigned long long)k) != 0ull) sl = (sl + 1) & mask;
+__device__ __forceinline__ void hash_insert(const HashDev &h, uint64_t k, T r) {
+    if (h.cslots) {
+        if constexpr (sizeof(T) == 4) {
+            uint64_t bk;
+            uint32_t st;
+            if (!chash_split(h, k, bk, st)) {  // a key outside its layer's span: cannot happen
+                if (h.err) atomicOr(h.err, 2u);
+                return;
+            }
+            unsigned long long *sl = (unsigned long long *)h.cslots;
+            for (uint32_t d = 0; d < kCompactMaxDisp; ++d) {
+                const uint64_t base = ((bk + d) & h.mask) * 8;
+                const unsigned long long v = ((unsigned long long)chash_tag(d, st) << 32) | rate_bits(r);
+                for (int j = 0; j < 8; ++j) {
+                    if (__hip_atomic_load(&sl[base + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull) continue;
+                    if (atomicCAS(&sl[base + j], 0ull, v) == 0ull) return;
+                }
+            }
+            if (h.err) atomicOr(h.err, 1u);
+        }
+        return;
+    }
+    ulonglong2 *slots = (ulonglong2 *)h.slots;
+    uint64_t sl = hash_mix(k) & h.mask;
+    while (atomicCAS((unsigned long long *)&slots[sl].x, 0ull, (unsigned long long)k) != 0ull) sl = (sl + 1) & h.mask;
     slots[sl].y = rate_bits(r);
 }
 
@@ -1048,7 +1073,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                                                 int do_check, HashDev h1, HashDev h2, TableDev<T> t1,
                                                 TableDev<T> t2, double w2, double w4, T max_scale,
                                                 T *__restrict__ rates, ulonglong2 *__restrict__ clear,
-                                                uint64_t clear_n, ulonglong2 *__restrict__ own, uint64_t own_mask) {
+                                                uint64_t clear_n, HashDev own) {
     extern __shared__ uint16_t lut[];
     // the table filled after this launch: disjoint from h1 / h2
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < clear_n;
@@ -1057,7 +1082,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
     stage_lut(lut, glut);
     // hash probes when both successor layers have a hash, else binary
     // searches narrowed by the bucket index (uniform over the launch)
-    const bool use_hash = h1.slots != nullptr && h2.slots != nullptr;
+    const bool use_hash = h1.present() && h2.present();
     // (the sizes of the successor books are read only by the searches: with
     // hashes their compaction may still be running on another stream)
     const uint64_t n1 = use_hash ? 0 : (t1.n_ptr ? *t1.n_ptr : t1.n);
@@ -1067,7 +1092,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
         const uint64_t b = keys[i];
         if (do_check && is_success(b, p)) {
             rates[i] = max_scale;
-            if (own) hash_put(own, own_mask, b, max_scale);
+            if (own.present()) hash_insert(own, b, max_scale);
             continue;
         }
         double sp = 0.0;
@@ -1132,23 +1157,60 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                 }
                 active = 0;
             }
+            // wide: slot[j] is the probed slot; compact: slot[j] is the home
+            // bucket, step[j] = 4 * displacement + pair within the bucket and
+            // want[j] the stored bits
+            uint32_t step[8], want[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) slot[j] = hash_mix(key[j]) & (j < 4 ? h1.mask : h2.mask);
+            for (int j = 0; j < 8; ++j) {
+                const HashDev &h = j < 4 ? h1 : h2;
+                step[j] = 0;
+                want[j] = 0;
+                if (h.cslots) {
+                    if (((active >> j) & 1u) && !chash_split(h, key[j], slot[j], want[j]))
+                        active &= ~(1u << j);  // constant bits differ: not in the layer
+                } else {
+                    slot[j] = hash_mix(key[j]) & h.mask;
+                }
+            }
             while (active) {
                 ulonglong2 ent[8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if ((active >> j) & 1u) ent[j] = (j < 4 ? h1.slots : h2.slots)[slot[j]];
+                for (int j = 0; j < 8; ++j) {
+                    if ((active >> j) & 1u) {
+                        const HashDev &h = j < 4 ? h1 : h2;
+                        ent[j] = h.cslots ? ((const ulonglong2 *)h.cslots)[((slot[j] + (step[j] >> 2)) & h.mask) * 4 +
+                                                                           (step[j] & 3)]
+                                          : h.slots[slot[j]];
+                    }
+                }
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     if ((active >> j) & 1u) {
-                        if (ent[j].x == key[j]) {
+                        const HashDev &h = j < 4 ? h1 : h2;
+                        if (h.cslots) {
+                            if constexpr (sizeof(T) == 4) {
+                                // two slots a round; an empty slot ends the run
+                                const uint32_t tag = chash_tag(step[j] >> 2, want[j]);
+                                if ((uint32_t)(ent[j].x >> 32) == tag) {
+                                    r[j] = rate_from_bits<T>(ent[j].x & 0xffffffffull);
+                                    active &= ~(1u << j);
+                                } else if (ent[j].x == 0) {
+                                    active &= ~(1u << j);
+                                } else if ((uint32_t)(ent[j].y >> 32) == tag) {
+                                    r[j] = rate_from_bits<T>(ent[j].y & 0xffffffffull);
+                                    active &= ~(1u << j);
+                                } else if (ent[j].y == 0 || ++step[j] >= 4 * kCompactMaxDisp) {
+                                    active &= ~(1u << j);
+                                }
+                            }
+                        } else if (ent[j].x == key[j]) {
                             r[j] = rate_from_bits<T>(ent[j].y);
                             active &= ~(1u << j);
                         } else if (ent[j].x == 0) {
                             active &= ~(1u << j);  // absent: rate 0
                         } else {
-                            slot[j] = (slot[j] + 1) & (j < 4 ? h1.mask : h2.mask);
+                            slot[j] = (slot[j] + 1) & h.mask;
                         }
                     }
                 }
@@ -1161,25 +1223,19 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
         }
         const T rv = e > 0 ? (T)(sp / (double)e) : (T)0;
         rates[i] = rv;
-        if (own && rv > (T)0) hash_put(own, own_mask, b, rv);
+        if (own.present() && rv > (T)0) hash_insert(own, b, rv);
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_hash_build(const uint64_t *__restrict__ keys, const T *__restrict__ rates,
-                                                    uint64_t bound, const uint64_t *__restrict__ n_ptr,
-                                                    ulonglong2 *__restrict__ slots, uint64_t mask) {
+                                                    uint64_t bound, const uint64_t *__restrict__ n_ptr, HashDev h) {
     const uint64_t n = n_ptr ? *n_ptr : bound;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t k = keys[i];
         if (k == 0) continue;
-        uint64_t sl = hash_mix(k) & mask;
-        // keys of a layer are unique, so the claim succeeds at the first
-        // empty slot of the run; the table is at most a quarter full
-        while (atomicCAS((unsigned long long *)&slots[sl].x, 0ull, (unsigned long long)k) != 0ull)
-            sl = (sl + 1) & mask;
-        slots[sl].y = rate_bits(rates[i]);
+        hash_insert(h, k, rates[i]);
     }
 }
 
@@ -1217,8 +1273,7 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
                                                              uint64_t *__restrict__ status,
                                                              unsigned long long *__restrict__ tile_counter,
                                                              unsigned long long tile_base, uint32_t epoch,
-                                                             uint32_t *__restrict__ err,
-                                                             ulonglong2 *__restrict__ hslots, uint64_t hmask) {
+                                                             uint32_t *__restrict__ err, HashDev fill) {
     __shared__ uint64_t scratch[kCompThreads / 64];
     __shared__ uint32_t s_tile;
     __shared__ uint64_t s_off;
@@ -1259,12 +1314,7 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
             out_keys[pos] = k;
             out_rates[pos] = r[j];
             ++pos;
-            if (hslots) {
-                uint64_t sl = hash_mix(k) & hmask;
-                while (atomicCAS((unsigned long long *)&hslots[sl].x, 0ull, (unsigned long long)k) != 0ull)
-                    sl = (sl + 1) & hmask;
-                hslots[sl].y = rate_bits(r[j]);
-            }
+            if (fill.present()) hash_insert(fill, k, r[j]);
         }
     }
     // non-negative IEEE floats and unsigned ints both order as their bits
@@ -1528,16 +1578,17 @@ template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
                   const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
-                  ulonglong2 *clear, uint64_t clear_n, ulonglong2 *own, uint64_t own_cap) {
+                  const HashDev &clear, const HashDev &own) {
+    const uint64_t clear_n = clear.bytes() / sizeof(ulonglong2);
     if (n == 0) {
-        if (clear_n) (void)hipMemsetAsync(clear, 0, clear_n * sizeof(ulonglong2), s);
+        if (clear_n) (void)hipMemsetAsync(clear.table(), 0, clear.bytes(), s);
         return;
     }
     const uint32_t want = blocks_for(n > clear_n ? n : clear_n, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     hipLaunchKernelGGL((k_solve<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n,
-                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates, clear, clear_n, own,
-                       own ? own_cap - 1 : 0ull);
+                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates, (ulonglong2 *)clear.table(),
+                       clear_n, own);
 }
 
 // Slots for a layer of at most `bound` records: a power of two, at most a
@@ -1553,12 +1604,11 @@ uint64_t hash_capacity(uint64_t bound) {
 
 template <typename T>
 void launch_hash_build(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t bound, const uint64_t *n_ptr,
-                       ulonglong2 *slots, uint64_t capacity) {
-    (void)hipMemsetAsync(slots, 0, capacity * sizeof(ulonglong2), s);
+                       const HashDev &h) {
+    (void)hipMemsetAsync(h.table(), 0, h.bytes(), s);
     if (bound == 0) return;
     const uint32_t grid = blocks_for(bound, 256) < 8192 ? blocks_for(bound, 256) : 8192;
-    hipLaunchKernelGGL((k_hash_build<T>), dim3(grid), dim3(256), 0, s, keys, rates, bound, n_ptr, slots,
-                       capacity - 1);
+    hipLaunchKernelGGL((k_hash_build<T>), dim3(grid), dim3(256), 0, s, keys, rates, bound, n_ptr, h);
 }
 
 template <typename T>
@@ -1574,8 +1624,8 @@ uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
                     T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw, T threshold,
-                    ulonglong2 *hslots, uint64_t hcap, bool hslots_zeroed) {
-    if (hslots && !hslots_zeroed) (void)hipMemsetAsync(hslots, 0, hcap * sizeof(ulonglong2), s);
+                    const HashDev &fill, bool zeroed) {
+    if (fill.present() && !zeroed) (void)hipMemsetAsync(fill.table(), 0, fill.bytes(), s);
     if (n == 0) {
         if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
         return;
@@ -1584,7 +1634,7 @@ void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_
     const uint32_t epoch = lw.next_epoch();
     hipLaunchKernelGGL((k_compact_lb<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, threshold,
                        out_keys, out_rates, max_bits, count_out, lw.status_uniq, lw.counters + 1, lw.claimed_uniq,
-                       epoch, lw.err, hslots, hslots ? hcap - 1 : 0ull);
+                       epoch, lw.err, fill);
     lw.claimed_uniq += tiles;
 }
 
@@ -1613,13 +1663,13 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
     template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
                                   const uint16_t *, const PatternDev &, int, const HashDev &,                 \
                                   const HashDev &, const TableDev<T> &, const TableDev<T> &, double, double,  \
-                                  T, T *, ulonglong2 *, uint64_t, ulonglong2 *, uint64_t);                    \
+                                  T, T *, const HashDev &, const HashDev &);                                  \
     template void launch_hash_build<T>(hipStream_t, const uint64_t *, const T *, uint64_t, const uint64_t *,  \
-                                       ulonglong2 *, uint64_t);                                               \
+                                       const HashDev &);                                                      \
     template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *, T *,      \
-                                    unsigned long long *, uint64_t *, LookbackWork &, T, ulonglong2 *,        \
-                                    uint64_t, bool);                                                                \
+                                    unsigned long long *, uint64_t *, LookbackWork &, T, const HashDev &,     \
+                                    bool);                                                                    \
     template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
 
 EGTB_INSTANTIATE(float)

@@ -182,12 +182,14 @@ struct Book {
     uint64_t *n_dev = nullptr;  // &LayerDev::n_live
     uint64_t n = 0;             // host copy, valid once the build finished
     uint32_t bits = 1;
-    const ulonglong2 *hslots = nullptr;  // solve-side hash (HashDev) while the layer has readers
-    uint64_t hmask = 0;
+    HashDev h{};  // solve-side hash while the layer has readers
 };
 
 struct LayerInfo {
     uint64_t n_raw = 0, n_live = 0, c2 = 0, c4 = 0;
+    // OR / AND of the layer's keys (from its forward step); the defaults
+    // mean "unknown" and give the full 64-bit span
+    uint64_t key_or = ~0ull, key_and = 0;
 };
 
 // Host memory that streamed books are copied into: anonymous pages mapped
@@ -235,6 +237,8 @@ struct Result {
     std::vector<DBuf> hash_bufs;  // ring of solve hashes: layer i's in slot i % hash_ring
     int hash_ring = 3;
     double hash_max_bytes = 0;    // larger layers are searched, not hashed
+    bool compact_hashes = true;   // compact solve-hash layout where a layer's key span allows
+    DBuf hash_err;                // set by a compact insert that found no slot
     bool book_index = true;       // books get the bucket index of the solve's search path
     std::vector<HostSlab> host_slabs;            // own the memory of host_keys / host_rates
     std::vector<HostView> host_keys, host_rates;  // per layer

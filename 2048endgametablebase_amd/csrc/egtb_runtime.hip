@@ -622,34 +622,82 @@ static uint32_t index_bits_for(uint64_t n) {
 constexpr double kTwoStreamRatesMax = 16.0 * 1024 * 1024 * 1024;
 constexpr int kDefaultSolveHashMB = 1024;
 
-// The table of book `layer`'s solve hash (its slots and mask go on the
-// book; `cap` gets its capacity), or nullptr when the layer is too large to
-// hash and the solve searches it instead.
-static ulonglong2 *hash_table_for(hipStream_t s, Result &res, int layer, uint64_t bound, uint64_t &cap) {
+// Keys per bucket (of 8 slots) at a compact table's size bound: buckets are
+// the power of two at or above bound / 4, so a table is 25-50% full at its
+// bound and less with the zero-rate states left out.  (At 6 keys a bucket,
+// 38-75% full, 444@64's largest layers displaced records by up to 15
+// buckets; at 4, by at most 5.)
+constexpr double kCompactKeysPerBucket = 4.0;
+
+// Layout of book `layer`'s solve hash for at most `bound` records (no table
+// yet): compact when the rates are 4 bytes and the layer's key span leaves at
+// most 26 stored bits at a size no larger than the wide table's, else wide;
+// not present when the table would exceed the hash budget (the solve
+// searches that layer instead).
+static HashDev hash_layout(const Result &res, int layer, uint64_t bound) {
+    HashDev h{};
+    const uint64_t wide_cap = hash_capacity(bound);
+    const double wide_bytes = (double)wide_cap * sizeof(ulonglong2);
+    const LayerInfo &li = res.info[layer];
+    const uint64_t diff = li.key_or ^ li.key_and;
+    if (res.compact_hashes && res.rate_size == 4 && (li.key_and & ~li.key_or) == 0) {
+        const uint32_t lo = diff ? (uint32_t)__builtin_ctzll(diff) : 0;
+        const uint32_t span = diff ? (uint32_t)(64 - __builtin_clzll(diff)) - lo : 0;
+        uint32_t cb = 0;
+        while ((double)(1ull << cb) * kCompactKeysPerBucket < (double)bound && cb < 40) ++cb;
+        if (span > kCompactStoredBits && cb < span - kCompactStoredBits) cb = span - kCompactStoredBits;
+        if (cb > span) cb = span;
+        const double bytes = (double)(1ull << cb) * 64.0;
+        if (span < 64 && span - cb <= kCompactStoredBits && bytes <= std::max(wide_bytes, 65536.0) &&
+            bytes <= res.hash_max_bytes) {
+            h.mask = (1ull << cb) - 1;
+            h.clo = lo;
+            h.cs = span;
+            h.cb = cb;
+            h.cspan = span ? ((1ull << span) - 1) : 0;
+            h.cconst = li.key_and & ~(h.cspan << lo);
+            h.cslots = reinterpret_cast<const uint64_t *>(uintptr_t(1));  // marks the layout; set with the table
+            return h;
+        }
+    }
+    if (wide_bytes > res.hash_max_bytes) return h;
+    h.mask = wide_cap - 1;
+    h.slots = reinterpret_cast<const ulonglong2 *>(uintptr_t(1));
+    return h;
+}
+
+// The table of book `layer`'s solve hash (the hash goes on the book), or a
+// hash that is not present when the layer is too large to hash and the
+// solve searches it instead.
+static HashDev hash_table_for(hipStream_t s, Result &res, int layer, uint64_t bound) {
     Book &bk = res.books[layer];
-    cap = hash_capacity(bound);
-    if ((double)cap * sizeof(ulonglong2) > res.hash_max_bytes) return nullptr;
+    HashDev h = hash_layout(res, layer, bound);
+    if (!h.present()) {
+        bk.h = HashDev{};
+        return bk.h;
+    }
     // with a ring of 3, at most three hashes are alive (layers i, i+1, i+2
     // while layer i is solved): layer i takes slot i % 3, whose previous
     // table (layer i+3) was last read by the solve of layer i+1, queued
     // before. A ring of 4 lets the solve of layer i+1 clear layer i's slot.
-    DBuf &h = res.hash_bufs[layer % res.hash_ring];
-    h.ensure(cap * sizeof(ulonglong2), s);
-    bk.hslots = (const ulonglong2 *)h.p;
-    bk.hmask = cap - 1;
-    return (ulonglong2 *)h.p;
+    DBuf &buf = res.hash_bufs[layer % res.hash_ring];
+    buf.ensure(h.bytes(), s);
+    if (h.cslots) h.cslots = (const uint64_t *)buf.p;
+    else h.slots = (const ulonglong2 *)buf.p;
+    h.err = (uint32_t *)res.hash_err.p;
+    bk.h = h;
+    return h;
 }
 
 // The solve hash of a book that is already in place (one loaded from its
 // file); solved layers get theirs from the compaction that makes them.
 template <typename T>
 static void build_hash(hipStream_t s, Result &res, int layer, uint64_t bound) {
-    uint64_t cap = 0;
-    ulonglong2 *slots = hash_table_for(s, res, layer, bound, cap);
-    if (!slots) return;
+    const HashDev h = hash_table_for(s, res, layer, bound);
+    if (!h.present()) return;
     const Book &bk = res.books[layer];
-    Timer t(s, "hash", (double)bound * (8 + sizeof(T)) + (double)cap * sizeof(ulonglong2));
-    launch_hash_build<T>(s, bk.keys, (const T *)bk.rates, bound, bk.n_dev, slots, cap);
+    Timer t(s, "hash", (double)bound * (8 + sizeof(T)) + (double)h.bytes());
+    launch_hash_build<T>(s, bk.keys, (const T *)bk.rates, bound, bk.n_dev, h);
 }
 
 template <typename T>
@@ -665,17 +713,9 @@ static TableDev<T> table_of(const Book &bk) {
     return t;
 }
 
-static HashDev hash_of(const Book &bk) {
-    HashDev h;
-    h.slots = bk.hslots;
-    h.mask = bk.hmask;
-    return h;
-}
+static HashDev hash_of(const Book &bk) { return bk.h; }
 
-static void drop_hash(Result &res, int layer) {
-    res.books[layer].hslots = nullptr;
-    res.books[layer].hmask = 0;
-}
+static void drop_hash(Result &res, int layer) { res.books[layer].h = HashDev{}; }
 
 // Carve book storage for a layer of at most `bound` live records, compact
 // (keys, rates) into it (stable, rate > 0) and index it.  Sizes stay on the
@@ -706,14 +746,11 @@ static void compact_book(hipStream_t s, Result &res, int layer, const uint64_t *
     // the compaction also fills the layer's solve hash (read by the solves
     // of layers i-1 and i-2; layer 0 is read by none)
     // (with hash_zeroed the table is in place already, cleared by the solve)
-    uint64_t hcap = bk.hmask ? bk.hmask + 1 : 0;
-    ulonglong2 *hslots = !index             ? nullptr
-                         : hash_zeroed      ? (ulonglong2 *)bk.hslots
-                                            : hash_table_for(s, res, layer, bound, hcap);
+    const HashDev fill = !index ? HashDev{} : hash_zeroed ? bk.h : hash_table_for(s, res, layer, bound);
     {
-        Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2 + (double)hcap * sizeof(ulonglong2));
+        Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2 + (double)fill.bytes());
         launch_compact<T>(s, keys, rates, bound, bk.keys, (T *)bk.rates, (unsigned long long *)&md->maxbits,
-                          &md->n_live, w.lw, T(0), hslots, hslots ? hcap : 0, hash_zeroed);
+                          &md->n_live, w.lw, T(0), fill, hash_zeroed);
     }
     if (index && res.book_index) {
         bk.bits = index_bits_for(bound);
@@ -1420,6 +1457,12 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     res.hash_ring = 3;
     res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
                          : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
+    res.rate_size = (int)sizeof(T);
+    // compact tables are 4-8x smaller but take more probe rounds per
+    // lookup: 442@512 k_solve 101.6 us a launch against 72 us wide
+    res.compact_hashes = o.solve_hash_layout == 1;
+    res.hash_err.alloc(256, s);
+    HIPCHK(hipMemsetAsync(res.hash_err.p, 0, 256, s));
     SortWork w;
     HostStage stage, meta_host_buf;
     DBuf cand[2], sort_tmp, counts, offs, rates, packed;
@@ -2038,6 +2081,15 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     cand[1].release();
     counts.release();
     offs.release();
+    // every layer's key span (the OR / AND its forward steps folded) sizes
+    // its compact solve hash; layer 0 (the seeds) gets no hash
+    if (res.compact_hashes) {
+        read_meta(s, res, H, 0, steps - 1);
+        for (int i = 1; i < steps; ++i) {
+            res.info[i].key_or = H[i].or_all;
+            res.info[i].key_and = H[i].and_all;
+        }
+    }
     const auto t_fwd = std::chrono::steady_clock::now();
 
     // when every layer gets a solve hash the solve never searches, and the
@@ -2045,8 +2097,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // its own)
     {
         bool all_hashed = res.hash_max_bytes > 0;
-        for (int i = 1; i < steps && all_hashed; ++i)
-            all_hashed = (double)hash_capacity(res.info[i].n_raw) * sizeof(ulonglong2) <= res.hash_max_bytes;
+        for (int i = 1; i < steps && all_hashed; ++i) all_hashed = hash_layout(res, i, res.info[i].n_raw).present();
         res.book_index = !all_hashed;
     }
 
@@ -2116,21 +2167,18 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         HIPCHK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
         EventGuard join_guard{join_ev};
         const int i0 = steps - 3;
-        uint64_t own_cap = 0;
-        ulonglong2 *own = i0 > 0 ? hash_table_for(s, res, i0, res.info[i0].n_raw, own_cap) : nullptr;
-        if (own) HIPCHK(hipMemsetAsync(own, 0, own_cap * sizeof(ulonglong2), s));
+        HashDev own = i0 > 0 ? hash_table_for(s, res, i0, res.info[i0].n_raw) : HashDev{};
+        if (own.present()) HIPCHK(hipMemsetAsync(own.table(), 0, own.bytes(), s));
         for (int i = i0; i >= 0; --i) {
             progress_set(prog_total - (uint32_t)i - 2u, prog_total);
             const uint64_t n = res.info[i].n_raw;
             const Book &b1 = res.books[i + 1], &b2 = res.books[i + 2];
-            uint64_t next_cap = 0;
-            ulonglong2 *next = i - 1 > 0 ? hash_table_for(s, res, i - 1, res.info[i - 1].n_raw, next_cap) : nullptr;
+            const HashDev next = i - 1 > 0 ? hash_table_for(s, res, i - 1, res.info[i - 1].n_raw) : HashDev{};
             T *ri = (T *)((char *)rates_all.p + roff[i]);
             {
                 Timer t(s, "solve", (double)n * (8 + sizeof(T)));
                 launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
-                                hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, ri, next,
-                                next ? next_cap : 0, own, own ? own_cap : 0);
+                                hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, ri, next, own);
             }
             carve_book<T>(res, i, n);  // before the fork: a new arena chunk is allocated on the build stream
             HIPCHK(hipEventRecord(fork_ev, s));
@@ -2139,7 +2187,6 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             drop_hash(res, i + 2);
             HIPCHK(hipGetLastError());
             own = next;
-            own_cap = next_cap;
         }
         HIPCHK(hipEventRecord(join_ev, s2));
         HIPCHK(hipStreamWaitEvent(s, join_ev, 0));
@@ -2162,15 +2209,13 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         // layer i's solve hash (ring slot i % 3; its previous table, layer
         // i+3's, was last read by the solve of layer i+1) is cleared by this
         // solve and filled by the compaction that makes book i
-        uint64_t hcap = 0;
-        ulonglong2 *hs = i > 0 ? hash_table_for(s, res, i, n, hcap) : nullptr;
+        const HashDev hs = i > 0 ? hash_table_for(s, res, i, n) : HashDev{};
         {
             Timer t(s, "solve", (double)n * (8 + sizeof(T)));
             launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
-                            hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p, hs,
-                            hs ? hcap : 0);
+                            hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p, hs);
         }
-        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0, hs != nullptr);
+        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0, hs.present());
         drop_hash(res, i + 2);  // its last reader was this layer's solve
         HIPCHK(hipGetLastError());
         if (streamer) {
@@ -2241,6 +2286,15 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // single-pass look-back); a raised flag means a partial prefix was used
     if (w.read_error(s) || w_side.read_error(s2))
         throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
+    {
+        uint32_t herr = 0;
+        HIPCHK(hipMemcpyAsync(&herr, res.hash_err.p, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (herr)
+            throw Error(EGTB_ERR_INTERNAL, "a compact solve hash had no free slot within 32 buckets (flags " +
+                                               std::to_string(herr) +
+                                               "); build again with solve_hash_layout = 0 (wide tables)");
+    }
     read_meta(s, res, H, 0, steps - 1);
     for (int i = 0; i < steps; ++i) {
         res.info[i].n_live = H[i].n_live;

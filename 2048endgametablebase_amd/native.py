@@ -58,6 +58,8 @@ class Options(ctypes.Structure):
         ("deletion_threshold", ctypes.c_double),
         ("solve_hash_mb", ctypes.c_int32),
         ("forward_hash_steps", ctypes.c_int32),
+        ("solve_hash_layout", ctypes.c_int32),
+        ("optimal_branch_only", ctypes.c_int32),
     ]
 
 

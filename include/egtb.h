@@ -74,6 +74,12 @@ typedef struct egtb_options {
     int32_t forward_hash_steps;      /* forward dedup: 0 = key sets (hashing) while they fit free HBM, then
                                         sort + unique; -1 = sort + unique at every step; k > 0 = key sets
                                         for steps below k only */
+    int32_t solve_hash_layout;       /* successor hashes of the solve: 0 = 16-byte slots; 1 = compact
+                                        8-byte slots in 64-byte buckets where the layer's key span allows
+                                        (4-byte rates), else 16-byte slots */
+    int32_t optimal_branch_only;     /* RunOptions::optimal_branch_only: after the solve, book i (i > 20)
+                                        keeps only the records some state of layers i-1 / i-2 reaches as its
+                                        best move (BookSolver.cpp:611-735) */
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file

@@ -200,3 +200,24 @@ def test_solve_lookup_paths_agree(pkg, solve_hash_mb):
             assert book_sha(keys, rates) == layer["sha256"], f"layer {layer['i']}, solve_hash_mb {solve_hash_mb}"
     finally:
         res.close()
+
+
+@pytest.mark.parametrize("tag,layout", [("442_256", 1), ("442_256", 0), ("LL_64", 1), ("444_64", 1),
+                                        ("L3_128_uint32", 1), ("t_64", 1)])
+def test_solve_hash_layouts(pkg, tag, layout):
+    """Compact (8-byte slots keyed by each layer's key span) and wide
+    (16-byte slots) successor hashes give the reference's tables, in the
+    two-stream in-HBM backward pass and, with files, the single-stream one;
+    the compact layout covers 4-byte rates on 10- and 12-free-cell patterns
+    with identity, diagonal and horizontal symmetry."""
+    g = load_golden(tag)
+    gp = golden_params(g)
+    res = pkg.run_pattern_build(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                                gp["docheck"], "", success_rate_dtype=gp["dtype"], write_files=False, keep=True,
+                                solve_hash_layout=layout)
+    try:
+        for layer in g["layers"]:
+            keys, rates = res.book(layer["i"])
+            assert book_sha(keys, rates) == layer["sha256"], f"{tag} layer {layer['i']}, layout {layout}"
+    finally:
+        res.close()
