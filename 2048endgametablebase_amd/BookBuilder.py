@@ -19,7 +19,10 @@ from . import native
 from .config import SingletonConfig
 from .patterns import build_params, pattern_catalog, steps_and_docheck
 
-_RATE_DTYPES = {"float32": np.float32, "float64": np.float64, "uint32": np.uint32, "uint64": np.uint64}
+# success_rate_dtype -> element type of the rates (the 1-float types store
+# rate - 1 in a float; FormationRuntime.h:256-312)
+_RATE_DTYPES = {"float32": np.float32, "float64": np.float64, "uint32": np.uint32, "uint64": np.uint64,
+                "1-float32": np.float32, "1-float64": np.float64}
 
 
 def save_config_to_txt(output_path: str) -> None:

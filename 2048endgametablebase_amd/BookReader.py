@@ -197,19 +197,24 @@ class Table:
             pass
 
 
-# record layout and raw-to-rate scale per dtype (FormationRuntime.h:197-200,
-# ReaderRuntime.cpp:270-277: integer rates are divided by their max scale)
+# record layout, raw-to-rate scale and the value of a missing key per dtype
+# (FormationRuntime.h:197-200; ReaderRuntime.cpp:183-206,270-277: integer
+# rates are divided by their max scale, the 1-float types store rate - 1 and
+# read a missing key as -1)
 _REC = {
-    "float32": (np.dtype([("board", "<u8"), ("rate", "<f4")]), 1.0),
-    "float64": (np.dtype([("board", "<u8"), ("rate", "<f8")]), 1.0),
-    "uint32": (np.dtype([("board", "<u8"), ("rate", "<u4")]), 4e9),
-    "uint64": (np.dtype([("board", "<u8"), ("rate", "<u8")]), 1.6e18),
+    "float32": (np.dtype([("board", "<u8"), ("rate", "<f4")]), 1.0, 0.0),
+    "float64": (np.dtype([("board", "<u8"), ("rate", "<f8")]), 1.0, 0.0),
+    "uint32": (np.dtype([("board", "<u8"), ("rate", "<u4")]), 4e9, 0.0),
+    "uint64": (np.dtype([("board", "<u8"), ("rate", "<u8")]), 1.6e18, 0.0),
+    "1-float32": (np.dtype([("board", "<u8"), ("rate", "<f4")]), 1.0, -1.0),
+    "1-float64": (np.dtype([("board", "<u8"), ("rate", "<f8")]), 1.0, -1.0),
 }
 
 
 def _record_layout(dtype: str):
     try:
-        return _REC[dtype]
+        rec, scale, _ = _REC[dtype]
+        return rec, scale
     except KeyError:
         raise ValueError(f"unsupported success_rate_dtype {dtype!r}") from None
 
@@ -231,7 +236,8 @@ def find_classic_value(pathname: str, filename: str, search_key: int, success_ra
         return None
     n = size // rec.itemsize
     if n == 0:
-        return 0.0
+        return _REC[success_rate_dtype][2]
+    zero = _REC[success_rate_dtype][2]
     recs = np.memmap(path, dtype=rec, mode="r", shape=(n,))
     key = int(search_key) & 0xFFFFFFFFFFFFFFFF
     left, right = 0, n - 1
@@ -247,7 +253,7 @@ def find_classic_value(pathname: str, filename: str, search_key: int, success_ra
             if mid == 0:
                 break
             right = mid - 1
-    return 0.0
+    return zero
 
 
 def spawn_random_tile(board: int, spawn_rate4: float, rng) -> int:

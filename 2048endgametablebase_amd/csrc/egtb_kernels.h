@@ -21,11 +21,12 @@ struct TableDev {
     uint32_t present;       // 1 when the layer exists (reader: its file exists)
 };
 
-// Exact-match lookup of `key` in a sorted table of n records; absent -> 0
-// (reference search_arr, BookSolver.cpp:317-327).
+// Exact-match lookup of `key` in a sorted table of n records: true and the
+// rate when present (reference search_arr, BookSolver.cpp:317-327, whose
+// miss reads as the rate type's zero value).
 template <typename T>
-__device__ __forceinline__ T table_lookup_n(const TableDev<T> &t, uint64_t n, uint64_t key) {
-    if (n == 0) return (T)0;
+__device__ __forceinline__ bool table_find(const TableDev<T> &t, uint64_t n, uint64_t key, T &rate) {
+    if (n == 0) return false;
     const uint64_t bk = key >> (64 - t.bits);
     uint64_t lo = t.idx[bk], hi = t.idx[bk + 1];
     while (lo < hi) {
@@ -33,12 +34,59 @@ __device__ __forceinline__ T table_lookup_n(const TableDev<T> &t, uint64_t n, ui
         if (t.keys[mid] < key) lo = mid + 1;
         else hi = mid;
     }
-    return (lo < n && t.keys[lo] == key) ? t.rates[lo] : (T)0;
+    if (lo < n && t.keys[lo] == key) {
+        rate = t.rates[lo];
+        return true;
+    }
+    return false;
+}
+
+// As table_find, also giving the record's position.
+template <typename T>
+__device__ __forceinline__ bool table_find_pos(const TableDev<T> &t, uint64_t n, uint64_t key, T &rate,
+                                               uint64_t &pos) {
+    if (n == 0) return false;
+    const uint64_t bk = key >> (64 - t.bits);
+    uint64_t lo = t.idx[bk], hi = t.idx[bk + 1];
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (t.keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < n && t.keys[lo] == key) {
+        rate = t.rates[lo];
+        pos = lo;
+        return true;
+    }
+    return false;
+}
+
+// Order-preserving unsigned image of a rate (floats of either sign order as
+// these keys; unsigned rates are their own), for the max-rate statistic.
+template <typename T>
+__host__ __device__ __forceinline__ uint64_t rate_order_key(T v) {
+    if constexpr (sizeof(T) == 4) {
+        const uint32_t b = __builtin_bit_cast(uint32_t, v);
+        if constexpr (T(-1) < T(0)) return (b >> 31) ? ~b : (b | 0x80000000u);
+        else return b;
+    } else {
+        const uint64_t b = __builtin_bit_cast(uint64_t, v);
+        if constexpr (T(-1) < T(0)) return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+        else return b;
+    }
 }
 
 template <typename T>
-__device__ __forceinline__ T table_lookup_dev(const TableDev<T> &t, uint64_t key) {
-    return table_lookup_n(t, t.n, key);
+__host__ __device__ __forceinline__ T rate_from_order_key(uint64_t k) {
+    if constexpr (sizeof(T) == 4) {
+        uint32_t b = (uint32_t)k;
+        if constexpr (T(-1) < T(0)) b = (b >> 31) ? (b & 0x7fffffffu) : ~b;
+        return __builtin_bit_cast(T, b);
+    } else {
+        uint64_t b = k;
+        if constexpr (T(-1) < T(0)) b = (b >> 63) ? (b & 0x7fffffffffffffffull) : ~b;
+        return __builtin_bit_cast(T, b);
+    }
 }
 
 // Hash of one solved layer, the structure the backward solve looks
@@ -237,22 +285,31 @@ template <typename T>
 // every state with a rate above zero.
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
-                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
-                  const HashDev &clear = HashDev{}, const HashDev &own = HashDev{});
+                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T zero,
+                  T *rates, const HashDev &clear = HashDev{}, const HashDev &own = HashDev{});
 
+// Final layers: success -> max_scale, else zero.
 template <typename T>
-void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,
+void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale, T zero,
                         T *rates);
 
-// Records with rate > threshold, in order, in one look-back launch; it
-// shares the unique kernel's status words (ensure_lookback with
-// compact_tiles(n) unique tiles first). With a hash `fill`, the table is
-// cleared (unless `zeroed`) and the kept records hashed into it too.
+// Records with rate > threshold (or, with `keep`, those whose keep byte is
+// set), in order, in one look-back launch; it shares the unique kernel's
+// status words (ensure_lookback with compact_tiles(n) unique tiles first).
+// With a hash `fill`, the table is cleared (unless `zeroed`) and the kept
+// records hashed into it too.
 uint64_t compact_tiles(uint64_t n);
 template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
                     T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw,
-                    T threshold = T(0), const HashDev &fill = HashDev{}, bool zeroed = false);
+                    T threshold = T(0), const HashDev &fill = HashDev{}, bool zeroed = false,
+                    const uint8_t *keep = nullptr);
+
+// Marks (mark[pos] = 1) the records of table t that the n parents, with
+// `spawn` on each empty cell, reach as their best move; t has a bucket index.
+template <typename T>
+void launch_opt_mark(hipStream_t s, const LaunchShape &ls, const uint64_t *parents, uint64_t n, const uint16_t *lut,
+                     const PatternDev &p, uint64_t spawn, const TableDev<T> &t, T zero, uint8_t *mark);
 
 template <typename T>
 void launch_pack_records(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint32_t *out);

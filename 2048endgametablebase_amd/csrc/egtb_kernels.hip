@@ -1022,10 +1022,12 @@ __global__ void k_bucket_index(const uint64_t *__restrict__ keys, uint64_t n_bou
 // rate(b) = max_scale when do_check and b is a success board; otherwise
 // with E empty nibbles visited in ascending order,
 //   sp = fma(best2_k, 1 - p4, sp); sp = fma(best4_k, p4, sp)   (double)
-//   rate = (T)(sp / E)   (E = 0 -> 0)
-// where best_x_k is the T-typed max over the valid moves of the spawned
-// board of the next (spawn 2) / after-next (spawn 4) layer's rate of the
-// canonical moved board (absent -> 0).  The fused update is what the
+//   rate = (T)(sp / E)   (E = 0 -> zero)
+// where best_x_k is the T-typed max, starting from zero, over the valid
+// moves of the spawned board of the next (spawn 2) / after-next (spawn 4)
+// layer's rate of the canonical moved board (absent -> zero).  zero and
+// max_scale are the rate type's (0 and 1 for probabilities, -1 and 0 for the
+// 1-float types, 0 and 4e9 / 1.6e18 for the integer ones).  The fused update is what the
 // reference's -Ofast -march=x86-64-v3 build computes (CMakeLists.txt:82-83).
 template <typename T>
 __device__ __forceinline__ uint64_t rate_bits(T v) {
@@ -1071,7 +1073,7 @@ template <typename T>
 __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ keys, uint64_t n,
                                                 const uint16_t *__restrict__ glut, PatternDev p,
                                                 int do_check, HashDev h1, HashDev h2, TableDev<T> t1,
-                                                TableDev<T> t2, double w2, double w4, T max_scale,
+                                                TableDev<T> t2, double w2, double w4, T max_scale, T zero,
                                                 T *__restrict__ rates, ulonglong2 *__restrict__ clear,
                                                 uint64_t clear_n, HashDev own) {
     extern __shared__ uint16_t lut[];
@@ -1121,7 +1123,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
             }
             T r[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) r[j] = (T)0;
+            for (int j = 0; j < 8; ++j) r[j] = zero;
             if (!use_hash) {
                 // bucket-narrowed binary searches, advanced in lockstep
                 uint32_t lo[8], hi[8];
@@ -1215,15 +1217,15 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                     }
                 }
             }
-            T best[2] = {(T)0, (T)0};
+            T best[2] = {zero, zero};
 #pragma unroll
             for (int j = 0; j < 8; ++j) best[j >> 2] = best[j >> 2] < r[j] ? r[j] : best[j >> 2];
             sp = __fma_rn((double)best[0], w2, sp);
             sp = __fma_rn((double)best[1], w4, sp);
         }
-        const T rv = e > 0 ? (T)(sp / (double)e) : (T)0;
+        const T rv = e > 0 ? (T)(sp / (double)e) : zero;
         rates[i] = rv;
-        if (own.present() && rv > (T)0) hash_insert(own, b, rv);
+        if (own.present() && rv > zero) hash_insert(own, b, rv);
     }
 }
 
@@ -1241,10 +1243,54 @@ __global__ __launch_bounds__(256) void k_hash_build(const uint64_t *__restrict__
 
 // Final layers (reference final_situation_process, BookSolver.cpp:237-261).
 template <typename T>
-__global__ void k_final_rates(const uint64_t *__restrict__ keys, uint64_t n, PatternDev p, T max_scale,
+__global__ void k_final_rates(const uint64_t *__restrict__ keys, uint64_t n, PatternDev p, T max_scale, T zero,
                               T *__restrict__ rates) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) rates[i] = is_success(keys[i], p) ? max_scale : (T)0;
+    if (i < n) rates[i] = is_success(keys[i], p) ? max_scale : zero;
+}
+
+// ---------------------------------------------------------------------------
+// Optimal-branch marking (reference find_optimal_branches,
+// BookSolver.cpp:611-646): for every parent and every empty cell, the parent
+// with `spawn` (1 = a 2, 2 = a 4) on that cell has its four moves looked up
+// in the target layer; the first move (left, right, up, down) with the
+// strictly largest rate above zero marks its record.  A spawn with no such
+// move marks record 0, as the reference's best_index = 0 does.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_opt_mark(const uint64_t *__restrict__ parents, uint64_t n,
+                                                   const uint16_t *__restrict__ glut, PatternDev p, uint64_t spawn,
+                                                   TableDev<T> t, T zero, uint8_t *__restrict__ mark) {
+    extern __shared__ uint16_t lut[];
+    stage_lut(lut, glut);
+    if (t.n == 0) return;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t b = parents[i];
+        uint32_t em = empty_cells(b);
+        const SpawnMoves smv(b, lut);
+        while (em) {
+            const int k = __builtin_ctz(em);
+            em &= em - 1;
+            const uint64_t sp = b | (spawn << (4 * k));
+            uint64_t mv[4];
+            smv.of(k, spawn, lut, mv);
+            T best = zero;
+            uint64_t best_idx = 0;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                if (mv[d] == sp || !is_pattern(mv[d], p)) continue;
+                uint64_t pos;
+                T r;
+                if (table_find_pos(t, t.n, canonical(mv[d], p.symm), r, pos) && r > best) {
+                    best = r;
+                    best_idx = pos;
+                }
+            }
+            mark[best_idx] = 1;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1273,7 +1319,8 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
                                                              uint64_t *__restrict__ status,
                                                              unsigned long long *__restrict__ tile_counter,
                                                              unsigned long long tile_base, uint32_t epoch,
-                                                             uint32_t *__restrict__ err, HashDev fill) {
+                                                             uint32_t *__restrict__ err, HashDev fill,
+                                                             const uint8_t *__restrict__ keep) {
     __shared__ uint64_t scratch[kCompThreads / 64];
     __shared__ uint32_t s_tile;
     __shared__ uint64_t s_off;
@@ -1283,13 +1330,16 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
     const uint64_t base = (uint64_t)tile * kCompTile + (uint64_t)threadIdx.x * kCompItems;
     T r[kCompItems];
     uint64_t cnt = 0;
-    T mx = (T)0;
+    T mx = thr;
+    uint32_t kept = 0;  // bit j: item j is kept (rate > thr, or its keep byte is set)
 #pragma unroll
     for (int j = 0; j < kCompItems; ++j) {
         const uint64_t idx = base + j;
-        r[j] = idx < n ? rates[idx] : (T)0;
-        cnt += r[j] > thr;
-        mx = mx < r[j] ? r[j] : mx;
+        r[j] = idx < n ? rates[idx] : thr;
+        const bool k = idx < n && (keep ? keep[idx] != 0 : r[j] > thr);
+        kept |= (uint32_t)k << j;
+        cnt += k;
+        if (k && !keep) mx = mx < r[j] ? r[j] : mx;
     }
     uint64_t total;
     const uint64_t ex = block_exclusive_sum(cnt, scratch, total);
@@ -1309,7 +1359,7 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
     if (count_out && (uint64_t)(tile + 1) * kCompTile >= n && threadIdx.x == 0) *count_out = s_off + total;
 #pragma unroll
     for (int j = 0; j < kCompItems; ++j) {
-        if (r[j] > thr) {
+        if ((kept >> j) & 1u) {
             const uint64_t k = keys[base + j];
             out_keys[pos] = k;
             out_rates[pos] = r[j];
@@ -1317,8 +1367,8 @@ __global__ __launch_bounds__(kCompThreads) void k_compact_lb(const uint64_t *__r
             if (fill.present()) hash_insert(fill, k, r[j]);
         }
     }
-    // non-negative IEEE floats and unsigned ints both order as their bits
-    if (mx > (T)0) atomicMax(max_bits, (unsigned long long)rate_bits(mx));
+    // the largest kept rate, as its order-preserving key (0 = none kept)
+    if (mx > thr) atomicMax(max_bits, (unsigned long long)rate_order_key(mx));
 }
 
 // SoA -> packed records {u64 board; T rate} (SuccessEntry<T>,
@@ -1374,6 +1424,10 @@ void configure_kernels() {
     (void)hipFuncSetAttribute((const void *)k_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_solve<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_solve<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_opt_mark<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_opt_mark<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_opt_mark<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_opt_mark<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 void launch_build_lut(hipStream_t s, uint16_t *lut) {
@@ -1577,8 +1631,8 @@ void launch_bucket_index(hipStream_t s, const uint64_t *keys, uint64_t n_bound, 
 template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
-                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T *rates,
-                  const HashDev &clear, const HashDev &own) {
+                  const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T zero,
+                  T *rates, const HashDev &clear, const HashDev &own) {
     const uint64_t clear_n = clear.bytes() / sizeof(ulonglong2);
     if (n == 0) {
         if (clear_n) (void)hipMemsetAsync(clear.table(), 0, clear.bytes(), s);
@@ -1587,7 +1641,7 @@ void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, ui
     const uint32_t want = blocks_for(n > clear_n ? n : clear_n, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     hipLaunchKernelGGL((k_solve<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n,
-                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, rates, (ulonglong2 *)clear.table(),
+                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, zero, rates, (ulonglong2 *)clear.table(),
                        clear_n, own);
 }
 
@@ -1612,19 +1666,29 @@ void launch_hash_build(hipStream_t s, const uint64_t *keys, const T *rates, uint
 }
 
 template <typename T>
-void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale,
+void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale, T zero,
                         T *rates) {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_final_rates<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, p, max_scale,
+    hipLaunchKernelGGL((k_final_rates<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, p, max_scale, zero,
                        rates);
 }
 
 uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
 
 template <typename T>
+void launch_opt_mark(hipStream_t s, const LaunchShape &ls, const uint64_t *parents, uint64_t n, const uint16_t *lut,
+                     const PatternDev &p, uint64_t spawn, const TableDev<T> &t, T zero, uint8_t *mark) {
+    if (n == 0 || t.n == 0) return;
+    const uint32_t want = blocks_for(n, ls.threads);
+    const uint32_t grid = want < ls.blocks ? want : ls.blocks;
+    hipLaunchKernelGGL((k_opt_mark<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, parents, n,
+                       lut, p, spawn, t, zero, mark);
+}
+
+template <typename T>
 void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_t n, uint64_t *out_keys,
                     T *out_rates, unsigned long long *max_bits, uint64_t *count_out, LookbackWork &lw, T threshold,
-                    const HashDev &fill, bool zeroed) {
+                    const HashDev &fill, bool zeroed, const uint8_t *keep) {
     if (fill.present() && !zeroed) (void)hipMemsetAsync(fill.table(), 0, fill.bytes(), s);
     if (n == 0) {
         if (count_out) (void)hipMemsetAsync(count_out, 0, 8, s);
@@ -1634,7 +1698,7 @@ void launch_compact(hipStream_t s, const uint64_t *keys, const T *rates, uint64_
     const uint32_t epoch = lw.next_epoch();
     hipLaunchKernelGGL((k_compact_lb<T>), dim3(tiles), dim3(kCompThreads), 0, s, keys, rates, n, threshold,
                        out_keys, out_rates, max_bits, count_out, lw.status_uniq, lw.counters + 1, lw.claimed_uniq,
-                       epoch, lw.err, fill);
+                       epoch, lw.err, fill, keep);
     lw.claimed_uniq += tiles;
 }
 
@@ -1663,13 +1727,16 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
     template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
                                   const uint16_t *, const PatternDev &, int, const HashDev &,                 \
                                   const HashDev &, const TableDev<T> &, const TableDev<T> &, double, double,  \
-                                  T, T *, const HashDev &, const HashDev &);                                  \
+                                  T, T, T *, const HashDev &, const HashDev &);                               \
     template void launch_hash_build<T>(hipStream_t, const uint64_t *, const T *, uint64_t, const uint64_t *,  \
                                        const HashDev &);                                                      \
-    template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T *); \
+    template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T, T *); \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *, T *,      \
                                     unsigned long long *, uint64_t *, LookbackWork &, T, const HashDev &,     \
-                                    bool);                                                                    \
+                                    bool, const uint8_t *);                                                   \
+    template void launch_opt_mark<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,            \
+                                     const uint16_t *, const PatternDev &, uint64_t, const TableDev<T> &, T,  \
+                                     uint8_t *);                                                              \
     template void launch_pack_records<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint32_t *);
 
 EGTB_INSTANTIATE(float)

@@ -222,6 +222,7 @@ struct Result {
     int steps = 0;
     int rate_size = 4;
     bool rate_is_float = true;
+    double rate_zero = 0.0;  // the rate type's zero value: -1 for the 1-float types
     int dev = 0;
     bool keep_tables = true;
     Arena raw_arena, book_arena;

@@ -739,7 +739,7 @@ static void carve_book(Result &res, int layer, uint64_t bound) {
 // and index it. Sizes stay on the device; nothing is synchronised.
 template <typename T>
 static void compact_book(hipStream_t s, Result &res, int layer, const uint64_t *keys, const T *rates, uint64_t bound,
-                         SortWork &w, bool index, bool hash_zeroed = false) {
+                         SortWork &w, bool index, T zero, bool hash_zeroed = false) {
     Book &bk = res.books[layer];
     LayerDev *md = res.meta_dev() + layer;
     if (bound) w.ensure_lookback(s, 0, 0, compact_tiles(bound));
@@ -750,7 +750,7 @@ static void compact_book(hipStream_t s, Result &res, int layer, const uint64_t *
     {
         Timer t(s, "compact", (double)bound * (sizeof(T) + 8) * 2 + (double)fill.bytes());
         launch_compact<T>(s, keys, rates, bound, bk.keys, (T *)bk.rates, (unsigned long long *)&md->maxbits,
-                          &md->n_live, w.lw, T(0), fill, hash_zeroed);
+                          &md->n_live, w.lw, zero, fill, hash_zeroed);
     }
     if (index && res.book_index) {
         bk.bits = index_bits_for(bound);
@@ -762,9 +762,9 @@ static void compact_book(hipStream_t s, Result &res, int layer, const uint64_t *
 // Carve and compact (the single-stream path).
 template <typename T>
 static void make_book(hipStream_t s, Result &res, int layer, const uint64_t *keys, const T *rates, uint64_t bound,
-                      SortWork &w, bool index, bool hash_zeroed = false) {
+                      SortWork &w, bool index, T zero, bool hash_zeroed = false) {
     carve_book<T>(res, layer, bound);
-    compact_book<T>(s, res, layer, keys, rates, bound, w, index, hash_zeroed);
+    compact_book<T>(s, res, layer, keys, rates, bound, w, index, zero, hash_zeroed);
 }
 
 void HostSlab::map(size_t n) {
@@ -1346,7 +1346,7 @@ static void remove_if_exists(const std::string &path) {
 // (reference read_split_layer_file on restart, BookSolver.cpp:500-507).
 template <typename T>
 static void load_book_file(hipStream_t s, Result &res, int layer, const std::string &path, HostStage &stage,
-                           DBuf &packed, LayerDev *H) {
+                           DBuf &packed, LayerDev *H, bool hash = true) {
     constexpr size_t rec = 8 + sizeof(T);
     FILE *f = std::fopen(path.c_str(), "rb");
     if (!f) throw Error(EGTB_ERR_IO, "cannot open " + path);
@@ -1377,7 +1377,7 @@ static void load_book_file(hipStream_t s, Result &res, int layer, const std::str
         launch_unpack_records(s, (const uint32_t *)packed.p, n, (int)sizeof(T), bk.keys, bk.rates);
     }
     launch_bucket_index(s, bk.keys, n, bk.n_dev, bk.bits, bk.idx);
-    build_hash<T>(s, res, layer, n);
+    if (hash) build_hash<T>(s, res, layer, n);
     HIPCHK(hipStreamSynchronize(s));  // the staging buffers are reused
 }
 
@@ -1396,16 +1396,13 @@ static void write_book_file(hipStream_t s, const std::string &path, const Book &
     write_file_atomic(path, stage.p, bytes);
 }
 
+// The stats row's max success rate from the largest kept rate's order key:
+// (max - zero) / (max_scale - zero), as the reference writes it
+// (BookSolver.cpp:570-581).
 template <typename T>
-static double rate_unit(uint64_t bits) {
-    T v;
-    if (sizeof(T) == 4) {
-        uint32_t b32 = (uint32_t)bits;
-        std::memcpy(&v, &b32, 4);
-    } else {
-        std::memcpy(&v, &bits, 8);
-    }
-    return RateTraits<T>::to_unit(v);
+static double rate_unit(uint64_t key, T zero, T max_scale) {
+    const T v = rate_from_order_key<T>(key);
+    return static_cast<double>(v - zero) / static_cast<double>(max_scale - zero);
 }
 
 // Copy layer records [lo, hi] of the device metadata into the pinned mirror
@@ -1422,13 +1419,17 @@ static void read_meta(hipStream_t s, Result &res, LayerDev *host, int lo, int hi
 // spawn-2 stream of step i+1 is appended behind it, so layer i+1 is one sort
 // + unique of that buffer.  Backward: sizes stay on the device, no sync.
 template <typename T>
-void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const egtb_options &o, Result &res) {
+void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const egtb_options &o, Result &res,
+               bool one_minus) {
     DeviceCtx &C = device_ctx(o.device);
     hipStream_t s = C.stream;
     const int steps = o.steps;
     const bool files = o.write_files != 0;
     const std::string prefix = o.pathname ? o.pathname : "";
-    const T max_scale = RateTraits<T>::max_scale();
+    // the rate type's scale and zero (FormationRuntime.h:261-303): the
+    // 1-float types store rate - 1, so success is 0 and a miss is -1
+    const T max_scale = one_minus ? T(0) : RateTraits<T>::max_scale();
+    const T zero = one_minus ? T(-1) : T(0);
     const double w4 = o.spawn_rate4, w2 = 1.0 - o.spawn_rate4;
     const uint32_t prog_total = (uint32_t)(2 * steps);
     progress_set(0, prog_total);
@@ -1458,6 +1459,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
                          : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
     res.rate_size = (int)sizeof(T);
+    res.rate_zero = static_cast<double>(zero);
+    if (o.optimal_branch_only && o.host_tables)
+        throw Error(EGTB_ERR_ARG, "optimal_branch_only needs the books in HBM (host_tables = 0)");
     // compact tables are 4-8x smaller but take more probe rounds per
     // lookup: 442@512 k_solve 101.6 us a launch against 72 us wide
     res.compact_hashes = o.solve_hash_layout == 1;
@@ -2123,8 +2127,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     rates.ensure(max_n * sizeof(T) + 8, s);
     for (int i = steps - 2; i <= steps - 1; ++i) {
         const uint64_t n = res.info[i].n_raw;
-        launch_final_rates<T>(s, res.raw[i], n, p, max_scale, (T *)rates.p);
-        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, true);
+        launch_final_rates<T>(s, res.raw[i], n, p, max_scale, zero, (T *)rates.p);
+        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, true, zero);
     }
     if (raw_writer) raw_writer->finish();
     if (files) {
@@ -2178,12 +2182,13 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             {
                 Timer t(s, "solve", (double)n * (8 + sizeof(T)));
                 launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
-                                hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, ri, next, own);
+                                hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, zero, ri, next,
+                                own);
             }
             carve_book<T>(res, i, n);  // before the fork: a new arena chunk is allocated on the build stream
             HIPCHK(hipEventRecord(fork_ev, s));
             HIPCHK(hipStreamWaitEvent(s2, fork_ev, 0));
-            compact_book<T>(s2, res, i, res.raw[i], ri, n, w, false);
+            compact_book<T>(s2, res, i, res.raw[i], ri, n, w, false, zero);
             drop_hash(res, i + 2);
             HIPCHK(hipGetLastError());
             own = next;
@@ -2213,9 +2218,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         {
             Timer t(s, "solve", (double)n * (8 + sizeof(T)));
             launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
-                            hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, (T *)rates.p, hs);
+                            hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, zero, (T *)rates.p,
+                            hs);
         }
-        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0, hs.present());
+        make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0, zero, hs.present());
         drop_hash(res, i + 2);  // its last reader was this layer's solve
         HIPCHK(hipGetLastError());
         if (streamer) {
@@ -2234,7 +2240,8 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 const double dt = std::chrono::duration<double>(l1 - l0).count();
                 char row[256];
                 std::snprintf(row, sizeof row, "%d,%llu,%g,%g mbps,%g,%s\n", i, (unsigned long long)n,
-                              rate_unit<T>(H[i].maxbits), round_to_2((double)n / std::max(dt, 0.01) / 1e6),
+                              rate_unit<T>(H[i].maxbits, zero, max_scale),
+                              round_to_2((double)n / std::max(dt, 0.01) / 1e6),
                               o.deletion_threshold, now_string().c_str());
                 FILE *f = std::fopen((prefix + "stats.txt").c_str(), "a");
                 if (f) {
@@ -2249,7 +2256,9 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                 // (reference BookSolver.cpp:584-588)
                 const int j = i + 2;
                 const uint64_t nj = H[j].n_live;
-                const T thr = static_cast<T>(o.deletion_threshold * (double)max_scale);
+                // the threshold in the rate type's own scale (BookSolver.cpp:590)
+                const T thr = static_cast<T>(o.deletion_threshold * static_cast<double>(max_scale - zero) +
+                                             static_cast<double>(zero));
                 thr_keys.ensure(nj * 8 + 8, s);
                 thr_rates.ensure(nj * sizeof(T) + 8, s);
                 thr_cnt.ensure(16, s);
@@ -2296,6 +2305,78 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
                                                "); build again with solve_hash_layout = 0 (wide tables)");
     }
     read_meta(s, res, H, 0, steps - 1);
+
+    // ---- optimal branches only (reference keep_only_optimal_branches_impl,
+    // BookSolver.cpp:687-735): from layer 21 up, book i keeps the records
+    // that some state of book i-2 (spawning a 4) or of book i-1 (spawning a
+    // 2) reaches as its best move, the parents being the already filtered
+    // books.  With files the pass reads the books from their files (so a
+    // deletion threshold applies first, as in the reference) and keeps its
+    // place in <pathname>optlayer for a resumed run; without files it filters
+    // the books in HBM.
+    if (o.optimal_branch_only && steps > 21) {
+        const std::string optpath = prefix + "optlayer";
+        int done = -1;  // layers up to here were filtered by an earlier run
+        if (files) {
+            if (FILE *f = std::fopen(optpath.c_str(), "r")) {
+                int v = 0;
+                if (std::fscanf(f, "%d", &v) == 1) done = v;
+                std::fclose(f);
+            }
+        }
+        DBuf mark, cnt;
+        cnt.alloc(64, s);
+        bool started = false;
+        for (int i = 21; i < steps; ++i) {
+            if (!started) {
+                if (done >= i) continue;
+                if (files) {
+                    const std::string p0 = prefix + std::to_string(i - 2) + ".book";
+                    const std::string p1 = prefix + std::to_string(i - 1) + ".book";
+                    if (!file_exists(p0) || !file_exists(p1)) continue;
+                    load_book_file<T>(s, res, i - 2, p0, stage, packed, H, false);
+                    load_book_file<T>(s, res, i - 1, p1, stage, packed, H, false);
+                }
+                started = true;
+            }
+            if (files) {
+                FILE *f = std::fopen(optpath.c_str(), "w");
+                if (!f) throw Error(EGTB_ERR_IO, "cannot write " + optpath);
+                std::fprintf(f, "%d", i - 1);
+                std::fclose(f);
+                load_book_file<T>(s, res, i, prefix + std::to_string(i) + ".book", stage, packed, H, false);
+            }
+            Book &bt = res.books[i];
+            const uint64_t nt = H[i].n_live;
+            if (!bt.idx) {  // in-HBM books of hashed builds carry no bucket index
+                bt.bits = index_bits_for(nt);
+                bt.idx = (uint32_t *)res.book_arena.alloc(((1ull << bt.bits) + 1) * 4);
+                launch_bucket_index(s, bt.keys, nt, nullptr, bt.bits, bt.idx);
+            }
+            mark.ensure(nt + 64, s);
+            HIPCHK(hipMemsetAsync(mark.p, 0, nt + 64, s));
+            TableDev<T> t{bt.keys, (const T *)bt.rates, bt.idx, nullptr, nt, bt.bits, 1};
+            launch_opt_mark<T>(s, C.shape, res.books[i - 2].keys, H[i - 2].n_live, C.lut, p, 2, t, zero,
+                               (uint8_t *)mark.p);
+            launch_opt_mark<T>(s, C.shape, res.books[i - 1].keys, H[i - 1].n_live, C.lut, p, 1, t, zero,
+                               (uint8_t *)mark.p);
+            uint64_t *keys2 = (uint64_t *)res.book_arena.alloc(nt * 8 + 8);
+            T *rates2 = (T *)res.book_arena.alloc(nt * sizeof(T) + 8);
+            w.ensure_lookback(s, 0, 0, compact_tiles(nt ? nt : 1));
+            launch_compact<T>(s, bt.keys, (const T *)bt.rates, nt, keys2, rates2, (unsigned long long *)cnt.p + 1,
+                              &res.meta_dev()[i].n_live, w.lw, zero, HashDev{}, false, (const uint8_t *)mark.p);
+            HIPCHK(hipGetLastError());
+            read_meta(s, res, H, i, i);
+            bt.keys = keys2;
+            bt.rates = rates2;
+            bt.n_dev = &res.meta_dev()[i].n_live;
+            bt.idx = nullptr;  // the index was over the unfiltered keys
+            if (files)
+                write_book_file<T>(s, prefix + std::to_string(i) + ".book", bt, H[i].n_live, stage, packed);
+        }
+        if (files) remove_if_exists(optpath);
+        if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
+    }
     for (int i = 0; i < steps; ++i) {
         res.info[i].n_live = H[i].n_live;
         res.books[i].n = H[i].n_live;
@@ -2328,12 +2409,15 @@ void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *p
     if (!(o->spawn_rate4 >= 0.0 && o->spawn_rate4 <= 1.0)) throw Error(EGTB_ERR_ARG, "spawn_rate4 out of [0, 1]");
     const PatternDev p = make_pattern(pattern, o->target);
     const std::string dt = o->success_rate_dtype ? o->success_rate_dtype : "float32";
-    if (dt != "float32" && dt != "float64" && dt != "uint32" && dt != "uint64")
-        throw Error(EGTB_ERR_ARG, "success_rate_dtype '" + dt + "' is not supported");
-    if (dt == "float32") run_build<float>(seeds, n_seeds, p, *o, res);
-    else if (dt == "float64") run_build<double>(seeds, n_seeds, p, *o, res);
-    else if (dt == "uint32") run_build<uint32_t>(seeds, n_seeds, p, *o, res);
-    else run_build<uint64_t>(seeds, n_seeds, p, *o, res);
+    // the reference's rate types (success_rate_kind_from_name,
+    // FormationRuntime.h:256-270); 1-float32 / 1-float64 store rate - 1
+    if (dt == "float32") run_build<float>(seeds, n_seeds, p, *o, res, false);
+    else if (dt == "float64") run_build<double>(seeds, n_seeds, p, *o, res, false);
+    else if (dt == "uint32") run_build<uint32_t>(seeds, n_seeds, p, *o, res, false);
+    else if (dt == "uint64") run_build<uint64_t>(seeds, n_seeds, p, *o, res, false);
+    else if (dt == "1-float32") run_build<float>(seeds, n_seeds, p, *o, res, true);
+    else if (dt == "1-float64") run_build<double>(seeds, n_seeds, p, *o, res, true);
+    else throw Error(EGTB_ERR_ARG, "success_rate_dtype '" + dt + "' is not supported");
 }
 
 }  // namespace egtb

@@ -18,6 +18,7 @@ struct TableSet {
     int rate_size = 4;
     bool is_float = true;
     double scale = 1.0;  // raw rate / scale = probability
+    double zero = 0.0;   // the value of a key absent from its layer (-1 for the 1-float types)
     double p4 = 0.1;
     // evaluate every symmetry op and keep the one whose best rate is largest
     // (> 0); reference prefer_max_result_ (ReaderRuntime.cpp:827-832,1053)

@@ -91,7 +91,8 @@ __device__ double fallback_value(uint64_t moved, const uint16_t *lut, double p4)
 template <typename T, typename OutT>
 __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ boards, uint64_t n,
                                                  const TableDev<T> *__restrict__ tabs, int steps,
-                                                 int64_t nums_adjust, PatternDev p, double scale, double p4,
+                                                 int64_t nums_adjust, PatternDev p, double scale, double zero,
+                                                 double p4,
                                                  int prefer_max, int round9, const uint16_t *__restrict__ glut,
                                                  OutT *__restrict__ out4,
                                                  int32_t *__restrict__ status, int32_t *__restrict__ op_out) {
@@ -118,8 +119,12 @@ __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ bo
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     if (mv[d] != x && is_pattern(mv[d], p)) {
-                        const T raw = table_lookup_dev(t, canonical(mv[d], p.symm));
-                        const double val = (double)raw / scale;
+                        // a present key reads as its normalised rate, an absent
+                        // one as the type's zero (find_success_entry_in_file,
+                        // ReaderRuntime.cpp:270-316)
+                        T raw;
+                        const double val = table_find(t, t.n, canonical(mv[d], p.symm), raw) ? (double)raw / scale
+                                                                                                  : zero;
                         r[c_absdir[op][d]] = (OutT)val;
                         // compared as the reader reports them: 4-byte rates
                         // rounded to 1e-9 (maybe_round_value, ReaderRuntime.cpp:208-214,
@@ -196,7 +201,7 @@ static void launch_lookup_t(hipStream_t s, const TableSet &ts, const DeviceCtx &
     const uint32_t grid = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((k_lookup<T, OutT>), dim3(grid ? grid : 1), dim3(threads), kLutEntries * sizeof(uint16_t), s,
                        boards, n, (const TableDev<T> *)ts.tabs.p, ts.steps, ts.nums_adjust, ts.pattern, ts.scale,
-                       ts.p4, ts.prefer_max, ts.rate_size == 4 ? 1 : 0, C.lut, out4, status, op_out);
+                       ts.zero, ts.p4, ts.prefer_max, ts.rate_size == 4 ? 1 : 0, C.lut, out4, status, op_out);
 }
 
 template <typename OutT>
@@ -225,13 +230,16 @@ struct DTypeDesc {
     int size;
     bool is_float;
     double scale;  // reader normalisation (ReaderRuntime.cpp:183-206,271-277)
+    double zero;   // what a missing key reads as (dtype_info_for_name's zero_value)
 };
 
 static DTypeDesc dtype_desc(const std::string &dt) {
-    if (dt == "float32") return {4, true, 1.0};
-    if (dt == "float64") return {8, true, 1.0};
-    if (dt == "uint32") return {4, false, 4e9};
-    if (dt == "uint64") return {8, false, 1.6e18};
+    if (dt == "float32") return {4, true, 1.0, 0.0};
+    if (dt == "float64") return {8, true, 1.0, 0.0};
+    if (dt == "uint32") return {4, false, 4e9, 0.0};
+    if (dt == "uint64") return {8, false, 1.6e18, 0.0};
+    if (dt == "1-float32") return {4, true, 1.0, -1.0};
+    if (dt == "1-float64") return {8, true, 1.0, -1.0};
     throw Error(EGTB_ERR_ARG, "success_rate_dtype '" + dt + "' is not supported");
 }
 
@@ -264,6 +272,7 @@ std::unique_ptr<TableSet> table_from_files(const char *prefix, int steps, const 
     ts->rate_size = dd.size;
     ts->is_float = dd.is_float;
     ts->scale = dd.scale;
+    ts->zero = dd.zero;
     ts->p4 = p4;
     ts->n.assign(steps, 0);
     const size_t rec = 8 + dd.size;
@@ -325,6 +334,7 @@ std::unique_ptr<TableSet> table_from_result(const Result &r, const egtb_pattern 
     ts->rate_size = r.rate_size;
     ts->is_float = r.rate_is_float;
     ts->scale = r.rate_is_float ? 1.0 : (r.rate_size == 4 ? 4e9 : 1.6e18);
+    ts->zero = r.rate_zero;
     ts->p4 = p4;
     ts->borrowed = &r;
     ts->n.assign(r.steps, 0);

@@ -88,10 +88,10 @@ def round9(x):
 # record layout and max scale per dtype (FormationRuntime.h:197-200,
 # ReaderRuntime.cpp:183-206: integer rates are divided by their max scale)
 REC_FORMAT = {"float32": ("<Qf", 1.0), "float64": ("<Qd", 1.0), "uint32": ("<QI", 4e9),
-              "uint64": ("<QQ", 1.6e18)}
+              "uint64": ("<QQ", 1.6e18), "1-float32": ("<Qf", 1.0, -1.0), "1-float64": ("<Qd", 1.0, -1.0)}
 
 
-def find_in_file(path, key, rec_fmt="<Qf", scale=1.0):
+def find_in_file(path, key, rec_fmt="<Qf", scale=1.0, zero=0.0):
     size = struct.calcsize(rec_fmt)
     with open(path, "rb") as fh:
         fh.seek(0, os.SEEK_END)
@@ -109,7 +109,7 @@ def find_in_file(path, key, rec_fmt="<Qf", scale=1.0):
                 if mid == 0:
                     break
                 right = mid - 1
-    return 0.0
+    return zero
 
 
 def is_pattern(b, masks):

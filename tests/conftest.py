@@ -71,4 +71,4 @@ def book_sha(keys, rates):
 def golden_params(g):
     return dict(seeds=[int(s, 16) for s in g["seeds"]], masks=[int(m, 16) for m in g["masks"]],
                 shifts=list(g["shifts"]), symm=g["symm"], target=g["target"], steps=g["steps"],
-                docheck=g["docheck"], dtype=g.get("dtype", "float32"))
+                docheck=g["docheck"], dtype=g.get("dtype", "float32"), options=g.get("options", {}))

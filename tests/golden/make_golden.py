@@ -11,7 +11,10 @@ writes tests/golden/<pattern>_<tile>.json holding, per layer:
                   the rate in the run's success_rate_dtype);
   * ``sample`` -- up to 16 evenly spaced records as [board_hex, rate_bits].
 
-Runs in a dtype other than float32 are written to <pattern>_<tile>_<dtype>.json.
+Runs in a dtype other than float32 are written to <pattern>_<tile>_<dtype>.json;
+runs with post-solve pruning add "_dt<threshold>" (RunOptions::deletion_threshold)
+and "_opt" (RunOptions::optimal_branch_only) to the tag, and record the
+options under "options".
 
 Usage:  python tests/golden/make_golden.py [name ...]
 """
@@ -41,9 +44,25 @@ RUNS = [
     ("L3", 7), ("L3", 8), ("442", 7), ("442", 8), ("442", 9), ("t", 8), ("LL", 7),
     ("4431", 7), ("LL", 6), ("444", 6), ("4432", 6), ("free8", 6), ("t", 6),
     ("442", 7, "uint32"), ("L3", 7, "uint32"), ("LL", 6, "uint32"), ("442", 7, "float64"),
-    ("442", 7, "uint64"),
+    ("442", 7, "uint64"), ("442", 7, "1-float32"), ("442", 7, "1-float64"), ("LL", 6, "1-float32"),
+    ("442", 7, "float32", {"deletion_threshold": 0.3}),
+    ("442", 7, "uint32", {"deletion_threshold": 0.05}),
+    ("442", 7, "float32", {"optimal_branch_only": True}),
+    ("442", 7, "uint32", {"optimal_branch_only": True, "deletion_threshold": 0.3}),
+    ("LL", 6, "float32", {"optimal_branch_only": True}),
 ]
-RATE_FORMAT = {"float32": "<f4", "uint32": "<u4", "float64": "<f8", "uint64": "<u8"}
+RATE_FORMAT = {"float32": "<f4", "uint32": "<u4", "float64": "<f8", "uint64": "<u8", "1-float32": "<f4",
+               "1-float64": "<f8"}
+
+
+def run_tag(name, t, dtype="float32", opts=None):
+    tag = f"{name}_{2 ** t}" + ("" if dtype == "float32" else f"_{dtype}")
+    opts = opts or {}
+    if opts.get("deletion_threshold"):
+        tag += f"_dt{opts['deletion_threshold']}"
+    if opts.get("optimal_branch_only"):
+        tag += "_opt"
+    return tag
 
 # free8 (docs_and_configs/patterns_config.json): no masks, every nibble a
 # success position, full symmetry, single seed.  Built from its seed as is.
@@ -62,12 +81,13 @@ def params(name, t):
                 steps=p["steps"], docheck=p["docheck_step"], is_free=p["is_free"])
 
 
-def digest_run(name, t, workdir, dtype="float32"):
+def digest_run(name, t, workdir, dtype="float32", opts=None):
     p = params(name, t)
     tile = 2 ** t
+    opts = opts or {}
     prefix = os.path.join(workdir, f"{name}_{tile}_")
     ref.run_build(p["seeds"], p["masks"], p["shifts"], p["symm"], t, p["steps"], p["docheck"],
-                  prefix, is_free=p["is_free"], dtype=dtype, threads=8)
+                  prefix, is_free=p["is_free"], dtype=dtype, threads=8, **opts)
     lengths = {}
     with open(prefix + "stats.txt") as fh:
         for row in list(csv.reader(fh))[1:]:
@@ -92,7 +112,7 @@ def digest_run(name, t, workdir, dtype="float32"):
         "pattern": name, "target": t, "steps": p["steps"], "docheck": p["docheck"],
         "seeds": [f"{s:016x}" for s in p["seeds"]], "masks": [f"{m:x}" for m in p["masks"]],
         "shifts": list(p["shifts"]), "symm": p["symm"], "spawn_rate4": 0.1,
-        "dtype": dtype, "source": "reference run_pattern_build_cpp, g++ -Ofast "
+        "dtype": dtype, "options": opts, "source": "reference run_pattern_build_cpp, g++ -Ofast "
         "-march=x86-64-v3 -flto (oracle/ref/Makefile)", "layers": layers,
     }
 
@@ -104,12 +124,13 @@ def main(argv):
     for run in RUNS:
         name, t = run[0], run[1]
         dtype = run[2] if len(run) > 2 else "float32"
-        tag = f"{name}_{2 ** t}" + ("" if dtype == "float32" else f"_{dtype}")
+        opts = run[3] if len(run) > 3 else {}
+        tag = run_tag(name, t, dtype, opts)
         if want and tag not in want and name not in want:
             continue
         work = tempfile.mkdtemp(prefix="egtb_golden_", dir="/tmp")
         try:
-            out = digest_run(name, t, work, dtype)
+            out = digest_run(name, t, work, dtype, opts)
         finally:
             shutil.rmtree(work, ignore_errors=True)
         with open(os.path.join(os.path.dirname(__file__), f"{tag}.json"), "w") as fh:

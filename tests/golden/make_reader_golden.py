@@ -47,6 +47,8 @@ CASES = {
     "LL_64": ("LL", 6, "float32", "LL", 600),
     "444_64": ("444", 6, "float32", "444", 600),
     "442_512": ("442", 9, "float32", "442", 2400),
+    # rate - 1 stored, a missing key reads as -1
+    "442_128_1-float32": ("442", 7, "1-float32", "442", 400),
 }
 
 

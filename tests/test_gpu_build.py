@@ -44,13 +44,45 @@ def test_layers_match_oracle(pkg, orc, pattern, target):
         ob.close()
 
 
+def file_build(pkg, gp, prefix, **extra):
+    pkg.run_pattern_build(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                          gp["docheck"], prefix, success_rate_dtype=gp["dtype"], write_files=True,
+                          deletion_threshold=gp["options"].get("deletion_threshold", 0.0),
+                          optimal_branch_only=gp["options"].get("optimal_branch_only", False), **extra)
+
+
+def file_digest_mismatches(g, prefix):
+    bad = []
+    for layer in g["layers"]:
+        path = prefix + f"{layer['i']}.book"
+        raw = open(path, "rb").read() if os.path.exists(path) else b""
+        if hashlib.sha256(raw).hexdigest() != layer["sha256"]:
+            bad.append(layer["i"])
+    return bad
+
+
 @pytest.mark.parametrize("tag", golden_tags())
-def test_matches_reference_golden(pkg, tag):
-    """Every solved layer's packed bytes hash to the reference's .book file."""
+def test_matches_reference_golden(pkg, tag, tmp_path):
+    """Every solved layer's packed bytes hash to the reference's .book file.
+    Runs with post-solve pruning (deletion_threshold, optimal_branch_only)
+    act on the files, so those are compared as written; an optimal-branch
+    run without a threshold is also checked on the books kept in HBM."""
     g = load_golden(tag)
     gp = golden_params(g)
     print(f"case golden {tag}", flush=True)
-    res = _build(pkg, gp, dtype=gp["dtype"])
+    if gp["options"]:
+        prefix = str(tmp_path / f"{tag}_")
+        file_build(pkg, gp, prefix)
+        bad = file_digest_mismatches(g, prefix)
+        assert not bad, f"layers differing from the reference: {bad[:10]}"
+        assert not os.path.exists(prefix + "optlayer")
+        if gp["options"].get("deletion_threshold"):
+            return
+        res = pkg.run_pattern_build(gp["seeds"], gp["masks"], gp["shifts"], gp["symm"], gp["target"], gp["steps"],
+                                    gp["docheck"], "", success_rate_dtype=gp["dtype"], write_files=False,
+                                    keep=True, optimal_branch_only=True)
+    else:
+        res = _build(pkg, gp, dtype=gp["dtype"])
     try:
         bad = []
         for layer in g["layers"]:
@@ -221,3 +253,28 @@ def test_solve_hash_layouts(pkg, tag, layout):
             assert book_sha(keys, rates) == layer["sha256"], f"{tag} layer {layer['i']}, layout {layout}"
     finally:
         res.close()
+
+
+def test_optimal_branch_pass_resumes(pkg, tmp_path):
+    """An optimal-branch pass cut short (its place in <pathname>optlayer, the
+    books above it still unfiltered) finishes on a rerun with the
+    reference's files; an unfinished pass of a threshold build likewise."""
+    for tag in ("442_128_opt", "442_128_uint32_dt0.3_opt"):
+        g = load_golden(tag)
+        gp = golden_params(g)
+        plain = dict(gp, options={"deletion_threshold": gp["options"].get("deletion_threshold", 0.0)})
+        a = str(tmp_path / f"plain_{tag}_")
+        b = str(tmp_path / f"opt_{tag}_")
+        file_build(pkg, plain, a)
+        file_build(pkg, gp, b)
+        assert not file_digest_mismatches(g, b)
+        # interrupt after layer 40: books 41.. as the solve left them
+        for i in range(41, gp["steps"]):
+            with open(a + f"{i}.book", "rb") as src, open(b + f"{i}.book", "wb") as dst:
+                dst.write(src.read())
+        with open(b + "optlayer", "w") as fh:
+            fh.write("40")
+        assert file_digest_mismatches(g, b)
+        file_build(pkg, gp, b)
+        assert not file_digest_mismatches(g, b), tag
+        assert not os.path.exists(b + "optlayer")

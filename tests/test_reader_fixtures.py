@@ -91,7 +91,7 @@ def test_fixtures_cover_the_edge_cases():
         results = [w for _, w in fx["expected"]]
         assert any(None in d.values() for d, dt in results if dt), tag
         assert any(dt == "" and set(d.values()) == {""} for d, dt in results), tag
-        assert sum(1 for d, dt in results if dt) > 300, tag
+        assert sum(1 for d, dt in results if dt) > 250, tag
         assert all(dt in ("", fx["dtype"]) for _, dt in results), tag
         for d, dt in results:
             if not dt:
