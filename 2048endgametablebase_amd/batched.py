@@ -11,15 +11,20 @@ import numpy as np
 from . import native
 from .patterns import build_params
 
-# The batched workload: 24 jobs of the catalogue's pinned patterns at
-# uneven targets -- 10 free cells at 64..1024, 12 free cells at 32/64, 14
-# free cells at 16 -- whose layers range from ~1e3 to ~5e7 states
-# (1.0e9 raw states in all; 6.2 s of plain builds back to back on MI355X).
+# The batched workload: 24 jobs of L3, 442 and t (10 free cells) at
+# targets 64..1024 and the 12-free-cell 444-style patterns 444, LL and 4431
+# at 32..128, plus 4432 / 4441 at 32 and 4442 at 16 -- layers from ~1e1 to
+# ~1e7 states, 3.7e9 raw states in all.  The 12-free-cell patterns stop at
+# 128: a batch keeps every job's raw layers and books in HBM at once (the
+# 10-free-cell dozen peaks at 71 GB, 444@256 adds 34 GB), 444@512 alone has
+# 1.9e9 states and 444@1024 4.2e9.
 DEFAULT_JOBS: List[Tuple[str, int]] = [
     ("L3", 6), ("L3", 7), ("L3", 8), ("L3", 9), ("L3", 10),
     ("442", 6), ("442", 7), ("442", 8), ("442", 9), ("442", 10),
-    ("t", 6), ("t", 7), ("t", 8), ("t", 9), ("t", 10),
-    ("LL", 5), ("LL", 6), ("444", 5), ("444", 6), ("4431", 5), ("4431", 6),
+    ("t", 7), ("t", 8), ("t", 9), ("t", 10),
+    ("444", 5), ("444", 6), ("444", 7),
+    ("LL", 5), ("LL", 6),
+    ("4431", 5), ("4431", 6),
     ("4432", 5), ("4441", 5), ("4442", 4),
 ]
 

@@ -33,6 +33,9 @@ struct DeviceCtx {
 };
 
 DeviceCtx &device_ctx(int dev);
+// hipMallocAsync / hipMalloc that, when the device is out of memory, hands
+// the pool's cached blocks back and tries once more
+hipError_t device_alloc(void **p, size_t nbytes, hipStream_t s, bool async);
 
 // Move-only device buffer, allocated and freed in stream order.
 struct DBuf {

@@ -93,12 +93,29 @@ DeviceCtx &device_ctx(int dev) {
 // Stream-ordered device buffers
 // ---------------------------------------------------------------------------
 
+// Device memory, from the stream-ordered pool (async) or not.  The pool
+// keeps freed blocks for reuse (its release threshold is unlimited), and
+// those can starve an allocation of another size or of the other kind: on
+// failure the device is synchronised, the pool's unused blocks are handed
+// back and the allocation is tried once more.
+hipError_t device_alloc(void **p, size_t nbytes, hipStream_t s, bool async) {
+    hipError_t e = async ? hipMallocAsync(p, nbytes, s) : hipMalloc(p, nbytes);
+    if (e != hipErrorOutOfMemory && e != hipErrorMemoryAllocation) return e;
+    (void)hipGetLastError();
+    int dev = 0;
+    hipMemPool_t pool;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess || hipMemPoolTrimTo(pool, 0) != hipSuccess)
+        return e;
+    return async ? hipMallocAsync(p, nbytes, s) : hipMalloc(p, nbytes);
+}
+
 void DBuf::alloc(size_t nbytes, hipStream_t s) {
     release();
     stream = s;
     bytes = nbytes;
     if (nbytes == 0) return;
-    hipError_t e = hipMallocAsync(&p, nbytes, s);
+    hipError_t e = device_alloc(&p, nbytes, s, true);
     if (e != hipSuccess) {
         p = nullptr;
         bytes = 0;
@@ -492,7 +509,7 @@ void *Arena::alloc(size_t bytes) {
     if (chunks_.empty() || chunks_.back().cap - chunks_.back().used < bytes) {
         size_t cap = bytes > min_chunk_ ? bytes : min_chunk_;
         void *p = nullptr;
-        hipError_t e = stream_ ? hipMallocAsync(&p, cap, stream_) : hipMalloc(&p, cap);
+        hipError_t e = device_alloc(&p, cap, stream_, stream_ != nullptr);
         if (e != hipSuccess)
             throw Error(EGTB_ERR_MEMORY, "device allocation of " + std::to_string(cap) + " bytes failed: " +
                                              hipGetErrorString(e));
@@ -566,7 +583,7 @@ void *BookRing::alloc(int layer, size_t bytes) {
         if (cur_ < 0) {
             const size_t cap = std::max(bytes, chunk_);
             void *p = nullptr;
-            const hipError_t e = hipMallocAsync(&p, cap, stream_);
+            const hipError_t e = device_alloc(&p, cap, stream_, true);
             if (e != hipSuccess)
                 throw Error(EGTB_ERR_MEMORY, "device allocation of " + std::to_string(cap) +
                                                  " bytes for streamed books failed: " + hipGetErrorString(e));

@@ -50,6 +50,7 @@ RUNS = [
     ("442", 7, "float32", {"optimal_branch_only": True}),
     ("442", 7, "uint32", {"optimal_branch_only": True, "deletion_threshold": 0.3}),
     ("LL", 6, "float32", {"optimal_branch_only": True}),
+    ("444", 9),
 ]
 RATE_FORMAT = {"float32": "<f4", "uint32": "<u4", "float64": "<f8", "uint64": "<u8", "1-float32": "<f4",
                "1-float64": "<f8"}

@@ -67,3 +67,41 @@ def test_batch_key_sets_match_sorted_dedup(pkg):
                 ka, ra = a.book(j, i)
                 kb, rb = b.book(j, i)
                 assert np.array_equal(ka, kb) and ra.tobytes() == rb.tobytes(), (pat, t, i)
+
+
+def test_default_mix_matches_goldens_and_plain(pkg):
+    """The benchmark's 24-job mix (L3 / 442 / t at 64-1024, the 12-free-cell
+    patterns at 32-128) in one batched build: every job whose table has a
+    reference golden equals it on every layer; every other job equals its
+    plain build on three layers spread over its range."""
+    from importlib import import_module
+
+    from conftest import golden_tags
+
+    batched = import_module("2048endgametablebase_amd.batched")
+    jobs = batched.DEFAULT_JOBS
+    assert len(jobs) == 24
+    have = set(golden_tags())
+    print(f"case batched default mix: {len(jobs)} jobs", flush=True)
+    with batched.build_batch(jobs) as br:
+        checked = 0
+        for j, (pat, t) in enumerate(jobs):
+            tag = f"{pat}_{2 ** t}"
+            if tag in have:
+                print(f"case job {j} {tag}: golden", flush=True)
+                for layer in load_golden(tag)["layers"]:
+                    keys, rates = br.book(j, layer["i"])
+                    assert book_sha(keys, rates) == layer["sha256"], (tag, layer["i"])
+                checked += 1
+                continue
+            print(f"case job {j} {tag}: plain build", flush=True)
+            res = pkg.build_in_memory(pat, t)
+            try:
+                for i in (res.steps // 4, res.steps // 2, 3 * res.steps // 4):
+                    pk, pr = res.book(i)
+                    bk, brr = br.book(j, i)
+                    assert np.array_equal(pk, bk) and np.array_equal(pr.view(np.uint32), brr.view(np.uint32)), \
+                        (tag, i)
+            finally:
+                res.close()
+        assert checked >= 8

@@ -48,22 +48,73 @@ def test_host_tables_match_golden(pkg, tag):
         res.close()
 
 
-def test_444_2048_in_hbm_and_host_tables_agree(pkg):
-    """The 12-free-cell 444@2048 build at full size on one GPU: its layers
-    come out sorted and unique, and streamed to host memory they hold the
-    same records as kept in HBM. (Large layers cross the raw arena's 1 GiB
-    chunks here, which no smaller case does.)"""
+def _tile_sum_table():
     import numpy as np
+
+    t = np.zeros(1 << 16, dtype=np.int64)
+    for x in range(1 << 16):
+        t[x] = sum(1 << v for v in ((x >> 0) & 15, (x >> 4) & 15, (x >> 8) & 15, (x >> 12) & 15) if v)
+    return t
+
+
+def _mirror_lr(k):
+    import numpy as np
+
+    return (((k & np.uint64(0xF000F000F000F000)) >> np.uint64(12)) | ((k & np.uint64(0x0F000F000F000F00)) >> np.uint64(4))
+            | ((k & np.uint64(0x00F000F000F000F0)) << np.uint64(4)) | ((k & np.uint64(0x000F000F000F000F)) << np.uint64(12)))
+
+
+def test_444_2048_in_hbm_and_host_tables_agree(pkg, orc):
+    """The 12-free-cell 444@2048 build at full size on one GPU (7.1e9
+    states, 1084 layers).  Every layer of the build kept in HBM holds: keys
+    strictly ascending and in horizontal canonical form (layer 0: the seeds
+    as given), in the pattern (bottom row pinned), of tile sum tile_sum + 2i (on
+    every 8th record), and float32 rates in (0, 1] with every success board
+    of a layer past docheck at 1.  Five layers' live records are solved
+    again by the CPU oracle from the stored books i+1 and i+2 and must match
+    bit for bit.  Streamed to host memory the tables hold the same records
+    (five layers compared in full, every layer's counts)."""
+    import numpy as np
+
+    p = pkg.patterns.build_params("444", 11)
+    tile_sum = pkg.patterns.pattern_catalog["444"].tile_sum
+    tsum = _tile_sum_table()
     picks = [40, 300, 540, 800, 1060]
     r = pkg.build_in_memory("444", 11)
     try:
         info = [r.layer_info(i) for i in range(r.steps)]
         print(f"case 444@2048 in HBM: {r.steps} layers, {sum(x[0] for x in info)} states", flush=True)
         sha = {}
-        for i in picks:
+        solve_picks = []
+        for i in range(r.steps):
             keys, rates = r.book(i)
-            assert keys.size > 1 and bool(np.all(keys[1:] > keys[:-1])), i
-            sha[i] = book_sha(keys, rates)
+            assert keys.size == info[i][1], i
+            if keys.size == 0:
+                continue
+            if i > 0:
+                assert bool(np.all(keys[1:] > keys[:-1])), f"layer {i} not strictly ascending"
+            assert bool(np.all((keys & np.uint64(0xFFFF)) == np.uint64(0xFFFF))), f"layer {i}: off the pattern"
+            sub = keys[::8]
+            if i > 0:  # layer 0 holds the seeds as given
+                assert bool(np.all(sub <= _mirror_lr(sub))), f"layer {i}: not in horizontal canonical form"
+            ts = sum(tsum[((sub >> np.uint64(16 * q)) & np.uint64(0xFFFF)).astype(np.int64)] for q in range(4))
+            assert bool(np.all(ts == tile_sum + 2 * i)), f"layer {i}: tile sums"
+            assert bool(np.all((rates > 0) & (rates <= 1))), f"layer {i}: rates outside (0, 1]"
+            if i > p["docheck_step"]:
+                succ = (((keys >> np.uint64(20)) & np.uint64(15)) == 11) | (((keys >> np.uint64(24)) & np.uint64(15)) == 11)
+                assert bool(np.all(rates[succ] == 1.0)), f"layer {i}: success boards below 1"
+            if i in picks:
+                sha[i] = book_sha(keys, rates)
+            if 100_000 < keys.size < 2_000_000 and i < r.steps - 2:
+                solve_picks.append(i)
+        solve_picks = [solve_picks[j] for j in np.linspace(0, len(solve_picks) - 1, 5).astype(int)]
+        print(f"case 444@2048 local recompute of layers {solve_picks}", flush=True)
+        assert len(set(solve_picks)) == 5
+        for i in solve_picks:
+            keys, rates = r.book(i)
+            got = orc.solve_layer(keys, r.book(i + 1), r.book(i + 2), p["masks"], p["shifts"], p["symm_mode"], 11,
+                                  i > p["docheck_step"], threads=16)
+            assert np.array_equal(got.view(np.uint32), rates.view(np.uint32)), f"layer {i}: recomputed rates"
     finally:
         r.close()
     h = pkg.build_in_memory("444", 11, host_tables=True)
