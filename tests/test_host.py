@@ -57,7 +57,7 @@ def test_invalid_arguments_fail_before_device_use(pkg):
         pkg.run_pattern_build([1], list(range(17)), [24], 0, 9, 304, 252, "", write_files=False)
     with pytest.raises(pkg.native.EgtbError, match="success_rate_dtype"):
         pkg.run_pattern_build([1], [0xFFFF], [24], 0, 9, 304, 252, "", write_files=False,
-                              success_rate_dtype="1-float32")
+                              success_rate_dtype="2-float32")
     assert "success_rate_dtype" in pkg.native.last_error()
 
 
