@@ -51,7 +51,8 @@ class _Args:
     def __init__(self, seeds, masks, shifts, symm, target, steps, docheck, pathname, is_free,
                  spawn_rate4, dtype, write_files, keep_raw, device, radix_bits, sort_schedule=0,
                  chunk_mkeys=0, host_tables=False, write_raw_files=False, deletion_threshold=0.0,
-                 solve_hash_mb=0, forward_hash_steps=0, solve_hash_layout=0, optimal_branch_only=False):
+                 solve_hash_mb=0, forward_hash_steps=0, solve_hash_layout=0, optimal_branch_only=False,
+                 compress=False):
         self.seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
         self.masks = np.ascontiguousarray(np.asarray(list(masks) or [0], dtype=np.uint64))
         self.shifts = np.ascontiguousarray(np.asarray(list(shifts) or [0], dtype=np.uint8))
@@ -82,6 +83,7 @@ class _Args:
         o.forward_hash_steps = int(forward_hash_steps)
         o.solve_hash_layout = int(solve_hash_layout)
         o.optimal_branch_only = int(bool(optimal_branch_only))
+        o.compress = int(bool(compress))
 
 
 class BuildResult:
@@ -168,7 +170,7 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
                       host_tables: bool = False, write_raw_files: int = 0,
                       deletion_threshold: float = 0.0, solve_hash_mb: int = 0,
                       forward_hash_steps: int = 0, solve_hash_layout: int = 0,
-                      optimal_branch_only: bool = False) -> Optional[BuildResult]:
+                      optimal_branch_only: bool = False, compress: bool = False) -> Optional[BuildResult]:
     """The native build call (reference formation_core.run_pattern_build,
     native_core/src/bindings_formation.cpp:220-229) with the PatternSpec and
     RunOptions fields spelled out.  ``keep=True`` returns the solved layers
@@ -178,7 +180,7 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
     a = _Args(arr_init, masks, shifts, symm_mode, target, steps, docheck_step, pathname, is_free,
               spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits, sort_schedule,
               chunk_mkeys, host_tables, write_raw_files, deletion_threshold, solve_hash_mb, forward_hash_steps,
-              solve_hash_layout, optimal_branch_only)
+              solve_hash_layout, optimal_branch_only, compress)
     L = native.lib()
     seeds_p = a.seeds.ctypes.data_as(native.u64p)
     if keep or keep_raw:
@@ -299,7 +301,9 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
         pathname, is_free=p["is_free"], spawn_rate4=float(cfg["4_spawn_rate"]), success_rate_dtype=dtype,
         write_files=True, device=int(cfg.get("device", 0)),
         write_raw_files=bool(cfg.get("write_raw_files", False)),
-        deletion_threshold=float(cfg.get("deletion_threshold", 0.0))))
+        deletion_threshold=float(cfg.get("deletion_threshold", 0.0)),
+        optimal_branch_only=bool(cfg.get("optimal_branch_only", False)),
+        compress=bool(cfg.get("compress", False))))
     return True
 
 

@@ -111,7 +111,7 @@ class Table:
         meta = pattern_catalog[pattern]
         if steps is None:
             folder, base = os.path.split(prefix)
-            rx = re.compile(re.escape(base) + r"(\d+)\.book$")
+            rx = re.compile(re.escape(base) + r"(\d+)\.(?:book|z)$")
             found = [int(m.group(1)) for f in os.listdir(folder or ".") if (m := rx.match(f))]
             steps = max(found) + 1 if found else 0
         masks = np.asarray(meta.pattern_masks or (0,), dtype=np.uint64)
@@ -305,7 +305,10 @@ class BookReader:
         for folder, dtype in path_list:
             if not os.path.exists(folder):
                 continue
-            if not os.path.exists(os.path.join(folder, f"{pattern_full}_{nums}.book")):
+            # the layer as a .book or in its compressed form (a .z directory;
+            # the same check as ReaderRuntime.cpp:482-498,765)
+            stem = os.path.join(folder, f"{pattern_full}_{nums}")
+            if not os.path.exists(stem + ".book") and not os.path.isdir(stem + ".z"):
                 continue
             res = cls._table(folder, pattern_full, dtype).move_on_dic_batch([b])[0]
             if res[1]:

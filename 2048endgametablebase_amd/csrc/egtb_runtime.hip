@@ -28,6 +28,7 @@
 #include "egtb.h"
 #include "egtb_kernels.h"
 #include "egtb_runtime.h"
+#include "egtb_zbook.h"
 
 namespace egtb {
 
@@ -2394,6 +2395,33 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         if (files) remove_if_exists(optpath);
         if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     }
+    // ---- compressed books (reference maybe_do_compress_classic,
+    // CompressionBridge.cpp:41-60): every final .book above 2 MiB, except
+    // layers 0-2, becomes a <pathname><i>.z directory (trie index + xz
+    // blocks) and the .book is removed
+    if (files && o.compress) {
+        constexpr size_t rec = 8 + sizeof(T);
+        for (int i = 3; i < steps; ++i) {
+            const std::string bp = prefix + std::to_string(i) + ".book";
+            struct stat st;
+            if (stat(bp.c_str(), &st) != 0 || (uint64_t)st.st_size <= 2097152ull || st.st_size % rec) continue;
+            const size_t n = (size_t)st.st_size / rec;
+            stage.ensure((size_t)st.st_size + 16);
+            FILE *f = std::fopen(bp.c_str(), "rb");
+            if (!f) throw Error(EGTB_ERR_IO, "cannot open " + bp);
+            const size_t got = std::fread(stage.p, 1, (size_t)st.st_size, f);
+            std::fclose(f);
+            if (got != (size_t)st.st_size) throw Error(EGTB_ERR_IO, "short read of " + bp);
+            std::vector<uint64_t> keys(n);
+            std::vector<uint8_t> rts(n * sizeof(T));
+            for (size_t r = 0; r < n; ++r) {
+                std::memcpy(&keys[r], (const char *)stage.p + r * rec, 8);
+                std::memcpy(rts.data() + r * sizeof(T), (const char *)stage.p + r * rec + 8, sizeof(T));
+            }
+            zbook_write(bp, keys.data(), rts.data(), n, (int)sizeof(T));
+            remove_if_exists(bp);
+        }
+    }
     for (int i = 0; i < steps; ++i) {
         res.info[i].n_live = H[i].n_live;
         res.books[i].n = H[i].n_live;
@@ -2669,6 +2697,35 @@ int egtb_bench_sort(const uint64_t *keys, size_t n, int bits, int schedule, int 
         (void)hipEventDestroy(e1);
         if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
         *ms_per_sort = total / reps;
+    });
+}
+
+int egtb_zbook_read(const char *book_path, int rate_size, uint64_t *keys, void *rates, size_t cap, size_t *n) {
+    return guarded([&] {
+        if (!book_path || !n) throw Error(EGTB_ERR_ARG, "book_path and n must not be NULL");
+        if (rate_size != 4 && rate_size != 8) throw Error(EGTB_ERR_ARG, "rate_size must be 4 or 8");
+        const std::string bp = book_path;
+        if (bp.size() < 5 || bp.compare(bp.size() - 5, 5, ".book") != 0)
+            throw Error(EGTB_ERR_ARG, "book_path must end in .book");
+        ZBook z = zbook_read(bp, rate_size);
+        *n = z.keys.size();
+        if (!keys && !rates) return;
+        if (cap < z.keys.size()) throw Error(EGTB_ERR_ARG, "buffers hold fewer records than the book");
+        if (keys && !z.keys.empty()) std::memcpy(keys, z.keys.data(), z.keys.size() * 8);
+        if (rates && !z.rates.empty()) std::memcpy(rates, z.rates.data(), z.rates.size());
+    });
+}
+
+int egtb_zbook_write(const char *book_path, const uint64_t *keys, const void *rates, size_t n, int rate_size) {
+    return guarded([&] {
+        if (!book_path || !keys || !rates) throw Error(EGTB_ERR_ARG, "book_path, keys and rates must not be NULL");
+        if (rate_size != 4 && rate_size != 8) throw Error(EGTB_ERR_ARG, "rate_size must be 4 or 8");
+        const std::string bp = book_path;
+        if (bp.size() < 5 || bp.compare(bp.size() - 5, 5, ".book") != 0)
+            throw Error(EGTB_ERR_ARG, "book_path must end in .book");
+        for (size_t r = 1; r < n; ++r)
+            if (keys[r] <= keys[r - 1]) throw Error(EGTB_ERR_ARG, "compressed books hold strictly ascending keys");
+        zbook_write(bp, keys, rates, n, rate_size);
     });
 }
 

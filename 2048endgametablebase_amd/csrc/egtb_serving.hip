@@ -24,6 +24,7 @@
 #include "egtb_kernels.h"
 #include "egtb_runtime.h"
 #include "egtb_serving.h"
+#include "egtb_zbook.h"
 
 namespace egtb {
 
@@ -285,6 +286,31 @@ std::unique_ptr<TableSet> table_from_files(const char *prefix, int steps, const 
         t.bits = 1;
         const std::string path = std::string(prefix) + std::to_string(i) + ".book";
         FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f && zbook_exists(path)) {
+            // a compressed layer (<pathname><i>.z): decoded on the host, then
+            // resident in HBM like a .book (ReaderRuntime.cpp:318-476 reads
+            // it block by block per probe instead)
+            t.present = 1;
+            ZBook z = zbook_read(path, dd.size);
+            const uint64_t n = z.keys.size();
+            ts->n[i] = n;
+            if (n == 0) continue;
+            uint64_t *keys = (uint64_t *)ts->storage.alloc(n * 8);
+            void *rates = ts->storage.alloc(n * dd.size);
+            HIPCHK(hipMemcpyAsync(keys, z.keys.data(), n * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(rates, z.rates.data(), n * dd.size, hipMemcpyHostToDevice, s));
+            const uint32_t bits = table_index_bits(n);
+            uint32_t *idx = (uint32_t *)ts->storage.alloc(((1ull << bits) + 1) * 4);
+            launch_bucket_index(s, keys, n, nullptr, bits, idx);
+            t.keys = keys;
+            t.rates = (const char *)rates;
+            t.idx = idx;
+            t.n_ptr = nullptr;
+            t.n = n;
+            t.bits = bits;
+            HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+            continue;
+        }
         if (!f) continue;  // absent layer: lookups there read as off-table
         t.present = 1;
         std::fseek(f, 0, SEEK_END);

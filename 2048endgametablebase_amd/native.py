@@ -60,6 +60,8 @@ class Options(ctypes.Structure):
         ("forward_hash_steps", ctypes.c_int32),
         ("solve_hash_layout", ctypes.c_int32),
         ("optimal_branch_only", ctypes.c_int32),
+        ("compress", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
     ]
 
 
@@ -158,6 +160,9 @@ SIGNATURES = {
     "egtb_table_set_prefer_max": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "egtb_sort_unique_u64": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, u64p,
                                              ctypes.POINTER(ctypes.c_size_t)]),
+    "egtb_zbook_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, u64p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+    "egtb_zbook_write": (ctypes.c_int, [ctypes.c_char_p, u64p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]),
     "egtb_board_moves": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int32, u64p, u64p, ctypes.c_int]),
     "egtb_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "egtb_server_attach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),

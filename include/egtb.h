@@ -80,6 +80,10 @@ typedef struct egtb_options {
     int32_t optimal_branch_only;     /* RunOptions::optimal_branch_only: after the solve, book i (i > 20)
                                         keeps only the records some state of layers i-1 / i-2 reaches as its
                                         best move (BookSolver.cpp:611-735) */
+    int32_t compress;                /* RunOptions::compress: at the end of a build with files, every book
+                                        above 2 MiB except layers 0-2 is replaced by its compressed form
+                                        <pathname><i>.z (CompressionBridge.cpp:41-60) */
+    int32_t reserved0;
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file
@@ -300,6 +304,25 @@ typedef struct egtb_loadgen_report {
 } egtb_loadgen_report;
 
 EGTB_EXPORT int egtb_server_loadgen(egtb_server *s, const egtb_loadgen_options *options, egtb_loadgen_report *report);
+
+/* ---- compressed books: the reference's .z layout ------------------------- */
+
+/* Records of the compressed form of a book: book_path is the .book path
+ * (<pathname><i>.book); its compressed form is the directory <pathname><i>.z
+ * holding the trie index (<stem>i, <stem>ii), the block table (<stem>s) and
+ * the xz blocks (<stem>z).  With keys and rates NULL only *n is set; else
+ * up to cap records are copied (keys ascending, rates of rate_size 4 or 8
+ * bytes, raw as stored).  Host work, no device.  Replaces reading a .z
+ * layer per probe (trie_search_with_context, ReaderRuntime.cpp:318-476);
+ * egtb_table_open reads compressed layers through it. */
+EGTB_EXPORT int egtb_zbook_read(const char *book_path, int rate_size, uint64_t *keys, void *rates, size_t cap,
+                                size_t *n);
+/* Writes the compressed form of a book of n strictly ascending records (the
+ * .book itself is left alone).  Same layout as trie_compress_progress_native
+ * (TrieCompression.cpp:124-551): blocks of about 32768 records, xz preset 1
+ * with CRC64, compressed on host threads.  Host work, no device. */
+EGTB_EXPORT int egtb_zbook_write(const char *book_path, const uint64_t *keys, const void *rates, size_t n,
+                                 int rate_size);
 
 /* ---- dedup primitives: drop-in for the reference's bookgen_native.so ----- */
 

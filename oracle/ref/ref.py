@@ -62,6 +62,11 @@ def lib():
                       ctypes.c_int64] + res
         L.egtb_ref_reader_free.restype = None
         L.egtb_ref_reader_free.argtypes = [ctypes.c_void_p]
+        L.egtb_ref_trie_compress.restype = ctypes.c_int
+        L.egtb_ref_trie_compress.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.egtb_ref_trie_search.restype = ctypes.c_int
+        L.egtb_ref_trie_search.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p,
+                                           ctypes.POINTER(ctypes.c_double)]
         f = L.egtb_ref_find_classic_value
         f.restype = ctypes.c_int
         f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p,
@@ -170,3 +175,19 @@ def find_classic_value(pathname, filename, key, dtype):
                                          ctypes.byref(v), ctypes.byref(f)):
         raise RuntimeError("find_classic_value_native raised")
     return v.value, bool(f.value)
+
+
+def trie_compress(book_path, dtype):
+    """trie_compress_progress_native: writes <book without .book>.z/ beside it."""
+    rc = lib().egtb_ref_trie_compress(book_path.encode(), dtype.encode())
+    if rc:
+        raise RuntimeError(f"trie_compress_progress_native failed ({rc})")
+
+
+def trie_search(path_prefix, board, dtype):
+    """trie_decompress_search_native: the normalised rate, or None."""
+    v = ctypes.c_double()
+    rc = lib().egtb_ref_trie_search(path_prefix.encode(), int(board), dtype.encode(), ctypes.byref(v))
+    if rc < 0:
+        raise RuntimeError("trie_decompress_search_native raised")
+    return v.value if rc else None

@@ -123,3 +123,28 @@ extern "C" int egtb_ref_find_classic_value(const char *pathname, const char *fil
         return -1;
     }
 }
+
+// The reference's .z compression of one .book (trie_compress_progress_native,
+// TrieCompression.cpp:538-551; the .book is left in place) and its per-probe
+// search of a compressed layer (trie_decompress_search_native): 1 found with
+// the normalised rate in *value, 0 when the search gives no value.
+#include "TrieCompression.h"
+
+extern "C" int egtb_ref_trie_compress(const char *book_path, const char *dtype) {
+    try {
+        return trie_compress_progress_native(book_path, dtype) ? 0 : -1;
+    } catch (...) {
+        return -2;
+    }
+}
+
+extern "C" int egtb_ref_trie_search(const char *path_prefix, uint64_t board, const char *dtype, double *value) {
+    try {
+        const std::optional<double> v = trie_decompress_search_native(path_prefix, board, dtype);
+        if (!v) return 0;
+        *value = *v;
+        return 1;
+    } catch (...) {
+        return -1;
+    }
+}

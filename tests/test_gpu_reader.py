@@ -263,3 +263,47 @@ def test_book_reader_uint32_matches_reference_reader_fixture(pkg, built_u32):
             cfg["filepath_map"][key] = old
     bad = same_answers(fx, got)
     assert not bad, (len(bad), bad[:3])
+
+
+def test_compressed_tables_serve_the_reference_answers(pkg, tmp_path):
+    """A 442@512 build with compress on: every book above 2 MiB (but layers
+    0-2) is replaced by its .z directory, each layer decodes to the
+    reference's .book bytes, and a table opened over the compressed files
+    answers the reference reader's 442@512 fixture positions exactly."""
+    import ctypes
+
+    from conftest import book_sha, load_golden
+
+    g = load_golden("442_512")
+    p = pkg.patterns.build_params("442", 9)
+    prefix = str(tmp_path / "442_512_")
+    print("case compressed 442@512", flush=True)
+    pkg.run_pattern_build(p["seeds"], p["masks"], p["shifts"], p["symm_mode"], 9, p["steps"], p["docheck_step"],
+                          prefix, success_rate_dtype="float32", write_files=True, compress=True)
+    L = pkg.native.lib()
+    n_z = 0
+    for layer in g["layers"]:
+        i = layer["i"]
+        book, zdir = prefix + f"{i}.book", prefix + f"{i}.z"
+        if os.path.isdir(zdir):
+            assert not os.path.exists(book) and i > 2
+            n = ctypes.c_size_t()
+            pkg.native.check(L.egtb_zbook_read(book.encode(), 4, None, None, 0, ctypes.byref(n)))
+            keys = np.empty(n.value, dtype=np.uint64)
+            rates = np.empty(n.value, dtype=np.float32)
+            pkg.native.check(L.egtb_zbook_read(book.encode(), 4, keys.ctypes.data_as(pkg.native.u64p),
+                                               rates.ctypes.data, n.value, ctypes.byref(n)))
+            assert book_sha(keys, rates) == layer["sha256"], i
+            n_z += 1
+        else:
+            assert os.path.getsize(book) <= 2 * 1024 * 1024 or i <= 2, i
+    assert n_z > 100
+    fx = load_reader_fixture("442_512")
+    t = pkg.Table.open(prefix, "442", 9, "float32")
+    try:
+        got = t.move_on_dic_batch([importlib.import_module("2048endgametablebase_amd.BookReader").encode_board(m)
+                                   for m, _ in fx["expected"]])
+        bad = same_answers(fx, got)
+        assert not bad, (len(bad), bad[:3])
+    finally:
+        t.close()
