@@ -1431,95 +1431,74 @@ static void read_meta(hipStream_t s, Result &res, LayerDev *host, int lo, int hi
     HIPCHK(hipStreamSynchronize(s));
 }
 
+// Destroys an event on the way out of a scope.
+struct EventGuard {
+    hipEvent_t e;
+    ~EventGuard() { (void)hipEventDestroy(e); }
+};
+
+// What the stages of one build share: the device, options and pattern,
+// the rate type's scale and zero, the per-layer records (pinned host mirror
+// H, device M), the resume state and the scratch one stage hands the next.
+template <typename T>
+struct BuildCtx {
+    DeviceCtx &C;
+    hipStream_t s;
+    const egtb_options &o;
+    const PatternDev &p;
+    Result &res;
+    const int steps;
+    const bool files;
+    const std::string prefix;
+    // the rate type's scale and zero (FormationRuntime.h:261-303): the
+    // 1-float types store rate - 1, so success is 0 and a miss is -1
+    const T max_scale, zero;
+    const double w4, w2;
+    const uint32_t prog_total;
+    std::unique_ptr<BookStreamer> streamer;
+    SortWork w;
+    HostStage stage, meta_host_buf;
+    DBuf rates, packed;
+    LayerDev *H = nullptr, *M = nullptr;
+    std::vector<char> have_book;
+    std::unique_ptr<RawFileWriter> raw_writer;
+    std::chrono::steady_clock::time_point t_begin, t_fwd;
+    bool two_stream = false;  // the backward pass's schedule
+    uint64_t max_n = 0;       // largest layer
+
+    BuildCtx(const egtb_options &o_, const PatternDev &p_, Result &res_, bool one_minus)
+        : C(device_ctx(o_.device)), s(C.stream), o(o_), p(p_), res(res_), steps(o_.steps),
+          files(o_.write_files != 0), prefix(o_.pathname ? o_.pathname : ""),
+          max_scale(one_minus ? T(0) : RateTraits<T>::max_scale()), zero(one_minus ? T(-1) : T(0)),
+          w4(o_.spawn_rate4), w2(1.0 - o_.spawn_rate4), prog_total((uint32_t)(2 * o_.steps)), have_book(o_.steps, 0) {}
+
+    // raw layer file of layer L (write_raw_files), after `producer`'s work
+    void queue_raw(int L, uint64_t n, hipStream_t producer = nullptr) {
+        if (raw_writer && !have_book[L])
+            raw_writer->write(prefix + std::to_string(L), res.raw[L], n, producer ? producer : s);
+    }
+};
+
 // The plain per-layer schedule.  Forward: one host synchronisation per step
 // (after the count pass, to size the candidate streams); the spawn-4 stream
 // of step i lands at the front of the candidate buffer of layer i+1 and the
 // spawn-2 stream of step i+1 is appended behind it, so layer i+1 is one sort
 // + unique of that buffer.  Backward: sizes stay on the device, no sync.
 template <typename T>
-void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const egtb_options &o, Result &res,
-               bool one_minus) {
-    DeviceCtx &C = device_ctx(o.device);
-    hipStream_t s = C.stream;
-    const int steps = o.steps;
-    const bool files = o.write_files != 0;
-    const std::string prefix = o.pathname ? o.pathname : "";
-    // the rate type's scale and zero (FormationRuntime.h:261-303): the
-    // 1-float types store rate - 1, so success is 0 and a miss is -1
-    const T max_scale = one_minus ? T(0) : RateTraits<T>::max_scale();
-    const T zero = one_minus ? T(-1) : T(0);
-    const double w4 = o.spawn_rate4, w2 = 1.0 - o.spawn_rate4;
-    const uint32_t prog_total = (uint32_t)(2 * steps);
-    progress_set(0, prog_total);
-
-    res.dev = o.device;
-    res.steps = steps;
-    res.rate_size = sizeof(T);
-    res.rate_is_float = std::is_floating_point<T>::value;
-    res.raw.assign(steps, nullptr);
-    res.books.assign(steps, Book{});
-    res.info.assign(steps, LayerInfo{});
-    if (o.host_tables) res.raw_arena.set_stream(s);
-    res.host_tables = o.host_tables != 0;
-    std::unique_ptr<BookStreamer> streamer;
-    if (res.host_tables) {
-        if (files) throw Error(EGTB_ERR_ARG, "host_tables and write_files cannot be combined");
-        res.book_ring.reset(s, steps);
-        res.host_keys.clear();
-        res.host_rates.clear();
-        res.host_keys.resize(steps);
-        res.host_rates.resize(steps);
-        streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T), steps);
-    }
-    res.hash_bufs.clear();
-    res.hash_bufs.resize(4);
-    res.hash_ring = 3;
-    res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
-                         : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
-    res.rate_size = (int)sizeof(T);
-    res.rate_zero = static_cast<double>(zero);
-    if (o.optimal_branch_only && o.host_tables)
-        throw Error(EGTB_ERR_ARG, "optimal_branch_only needs the books in HBM (host_tables = 0)");
-    // compact tables are 4-8x smaller but take more probe rounds per
-    // lookup: 442@512 k_solve 101.6 us a launch against 72 us wide
-    res.compact_hashes = o.solve_hash_layout == 1;
-    res.hash_err.alloc(256, s);
-    HIPCHK(hipMemsetAsync(res.hash_err.p, 0, 256, s));
-    SortWork w;
-    HostStage stage, meta_host_buf;
-    DBuf cand[2], sort_tmp, counts, offs, rates, packed;
-
-    const auto t_begin = std::chrono::steady_clock::now();
-
-    // per-layer device records: zero sizes, OR = 0, AND = all ones
-    meta_host_buf.ensure((size_t)(steps + 1) * sizeof(LayerDev));
-    LayerDev *H = (LayerDev *)meta_host_buf.p;
-    for (int i = 0; i <= steps; ++i) {
-        std::memset(&H[i], 0, sizeof(LayerDev));
-        H[i].and_all = ~0ull;
-    }
-    H[0].n_raw = n_seeds;
-    res.meta.alloc((size_t)(steps + 1) * sizeof(LayerDev), s);
-    HIPCHK(hipMemcpyAsync(res.meta.p, H, (size_t)(steps + 1) * sizeof(LayerDev), hipMemcpyHostToDevice, s));
-
-    // resume (reference handle_restart / handle_restart_recalculate): drop
-    // invalid leftovers, note which layers are already solved on disk
-    std::vector<char> have_book(steps, 0);
-    std::unique_ptr<RawFileWriter> raw_writer;
-    if (files) {
-        for (int i = 0; i < steps; ++i) {
-            const std::string bp = prefix + std::to_string(i) + ".book";
-            drop_invalid_restart_file(bp, 8 + sizeof(T));
-            drop_invalid_restart_file(prefix + std::to_string(i), 8);
-            have_book[i] = valid_restart_file(bp, 8 + sizeof(T)) ? 1 : 0;
-        }
-        if (o.write_raw_files) raw_writer = std::make_unique<RawFileWriter>(o.device);
-    }
-    auto queue_raw = [&](int L, uint64_t n, hipStream_t producer = nullptr) {
-        if (raw_writer && !have_book[L])
-            raw_writer->write(prefix + std::to_string(L), res.raw[L], n, producer ? producer : s);
-    };
-    LayerDev *M = res.meta_dev();
+static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) {
+    DeviceCtx &C = b.C;
+    hipStream_t s = b.s;
+    const egtb_options &o = b.o;
+    const PatternDev &p = b.p;
+    Result &res = b.res;
+    const int steps = b.steps;
+    const uint32_t prog_total = b.prog_total;
+    std::unique_ptr<BookStreamer> &streamer = b.streamer;
+    SortWork &w = b.w;
+    LayerDev *const H = b.H;
+    LayerDev *const M = b.M;
+    auto queue_raw = [&](int L, uint64_t n, hipStream_t producer = nullptr) { b.queue_raw(L, n, producer); };
+    DBuf cand[2], sort_tmp, counts, offs;
 
     // ---- forward (reference generate_process, BookGenerator.cpp:963-1121)
     res.raw[0] = (uint64_t *)res.raw_arena.alloc(n_seeds * 8);
@@ -1623,10 +1602,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // pass and per-parent offsets instead.
     hipEvent_t meta_ev;
     HIPCHK(hipEventCreateWithFlags(&meta_ev, hipEventDisableTiming));
-    struct EventGuard {
-        hipEvent_t e;
-        ~EventGuard() { (void)hipEventDestroy(e); }
-    } meta_ev_guard{meta_ev};
+    EventGuard meta_ev_guard{meta_ev};
     auto queue_meta = [&](int lo, int hi) {
         HIPCHK(hipMemcpyAsync(H + lo, res.meta_dev() + lo, (size_t)(hi - lo + 1) * sizeof(LayerDev),
                               hipMemcpyDeviceToHost, s));
@@ -2112,35 +2088,35 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             res.info[i].key_and = H[i].and_all;
         }
     }
-    const auto t_fwd = std::chrono::steady_clock::now();
+    // the second stream's sorts carry their own look-back error flag (the
+    // small unique layers are sorted there with the single-pass look-back):
+    // a raised flag means a partial prefix was used
+    if (w_side.read_error(s2)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
+}
 
-    // when every layer gets a solve hash the solve never searches, and the
-    // books go without the bucket index (a table set over the result makes
-    // its own)
-    {
-        bool all_hashed = res.hash_max_bytes > 0;
-        for (int i = 1; i < steps && all_hashed; ++i) all_hashed = hash_layout(res, i, res.info[i].n_raw).present();
-        res.book_index = !all_hashed;
-    }
-
-    // Two streams for the backward pass of an in-HBM build: each solve
-    // fills its own layer's hash, so the compaction that makes the book
-    // leaves the chain of solves and runs on the second stream; every layer
-    // keeps its own rates until its compaction has read them.
-    uint64_t rates_all_bytes = 0;
-    for (int i = 0; i < steps; ++i) rates_all_bytes += (res.info[i].n_raw * sizeof(T) + 255) & ~(uint64_t)255;
-    // the per-layer rates must also fit in half the free HBM, so a build that
-    // fits with one stream never fails for want of the second's buffers
-    size_t hbm_free = 0, hbm_total = 0;
-    HIPCHK(hipMemGetInfo(&hbm_free, &hbm_total));
-    const bool two_stream = steps >= 3 && !files && !streamer && !res.book_index && o.deletion_threshold <= 0.0 &&
-                            (double)rates_all_bytes <= kTwoStreamRatesMax &&
-                            rates_all_bytes + 256 <= (uint64_t)hbm_free / 2;
-    if (two_stream) res.hash_ring = 4;
-
+// Final layers and the stats header of a build with files.
+template <typename T>
+static void final_layers(BuildCtx<T> &b) {
+    hipStream_t s = b.s;
+    const egtb_options &o = b.o;
+    const PatternDev &p = b.p;
+    Result &res = b.res;
+    const int steps = b.steps;
+    const bool files = b.files;
+    const std::string &prefix = b.prefix;
+    const T max_scale = b.max_scale;
+    const T zero = b.zero;
+    SortWork &w = b.w;
+    HostStage &stage = b.stage;
+    DBuf &rates = b.rates;
+    DBuf &packed = b.packed;
+    LayerDev *const H = b.H;
+    std::vector<char> &have_book = b.have_book;
+    std::unique_ptr<RawFileWriter> &raw_writer = b.raw_writer;
     // ---- final layers (reference final_steps, BookSolver.cpp:237-284)
 
-    uint64_t max_n = 0;
+    uint64_t &max_n = b.max_n;
+    max_n = 0;
     for (int i = 0; i < steps; ++i) max_n = std::max(max_n, res.info[i].n_raw);
     rates.ensure(max_n * sizeof(T) + 8, s);
     for (int i = steps - 2; i <= steps - 1; ++i) {
@@ -2164,6 +2140,34 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         }
     }
 
+}
+
+// The backward pass, on two streams or one (the schedule b.two_stream
+// names), then the checks of the flags its kernels raise.
+template <typename T>
+static void backward_pass(BuildCtx<T> &b) {
+    DeviceCtx &C = b.C;
+    hipStream_t s = b.s;
+    const egtb_options &o = b.o;
+    const PatternDev &p = b.p;
+    Result &res = b.res;
+    const int steps = b.steps;
+    const bool files = b.files;
+    const std::string &prefix = b.prefix;
+    const T max_scale = b.max_scale;
+    const T zero = b.zero;
+    const double w4 = b.w4;
+    const double w2 = b.w2;
+    const uint32_t prog_total = b.prog_total;
+    std::unique_ptr<BookStreamer> &streamer = b.streamer;
+    SortWork &w = b.w;
+    HostStage &stage = b.stage;
+    DBuf &rates = b.rates;
+    DBuf &packed = b.packed;
+    LayerDev *const H = b.H;
+    std::vector<char> &have_book = b.have_book;
+    const bool two_stream = b.two_stream;
+    const uint64_t max_n = b.max_n;
     // ---- backward (reference recalculate_process_impl, BookSolver.cpp:513-605)
     if (two_stream) {
         // layer i: the solve (build stream) reads the hashes of layers i+1
@@ -2308,11 +2312,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         HIPCHK(hipStreamSynchronize(s));
         if (hipDeviceGetDefaultMemPool(&pool, res.dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
     }
-    // both sort states carry a look-back error flag: the build stream's and
-    // the second stream's (which sorts the small unique layers with the
-    // single-pass look-back); a raised flag means a partial prefix was used
-    if (w.read_error(s) || w_side.read_error(s2))
-        throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
+    // the build stream's look-back error flag (the second stream's is read
+    // at the end of the forward pass): a raised flag means a partial prefix
+    // was used
+    if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     {
         uint32_t herr = 0;
         HIPCHK(hipMemcpyAsync(&herr, res.hash_err.p, 4, hipMemcpyDeviceToHost, s));
@@ -2324,6 +2327,23 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     }
     read_meta(s, res, H, 0, steps - 1);
 
+}
+
+template <typename T>
+static void optimal_branch_pass(BuildCtx<T> &b) {
+    DeviceCtx &C = b.C;
+    hipStream_t s = b.s;
+    const egtb_options &o = b.o;
+    const PatternDev &p = b.p;
+    Result &res = b.res;
+    const int steps = b.steps;
+    const bool files = b.files;
+    const std::string &prefix = b.prefix;
+    const T zero = b.zero;
+    SortWork &w = b.w;
+    HostStage &stage = b.stage;
+    DBuf &packed = b.packed;
+    LayerDev *const H = b.H;
     // ---- optimal branches only (reference keep_only_optimal_branches_impl,
     // BookSolver.cpp:687-735): from layer 21 up, book i keeps the records
     // that some state of book i-2 (spawning a 4) or of book i-1 (spawning a
@@ -2395,6 +2415,15 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         if (files) remove_if_exists(optpath);
         if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back kernel exceeded its spin limit");
     }
+}
+
+template <typename T>
+static void compress_pass(BuildCtx<T> &b) {
+    const egtb_options &o = b.o;
+    const int steps = b.steps;
+    const bool files = b.files;
+    const std::string &prefix = b.prefix;
+    HostStage &stage = b.stage;
     // ---- compressed books (reference maybe_do_compress_classic,
     // CompressionBridge.cpp:41-60): every final .book above 2 MiB, except
     // layers 0-2, becomes a <pathname><i>.z directory (trie index + xz
@@ -2422,6 +2451,110 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
             remove_if_exists(bp);
         }
     }
+}
+
+// One build: set-up and resume scan, the forward pass, the choice of the
+// backward schedule, the final layers, the backward pass, the optional
+// post-solve passes (optimal branches, compression) and the timings.
+template <typename T>
+void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const egtb_options &o, Result &res,
+               bool one_minus) {
+    BuildCtx<T> b(o, p, res, one_minus);
+    hipStream_t s = b.s;
+    const int steps = b.steps;
+    const bool files = b.files;
+    const T zero = b.zero;
+    progress_set(0, b.prog_total);
+
+    res.dev = o.device;
+    res.steps = steps;
+    res.rate_size = sizeof(T);
+    res.rate_is_float = std::is_floating_point<T>::value;
+    res.raw.assign(steps, nullptr);
+    res.books.assign(steps, Book{});
+    res.info.assign(steps, LayerInfo{});
+    if (o.host_tables) res.raw_arena.set_stream(s);
+    res.host_tables = o.host_tables != 0;
+    if (res.host_tables) {
+        if (files) throw Error(EGTB_ERR_ARG, "host_tables and write_files cannot be combined");
+        res.book_ring.reset(s, steps);
+        res.host_keys.clear();
+        res.host_rates.clear();
+        res.host_keys.resize(steps);
+        res.host_rates.resize(steps);
+        b.streamer = std::make_unique<BookStreamer>(res, (int)sizeof(T), steps);
+    }
+    res.hash_bufs.clear();
+    res.hash_bufs.resize(4);
+    res.hash_ring = 3;
+    res.hash_max_bytes = o.solve_hash_mb < 0 ? 0.0
+                         : (double)(o.solve_hash_mb ? o.solve_hash_mb : kDefaultSolveHashMB) * (1 << 20);
+    res.rate_size = (int)sizeof(T);
+    res.rate_zero = static_cast<double>(zero);
+    if (o.optimal_branch_only && o.host_tables)
+        throw Error(EGTB_ERR_ARG, "optimal_branch_only needs the books in HBM (host_tables = 0)");
+    // compact tables are 4-8x smaller but take more probe rounds per
+    // lookup: 442@512 k_solve 101.6 us a launch against 72 us wide
+    res.compact_hashes = o.solve_hash_layout == 1;
+    res.hash_err.alloc(256, s);
+    HIPCHK(hipMemsetAsync(res.hash_err.p, 0, 256, s));
+
+    b.t_begin = std::chrono::steady_clock::now();
+
+    // per-layer device records: zero sizes, OR = 0, AND = all ones
+    b.meta_host_buf.ensure((size_t)(steps + 1) * sizeof(LayerDev));
+    LayerDev *H = b.H = (LayerDev *)b.meta_host_buf.p;
+    for (int i = 0; i <= steps; ++i) {
+        std::memset(&H[i], 0, sizeof(LayerDev));
+        H[i].and_all = ~0ull;
+    }
+    H[0].n_raw = n_seeds;
+    res.meta.alloc((size_t)(steps + 1) * sizeof(LayerDev), s);
+    HIPCHK(hipMemcpyAsync(res.meta.p, H, (size_t)(steps + 1) * sizeof(LayerDev), hipMemcpyHostToDevice, s));
+
+    // resume (reference handle_restart / handle_restart_recalculate): drop
+    // invalid leftovers, note which layers are already solved on disk
+    if (files) {
+        for (int i = 0; i < steps; ++i) {
+            const std::string bp = b.prefix + std::to_string(i) + ".book";
+            drop_invalid_restart_file(bp, 8 + sizeof(T));
+            drop_invalid_restart_file(b.prefix + std::to_string(i), 8);
+            b.have_book[i] = valid_restart_file(bp, 8 + sizeof(T)) ? 1 : 0;
+        }
+        if (o.write_raw_files) b.raw_writer = std::make_unique<RawFileWriter>(o.device);
+    }
+    b.M = res.meta_dev();
+
+    forward_pass(b, seeds, n_seeds);
+    b.t_fwd = std::chrono::steady_clock::now();
+
+    // when every layer gets a solve hash the solve never searches, and the
+    // books go without the bucket index (a table set over the result makes
+    // its own)
+    {
+        bool all_hashed = res.hash_max_bytes > 0;
+        for (int i = 1; i < steps && all_hashed; ++i) all_hashed = hash_layout(res, i, res.info[i].n_raw).present();
+        res.book_index = !all_hashed;
+    }
+
+    // Two streams for the backward pass of an in-HBM build: each solve
+    // fills its own layer's hash, so the compaction that makes the book
+    // leaves the chain of solves and runs on the second stream; every layer
+    // keeps its own rates until its compaction has read them.
+    uint64_t rates_all_bytes = 0;
+    for (int i = 0; i < steps; ++i) rates_all_bytes += (res.info[i].n_raw * sizeof(T) + 255) & ~(uint64_t)255;
+    // the per-layer rates must also fit in half the free HBM, so a build that
+    // fits with one stream never fails for want of the second's buffers
+    size_t hbm_free = 0, hbm_total = 0;
+    HIPCHK(hipMemGetInfo(&hbm_free, &hbm_total));
+    b.two_stream = steps >= 3 && !files && !b.streamer && !res.book_index && o.deletion_threshold <= 0.0 &&
+                   (double)rates_all_bytes <= kTwoStreamRatesMax && rates_all_bytes + 256 <= (uint64_t)hbm_free / 2;
+    if (b.two_stream) res.hash_ring = 4;
+
+    final_layers(b);
+    backward_pass(b);
+    optimal_branch_pass(b);
+    compress_pass(b);
     for (int i = 0; i < steps; ++i) {
         res.info[i].n_live = H[i].n_live;
         res.books[i].n = H[i].n_live;
@@ -2439,10 +2572,10 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
         res.raw.assign(steps, nullptr);
     }
     if (!res.keep_tables) res.book_arena.release();
-    res.fwd_ms = std::chrono::duration<double, std::milli>(t_fwd - t_begin).count();
-    res.bwd_ms = std::chrono::duration<double, std::milli>(t_end - t_fwd).count();
-    res.total_ms = std::chrono::duration<double, std::milli>(t_end - t_begin).count();
-    progress_set(prog_total, prog_total);
+    res.fwd_ms = std::chrono::duration<double, std::milli>(b.t_fwd - b.t_begin).count();
+    res.bwd_ms = std::chrono::duration<double, std::milli>(t_end - b.t_fwd).count();
+    res.total_ms = std::chrono::duration<double, std::milli>(t_end - b.t_begin).count();
+    progress_set(b.prog_total, b.prog_total);
 }
 
 void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *o,

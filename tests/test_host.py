@@ -91,14 +91,15 @@ def test_key_set_sizing_and_forward_query_check_arguments(pkg):
 
 
 def test_every_sort_state_error_flag_is_read():
-    """run_build keeps one sort state per stream, each with its own
-    look-back error flag; every one of them must be read before the build
-    returns, or a sort that hit its spin limit would pass unnoticed."""
+    """A build keeps one sort state per stream (the build context's and the
+    forward pass's second-stream one), each with its own look-back error
+    flag; every one of them must be read before the build returns, or a
+    sort that hit its spin limit would pass unnoticed."""
     import re
 
     src = open(os.path.join(os.path.dirname(__file__), "..", "2048endgametablebase_amd", "csrc",
                             "egtb_runtime.hip")).read()
-    start = src.index("void run_build(")
+    start = src.index("struct BuildCtx {")
     end = src.index("\nvoid build_dispatch(", start)
     body = src[start:end]
     states = re.findall(r"\bSortWork\s+(\w+)\s*;", body)
