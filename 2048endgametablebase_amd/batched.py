@@ -97,7 +97,7 @@ def compare_with_plain(jobs: Sequence[Tuple[str, int]], reps: int = 1) -> dict:
 
 
 def build_batch(jobs: Sequence[Tuple[str, int]] = DEFAULT_JOBS, spawn_rate4: float = 0.1, device: int = 0,
-                radix_bits: int = 0, sort_schedule: int = 0) -> BatchResult:
+                radix_bits: int = 0, sort_schedule: int = 0, forward_hash_steps: int = 0) -> BatchResult:
     params = [build_params(p, t) for p, t in jobs]
     keep = []
     arr = (native.BatchJob * len(params))()
@@ -120,6 +120,7 @@ def build_batch(jobs: Sequence[Tuple[str, int]] = DEFAULT_JOBS, spawn_rate4: flo
     o.device = device
     o.radix_bits = radix_bits
     o.sort_schedule = sort_schedule
+    o.forward_hash_steps = forward_hash_steps
     h = ctypes.c_void_p()
     native.check(native.lib().egtb_batch_build(arr, len(params), ctypes.byref(o), ctypes.byref(h)))
     return BatchResult(h.value, jobs)

@@ -25,6 +25,25 @@ struct PatternDev {
     uint32_t pad;
 };
 
+// One job of a batched build: many (pattern, target) builds share one key
+// space.  Every mask of a batched job pins nibbles 0 and 1, so the low byte
+// of each of its boards is 0xFF and a board of job j is stored as the key
+// (board >> 8) | j << 56.  Sorted keys then group the jobs and order each
+// job's boards as their plain values; kernels take a key's pattern, layer
+// count, docheck step and spawn weights from the job table.
+struct JobDev {
+    PatternDev p;
+    int32_t steps;    // layers of the job (its final layers are steps-2, steps-1)
+    int32_t docheck;  // success boards are terminal in layers above this one
+    double w2, w4;    // spawn weights: 1 - p4, p4
+};
+
+__device__ __forceinline__ uint64_t job_key(uint64_t board, uint32_t job) {
+    return (board >> 8) | ((uint64_t)job << 56);
+}
+__device__ __forceinline__ uint64_t job_board(uint64_t key) { return (key << 8) | 0xFFull; }
+__device__ __forceinline__ uint32_t job_of(uint64_t key) { return (uint32_t)(key >> 56); }
+
 // Main-diagonal transpose of the 4x4 nibble matrix by two delta swaps
 // (2x2 blocks, then cells inside each block).  Same map as the reference's
 // reverse_board / ReverseUL (CommonMover.h:33-37).

@@ -196,6 +196,16 @@ struct LaunchShape {
     uint32_t threads;  // threads per block
 };
 
+// The job table of a batched build (tab == nullptr: a plain build, whose
+// pattern and docheck flag the launch passes).  err gets bit 1 when a
+// canonical child's low byte is not 0xFF (it cannot be stored as a job key)
+// and bit 2 when a child's key is 0 (the empty slot of key sets and hashes).
+struct JobArg {
+    const JobDev *tab = nullptr;
+    uint32_t *err = nullptr;
+    int32_t step = 0;  // forward: the step (parents in layer step-1); solve: the layer
+};
+
 void configure_kernels();
 void launch_build_lut(hipStream_t s, uint16_t *lut);
 // Sizes: *_bound is a host-side upper bound (launch geometry); when the
@@ -204,14 +214,15 @@ void launch_build_lut(hipStream_t s, uint16_t *lut);
 void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                    const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
-                   uint64_t *or_and4, uint64_t offs_base = 0, uint64_t cap2 = ~0ull, uint64_t cap4 = ~0ull);
+                   uint64_t *or_and4, uint64_t offs_base = 0, uint64_t cap2 = ~0ull, uint64_t cap4 = ~0ull,
+                   const JobArg &ja = JobArg{});
 // One-launch forward step (any candidate order): *tot (zero before the
 // launch) gets c2 | c4 << 32; children at or past cap2 / cap4 are counted,
 // not written.
 void launch_expand_alloc(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                          const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                          uint64_t *tot, uint64_t *out2, uint64_t *out4, uint64_t *or_and2, uint64_t *or_and4,
-                         uint64_t cap2, uint64_t cap4);
+                         uint64_t cap2, uint64_t cap4, const JobArg &ja = JobArg{});
 // Forward step into key sets (open addressing, 0 = empty, capacities are
 // powers of two): spawn-2 children into set2, spawn-4 children into set4
 // (either may be null: counted, not inserted). *tot gets c2 | c4 << 32;
@@ -221,7 +232,7 @@ void launch_expand_alloc(hipStream_t s, const LaunchShape &ls, const uint64_t *p
 void launch_expand_hash(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                         const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *tot,
                         uint64_t *set2, uint64_t cap2, uint64_t *set4, uint64_t cap4, uint64_t *meta2,
-                        uint64_t *meta4);
+                        uint64_t *meta4, const JobArg &ja = JobArg{});
 // keys[0..n) (nonzero) into a set of cap > n slots (a power of two); cannot
 // overflow.
 void launch_set_insert(hipStream_t s, const uint64_t *keys, uint64_t n, uint64_t *set, uint64_t cap);
@@ -286,12 +297,13 @@ template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
                   const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T zero,
-                  T *rates, const HashDev &clear = HashDev{}, const HashDev &own = HashDev{});
+                  T *rates, const HashDev &clear = HashDev{}, const HashDev &own = HashDev{},
+                  const JobArg &ja = JobArg{});
 
 // Final layers: success -> max_scale, else zero.
 template <typename T>
 void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale, T zero,
-                        T *rates);
+                        T *rates, const JobArg &ja = JobArg{});
 
 // Records with rate > threshold (or, with `keep`, those whose keep byte is
 // set), in order, in one look-back launch; it shares the unique kernel's

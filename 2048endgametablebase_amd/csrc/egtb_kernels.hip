@@ -13,6 +13,7 @@
 //
 // Launch wrappers at the bottom are the only entry points used by the host.
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include <utility>
@@ -55,6 +56,53 @@ __global__ void k_build_lut(uint16_t *lut) {
 // Forward expansion (reference gen_boards_* , BookGenerator.cpp:473-760)
 // ---------------------------------------------------------------------------
 
+// Word offsets, from a layer record's or_all (the pointer the expand
+// kernels get for the layer of their spawn-2 children), of the step's
+// exact 64-bit candidate totals and of its key set's overflow flag.
+constexpr int kExactC2 = (int)((offsetof(LayerDev, c2x) - offsetof(LayerDev, or_all)) / sizeof(uint64_t));
+constexpr int kExactC4 = (int)((offsetof(LayerDev, c4x) - offsetof(LayerDev, or_all)) / sizeof(uint64_t));
+constexpr int kSetOverflow = (int)((offsetof(LayerDev, pad) - offsetof(LayerDev, or_all)) / sizeof(uint64_t));
+
+// How a stored key is expanded.  Plain builds store boards, and one
+// pattern and docheck flag (launch arguments) hold for the whole launch.
+// Batched builds store job keys and take both from the job table; a job
+// whose last forward step is past expands nothing, so its layers end at its
+// own layer count.
+template <bool JOBS>
+struct ParentView;
+
+template <>
+struct ParentView<false> {
+    uint64_t board;
+    bool live;
+    __device__ __forceinline__ ParentView(uint64_t key, const PatternDev &p, int do_check, const JobArg &)
+        : board(key), live(!(do_check && is_success(key, p))) {}
+    __device__ __forceinline__ const PatternDev &pattern(const PatternDev &p, const JobArg &) const { return p; }
+    __device__ __forceinline__ uint64_t child_key(uint64_t c, uint32_t &) const { return c; }
+};
+
+template <>
+struct ParentView<true> {
+    uint64_t board;
+    uint32_t job;
+    bool live;
+    __device__ __forceinline__ ParentView(uint64_t key, const PatternDev &, int, const JobArg &ja)
+        : board(job_board(key)), job(job_of(key)) {
+        const JobDev &J = ja.tab[job];
+        live = ja.step <= J.steps - 2 && !(ja.step > J.docheck && is_success(board, J.p));
+    }
+    __device__ __forceinline__ const PatternDev &pattern(const PatternDev &, const JobArg &ja) const {
+        return ja.tab[job].p;
+    }
+    // a child's key; bit 1 of bad: the child leaves the job's pinned low
+    // byte, bit 2: its key is 0
+    __device__ __forceinline__ uint64_t child_key(uint64_t c, uint32_t &bad) const {
+        const uint64_t k = job_key(c, job);
+        bad |= ((c & 0xFF) != 0xFF ? 2u : 0u) | (k == 0 ? 4u : 0u);
+        return k;
+    }
+};
+
 // One thread per parent.  For each empty nibble k (ascending) and spawn
 // value 1 (a 2) / 2 (a 4): the four moves of the spawned board; a move is
 // kept when it changed the board and the result is in the pattern; the
@@ -69,7 +117,7 @@ __global__ void k_build_lut(uint16_t *lut) {
 //   same way) into out2 / out4; indices at or past cap2 / cap4 are dropped
 //   (a write queued before its totals were known; the host then grows the
 //   buffers and writes again).
-template <bool WRITE>
+template <bool WRITE, bool JOBS>
 __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ par, uint64_t n_bound,
                                                  const uint64_t *__restrict__ n_ptr,
                                                  const uint16_t *__restrict__ glut, PatternDev p,
@@ -79,15 +127,18 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                                                  uint64_t *__restrict__ out4,
                                                  uint64_t *__restrict__ or_and2,
                                                  uint64_t *__restrict__ or_and4, uint64_t cap2,
-                                                 uint64_t cap4) {
+                                                 uint64_t cap4, JobArg ja) {
     extern __shared__ uint16_t lut[];
-    __shared__ uint64_t red[4][16];
+    __shared__ uint64_t red[6][16];
     stage_lut(lut, glut);
     const uint64_t n = n_ptr ? (*n_ptr < n_bound ? *n_ptr : n_bound) : n_bound;
-    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull;
+    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull, x2 = 0, x4 = 0;
+    uint32_t bad = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint64_t b = par[i];
+        const ParentView<JOBS> pv(par[i], p, do_check, ja);
+        const uint64_t b = pv.board;
+        const PatternDev &P = pv.pattern(p, ja);
         uint32_t c2 = 0, c4 = 0;
         uint64_t w2 = 0, w4 = 0;
         if (WRITE) {
@@ -95,7 +146,7 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
             w2 = o & 0xFFFFFFFFull;
             w4 = o >> 32;
         }
-        if (!(do_check && is_success(b, p))) {
+        if (pv.live) {
             uint32_t em = empty_cells(b);
             const SpawnMoves smv(b, lut);
             while (em) {
@@ -108,8 +159,8 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                     smv.of(k, (uint64_t)s, lut, mv);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
-                        if (mv[d] != sp && is_pattern(mv[d], p)) {
-                            const uint64_t c = canonical(mv[d], p.symm);
+                        if (mv[d] != sp && is_pattern(mv[d], P)) {
+                            const uint64_t c = pv.child_key(canonical(mv[d], P.symm), bad);
                             if (s == 1) {
                                 if (WRITE && w2 + c2 < cap2) out2[w2 + c2] = c;
                                 else { or2 |= c; and2 &= c; }
@@ -124,16 +175,24 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                 }
             }
         }
-        if (!WRITE) counts[i] = (uint64_t)c2 | ((uint64_t)c4 << 32);
+        if (!WRITE) {
+            counts[i] = (uint64_t)c2 | ((uint64_t)c4 << 32);
+            x2 += c2;
+            x4 += c4;
+        }
     }
+    if (JOBS && bad) atomicOr(ja.err, bad);
     if (!WRITE) {
-        // fold the span accumulators: wave, then one atomic per wave
+        // fold the span accumulators and the exact totals: wave, then one
+        // set of atomics per block
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             or2 |= __shfl_xor(or2, off, 64);
             and2 &= __shfl_xor(and2, off, 64);
             or4 |= __shfl_xor(or4, off, 64);
             and4 &= __shfl_xor(and4, off, 64);
+            x2 += __shfl_xor(x2, off, 64);
+            x4 += __shfl_xor(x4, off, 64);
         }
         const uint32_t wave = threadIdx.x >> 6;
         if (lane_id() == 0) {
@@ -141,6 +200,8 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
             red[1][wave] = and2;
             red[2][wave] = or4;
             red[3][wave] = and4;
+            red[4][wave] = x2;
+            red[5][wave] = x4;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -149,11 +210,15 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
                 and2 &= red[1][w];
                 or4 |= red[2][w];
                 and4 &= red[3][w];
+                x2 += red[4][w];
+                x4 += red[5][w];
             }
             atomicOr((unsigned long long *)&or_and2[0], (unsigned long long)or2);
             atomicAnd((unsigned long long *)&or_and2[1], (unsigned long long)and2);
             atomicOr((unsigned long long *)&or_and4[0], (unsigned long long)or4);
             atomicAnd((unsigned long long *)&or_and4[1], (unsigned long long)and4);
+            if (x2) atomicAdd((unsigned long long *)&or_and2[kExactC2], (unsigned long long)x2);
+            if (x4) atomicAdd((unsigned long long *)&or_and2[kExactC4], (unsigned long long)x4);
         }
     }
 }
@@ -165,6 +230,7 @@ __global__ __launch_bounds__(1024) void k_expand(const uint64_t *__restrict__ pa
 // written. Writes at or past cap2 / cap4 are dropped while the counter still
 // counts them, so the totals are exact either way. The key spans of both
 // child streams are folded into or_and2 / or_and4.
+template <bool JOBS>
 __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restrict__ par, uint64_t n_bound,
                                                        const uint64_t *__restrict__ n_ptr,
                                                        const uint16_t *__restrict__ glut, PatternDev p,
@@ -172,19 +238,22 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
                                                        uint64_t *__restrict__ out2, uint64_t *__restrict__ out4,
                                                        uint64_t *__restrict__ or_and2,
                                                        uint64_t *__restrict__ or_and4, uint64_t cap2,
-                                                       uint64_t cap4) {
+                                                       uint64_t cap4, JobArg ja) {
     extern __shared__ uint16_t lut[];
-    __shared__ uint64_t red[4][16];
+    __shared__ uint64_t red[6][16];
     __shared__ uint64_t scratch[16];
     __shared__ uint64_t s_base;
     stage_lut(lut, glut);
     const uint64_t n = n_ptr ? (*n_ptr < n_bound ? *n_ptr : n_bound) : n_bound;
-    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull;
+    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull, x2 = 0, x4 = 0;
+    uint32_t bad = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
         const uint64_t i = i0 + threadIdx.x;
-        const uint64_t b = i < n ? par[i] : 0;
-        const bool live = i < n && !(do_check && is_success(b, p));
+        const ParentView<JOBS> pv(i < n ? par[i] : 0, p, do_check, ja);
+        const uint64_t b = pv.board;
+        const PatternDev &P = pv.pattern(p, ja);
+        const bool live = i < n && pv.live;
         const SpawnMoves smv(b, lut);  // for both passes
         uint32_t c2 = 0, c4 = 0;
         if (live) {
@@ -199,8 +268,8 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
                     smv.of(k, (uint64_t)s, lut, mv);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
-                        if (mv[d] != sp && is_pattern(mv[d], p)) {
-                            const uint64_t c = canonical(mv[d], p.symm);
+                        if (mv[d] != sp && is_pattern(mv[d], P)) {
+                            const uint64_t c = pv.child_key(canonical(mv[d], P.symm), bad);
                             if (s == 1) {
                                 or2 |= c;
                                 and2 &= c;
@@ -217,7 +286,11 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
         }
         uint64_t block_total;
         const uint64_t ex = block_exclusive_sum((uint64_t)c2 | ((uint64_t)c4 << 32), scratch, block_total);
-        if (threadIdx.x == 0) s_base = atomicAdd(tot, (unsigned long long)block_total);
+        if (threadIdx.x == 0) {
+            s_base = atomicAdd(tot, (unsigned long long)block_total);
+            x2 += block_total & 0xFFFFFFFFull;  // a block's halves stay far below 2^32
+            x4 += block_total >> 32;
+        }
         __syncthreads();
         const uint64_t at = s_base + ex;  // both halves: c2 and c4 of a step stay below 2^32
         uint64_t w2 = at & 0xFFFFFFFFull, w4 = at >> 32;
@@ -234,8 +307,8 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
                     smv.of(k, (uint64_t)s, lut, mv);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
-                        if (mv[d] != sp && is_pattern(mv[d], p)) {
-                            const uint64_t c = canonical(mv[d], p.symm);
+                        if (mv[d] != sp && is_pattern(mv[d], P)) {
+                            const uint64_t c = pv.child_key(canonical(mv[d], P.symm), bad);
                             if (s == 1) {
                                 if (w2 < cap2) out2[w2] = c;
                                 ++w2;
@@ -249,6 +322,7 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
             }
         }
     }
+    if (JOBS && bad) atomicOr(ja.err, bad);
     // fold the span accumulators: wave, then one atomic per block
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -276,6 +350,8 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
         atomicAnd((unsigned long long *)&or_and2[1], (unsigned long long)and2);
         atomicOr((unsigned long long *)&or_and4[0], (unsigned long long)or4);
         atomicAnd((unsigned long long *)&or_and4[1], (unsigned long long)and4);
+        if (x2) atomicAdd((unsigned long long *)&or_and2[kExactC2], (unsigned long long)x2);
+        if (x4) atomicAdd((unsigned long long *)&or_and2[kExactC4], (unsigned long long)x4);
     }
 }
 
@@ -307,26 +383,30 @@ __device__ __forceinline__ bool set_insert(uint64_t *__restrict__ tab, uint64_t 
     return false;
 }
 
+template <bool JOBS>
 __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict__ par, uint64_t n_bound,
                                                       const uint64_t *__restrict__ n_ptr,
                                                       const uint16_t *__restrict__ glut, PatternDev p, int do_check,
                                                       unsigned long long *__restrict__ tot, uint64_t *__restrict__ set2,
                                                       uint64_t mask2, uint64_t *__restrict__ set4, uint64_t mask4,
-                                                      uint64_t *__restrict__ meta2, uint64_t *__restrict__ meta4) {
+                                                      uint64_t *__restrict__ meta2, uint64_t *__restrict__ meta4,
+                                                      JobArg ja) {
     extern __shared__ uint16_t lut[];
     __shared__ uint64_t red[4][16];
-    __shared__ uint64_t wtot[16];
+    __shared__ uint64_t wtot[2][16];
     __shared__ uint32_t s_fail;
     stage_lut(lut, glut);
     if (threadIdx.x == 0) s_fail = 0;
     __syncthreads();
     const uint64_t n = n_ptr ? (*n_ptr < n_bound ? *n_ptr : n_bound) : n_bound;
-    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull, count = 0;
-    uint32_t fail = 0;
+    uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull, x2 = 0, x4 = 0;
+    uint32_t fail = 0, bad = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint64_t b = par[i];
-        if (do_check && is_success(b, p)) continue;
+        const ParentView<JOBS> pv(par[i], p, do_check, ja);
+        if (!pv.live) continue;
+        const uint64_t b = pv.board;
+        const PatternDev &P = pv.pattern(p, ja);
         const SpawnMoves smv(b, lut);
         // the last child per (spawn value, move): spawns in neighbouring
         // cells often slide to the same board, which needs no second insert
@@ -342,20 +422,20 @@ __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict
                 smv.of(k, (uint64_t)sv, lut, mv);
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
-                    if (mv[d] != sp && is_pattern(mv[d], p)) {
-                        const uint64_t c = canonical(mv[d], p.symm);
+                    if (mv[d] != sp && is_pattern(mv[d], P)) {
+                        const uint64_t c = pv.child_key(canonical(mv[d], P.symm), bad);
                         const int j = (sv - 1) * 4 + d;
                         const bool again = c == last[j];
                         last[j] = c;
                         if (sv == 1) {
                             or2 |= c;
                             and2 &= c;
-                            count += 1;
+                            ++x2;
                             if (set2 && !again && !set_insert(set2, mask2, c)) fail |= 1u;
                         } else {
                             or4 |= c;
                             and4 &= c;
-                            count += 1ull << 32;
+                            ++x4;
                             if (set4 && !again && !set_insert(set4, mask4, c)) fail |= 2u;
                         }
                     }
@@ -370,16 +450,19 @@ __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict
         and2 &= __shfl_xor(and2, off, 64);
         or4 |= __shfl_xor(or4, off, 64);
         and4 &= __shfl_xor(and4, off, 64);
-        count += __shfl_xor(count, off, 64);
+        x2 += __shfl_xor(x2, off, 64);
+        x4 += __shfl_xor(x4, off, 64);
     }
     const uint32_t wave = threadIdx.x >> 6;
     if (fail) atomicOr(&s_fail, fail);
+    if (JOBS && bad) atomicOr(ja.err, bad);
     if (lane_id() == 0) {
         red[0][wave] = or2;
         red[1][wave] = and2;
         red[2][wave] = or4;
         red[3][wave] = and4;
-        wtot[wave] = count;
+        wtot[0][wave] = x2;
+        wtot[1][wave] = x4;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -388,15 +471,21 @@ __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict
             and2 &= red[1][w];
             or4 |= red[2][w];
             and4 &= red[3][w];
-            count += wtot[w];
+            x2 += wtot[0][w];
+            x4 += wtot[1][w];
         }
-        if (count) atomicAdd(tot, (unsigned long long)count);
+        // the packed total (c2 | c4 << 32) and the exact 64-bit halves: a
+        // block's halves stay far below 2^32, the packed sum over the launch
+        // is exact while the step's totals are
+        if (x2 | x4) atomicAdd(tot, (unsigned long long)(x2 | (x4 << 32)));
+        if (x2) atomicAdd((unsigned long long *)&meta2[kExactC2], (unsigned long long)x2);
+        if (x4) atomicAdd((unsigned long long *)&meta2[kExactC4], (unsigned long long)x4);
         atomicOr((unsigned long long *)&meta2[0], (unsigned long long)or2);
         atomicAnd((unsigned long long *)&meta2[1], (unsigned long long)and2);
         atomicOr((unsigned long long *)&meta4[0], (unsigned long long)or4);
         atomicAnd((unsigned long long *)&meta4[1], (unsigned long long)and4);
-        if (s_fail & 1u) atomicOr((unsigned long long *)&meta2[4], 1ull);
-        if (s_fail & 2u) atomicOr((unsigned long long *)&meta4[4], 1ull);
+        if (s_fail & 1u) atomicOr((unsigned long long *)&meta2[kSetOverflow], 1ull);
+        if (s_fail & 2u) atomicOr((unsigned long long *)&meta4[kSetOverflow], 1ull);
     }
 }
 
@@ -1069,13 +1158,17 @@ __device__ __forceinline__ void hash_insert(const HashDev &h, uint64_t k, T r) {
     slots[sl].y = rate_bits(r);
 }
 
-template <typename T>
+// Batched builds (JOBS): a key takes its pattern, docheck step and spawn
+// weights from its job, a key in one of its job's two final layers gets the
+// final rate (success -> max_scale, else zero), and successors are looked up
+// under the same job's keys.
+template <typename T, bool JOBS>
 __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ keys, uint64_t n,
                                                 const uint16_t *__restrict__ glut, PatternDev p,
                                                 int do_check, HashDev h1, HashDev h2, TableDev<T> t1,
                                                 TableDev<T> t2, double w2, double w4, T max_scale, T zero,
                                                 T *__restrict__ rates, ulonglong2 *__restrict__ clear,
-                                                uint64_t clear_n, HashDev own) {
+                                                uint64_t clear_n, HashDev own, JobArg ja) {
     extern __shared__ uint16_t lut[];
     // the table filled after this launch: disjoint from h1 / h2
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < clear_n;
@@ -1091,10 +1184,29 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
     const uint64_t n2 = use_hash ? 0 : (t2.n_ptr ? *t2.n_ptr : t2.n);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint64_t b = keys[i];
-        if (do_check && is_success(b, p)) {
+        const uint64_t key = keys[i];
+        uint64_t b = key;
+        uint32_t job = 0;
+        bool chk = do_check != 0;
+        double W2 = w2, W4 = w4;
+        if constexpr (JOBS) {
+            b = job_board(key);
+            job = job_of(key);
+            const JobDev &J = ja.tab[job];
+            if (ja.step >= J.steps - 2) {  // the job's final layers
+                const T rv = is_success(b, J.p) ? max_scale : zero;
+                rates[i] = rv;
+                if (own.present() && rv > zero) hash_insert(own, key, rv);
+                continue;
+            }
+            chk = ja.step > J.docheck;
+            W2 = J.w2;
+            W4 = J.w4;
+        }
+        const PatternDev &P = JOBS ? ja.tab[job].p : p;
+        if (chk && is_success(b, P)) {
             rates[i] = max_scale;
-            if (own.present()) hash_insert(own, b, max_scale);
+            if (own.present()) hash_insert(own, key, max_scale);
             continue;
         }
         double sp = 0.0;
@@ -1107,7 +1219,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
             // the 8 successors of this cell: spawn 2 (j < 4) and spawn 4
             // (j >= 4) times the four moves; their probes advance in
             // lockstep so the loads of one round are all in flight together
-            uint64_t key[8], slot[8];
+            uint64_t skey[8], slot[8];
             uint32_t active = 0;
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
@@ -1117,8 +1229,9 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     const int j = s * 4 + d;
-                    key[j] = canonical(mv[d], p.symm);
-                    if (mv[d] != sp_board && is_pattern(mv[d], p)) active |= 1u << j;
+                    const uint64_t c = canonical(mv[d], P.symm);
+                    skey[j] = JOBS ? job_key(c, job) : c;
+                    if (mv[d] != sp_board && is_pattern(mv[d], P)) active |= 1u << j;
                 }
             }
             T r[8];
@@ -1132,7 +1245,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                     lo[j] = hi[j] = 0;
                     const TableDev<T> &t = j < 4 ? t1 : t2;
                     if (((active >> j) & 1u) && (j < 4 ? n1 : n2)) {
-                        const uint64_t bk = key[j] >> (64 - t.bits);
+                        const uint64_t bk = skey[j] >> (64 - t.bits);
                         lo[j] = t.idx[bk];
                         hi[j] = t.idx[bk + 1];
                     }
@@ -1143,7 +1256,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                     for (int j = 0; j < 8; ++j) {
                         if (lo[j] < hi[j]) {
                             const uint32_t mid = (lo[j] + hi[j]) >> 1;
-                            if ((j < 4 ? t1.keys : t2.keys)[mid] < key[j]) lo[j] = mid + 1;
+                            if ((j < 4 ? t1.keys : t2.keys)[mid] < skey[j]) lo[j] = mid + 1;
                             else hi[j] = mid;
                         }
                     }
@@ -1154,7 +1267,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const TableDev<T> &t = j < 4 ? t1 : t2;
-                    if (((active >> j) & 1u) && lo[j] < (j < 4 ? n1 : n2) && t.keys[lo[j]] == key[j])
+                    if (((active >> j) & 1u) && lo[j] < (j < 4 ? n1 : n2) && t.keys[lo[j]] == skey[j])
                         r[j] = t.rates[lo[j]];
                 }
                 active = 0;
@@ -1169,10 +1282,10 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                 step[j] = 0;
                 want[j] = 0;
                 if (h.cslots) {
-                    if (((active >> j) & 1u) && !chash_split(h, key[j], slot[j], want[j]))
+                    if (((active >> j) & 1u) && !chash_split(h, skey[j], slot[j], want[j]))
                         active &= ~(1u << j);  // constant bits differ: not in the layer
                 } else {
-                    slot[j] = hash_mix(key[j]) & h.mask;
+                    slot[j] = hash_mix(skey[j]) & h.mask;
                 }
             }
             while (active) {
@@ -1206,7 +1319,7 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
                                     active &= ~(1u << j);
                                 }
                             }
-                        } else if (ent[j].x == key[j]) {
+                        } else if (ent[j].x == skey[j]) {
                             r[j] = rate_from_bits<T>(ent[j].y);
                             active &= ~(1u << j);
                         } else if (ent[j].x == 0) {
@@ -1220,12 +1333,12 @@ __global__ __launch_bounds__(1024) void k_solve(const uint64_t *__restrict__ key
             T best[2] = {zero, zero};
 #pragma unroll
             for (int j = 0; j < 8; ++j) best[j >> 2] = best[j >> 2] < r[j] ? r[j] : best[j >> 2];
-            sp = __fma_rn((double)best[0], w2, sp);
-            sp = __fma_rn((double)best[1], w4, sp);
+            sp = __fma_rn((double)best[0], W2, sp);
+            sp = __fma_rn((double)best[1], W4, sp);
         }
         const T rv = e > 0 ? (T)(sp / (double)e) : zero;
         rates[i] = rv;
-        if (own.present() && rv > zero) hash_insert(own, b, rv);
+        if (own.present() && rv > zero) hash_insert(own, key, rv);
     }
 }
 
@@ -1242,11 +1355,14 @@ __global__ __launch_bounds__(256) void k_hash_build(const uint64_t *__restrict__
 }
 
 // Final layers (reference final_situation_process, BookSolver.cpp:237-261).
-template <typename T>
+template <typename T, bool JOBS>
 __global__ void k_final_rates(const uint64_t *__restrict__ keys, uint64_t n, PatternDev p, T max_scale, T zero,
-                              T *__restrict__ rates) {
+                              T *__restrict__ rates, JobArg ja) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) rates[i] = is_success(keys[i], p) ? max_scale : zero;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if constexpr (JOBS) rates[i] = is_success(job_board(k), ja.tab[job_of(k)].p) ? max_scale : zero;
+    else rates[i] = is_success(k, p) ? max_scale : zero;
 }
 
 // ---------------------------------------------------------------------------
@@ -1416,14 +1532,26 @@ __global__ void k_unpack_records(const uint32_t *__restrict__ recs, uint64_t n, 
 
 static inline uint32_t blocks_for(uint64_t n, uint64_t per) { return (uint32_t)((n + per - 1) / per); }
 
+template <typename T>
+static void configure_solve(int lds) {
+    (void)hipFuncSetAttribute((const void *)k_solve<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_solve<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
 void configure_kernels() {
     const int lds = kLutEntries * sizeof(uint16_t);
-    (void)hipFuncSetAttribute((const void *)k_expand<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_expand<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_solve<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_solve<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_solve<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    (void)hipFuncSetAttribute((const void *)k_solve<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand_alloc<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand_alloc<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand_hash<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_expand_hash<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    configure_solve<float>(lds);
+    configure_solve<double>(lds);
+    configure_solve<uint32_t>(lds);
+    configure_solve<uint64_t>(lds);
     (void)hipFuncSetAttribute((const void *)k_opt_mark<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_opt_mark<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((const void *)k_opt_mark<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -1437,43 +1565,54 @@ void launch_build_lut(hipStream_t s, uint16_t *lut) {
 void launch_expand(hipStream_t s, bool write, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                    const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                    uint64_t *counts, const uint64_t *offs, uint64_t *out2, uint64_t *out4, uint64_t *or_and2,
-                   uint64_t *or_and4, uint64_t offs_base, uint64_t cap2, uint64_t cap4) {
+                   uint64_t *or_and4, uint64_t offs_base, uint64_t cap2, uint64_t cap4, const JobArg &ja) {
     if (n_bound == 0) return;
     const uint32_t want = blocks_for(n_bound, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     const size_t lds = kLutEntries * sizeof(uint16_t);
-    if (write)
-        hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
-                           do_check, counts, offs, offs_base, out2, out4, or_and2, or_and4, cap2, cap4);
-    else
-        hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p,
-                           do_check, counts, offs, (uint64_t)0, out2, out4, or_and2, or_and4, (uint64_t)0,
-                           (uint64_t)0);
+    if (!write) {
+        offs_base = 0;
+        cap2 = cap4 = 0;
+    }
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check, counts,
+                           offs, offs_base, out2, out4, or_and2, or_and4, cap2, cap4, ja);
+    };
+    if (ja.tab) write ? go(k_expand<true, true>) : go(k_expand<false, true>);
+    else write ? go(k_expand<true, false>) : go(k_expand<false, false>);
 }
 
 void launch_expand_alloc(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                          const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check,
                          uint64_t *tot, uint64_t *out2, uint64_t *out4, uint64_t *or_and2, uint64_t *or_and4,
-                         uint64_t cap2, uint64_t cap4) {
+                         uint64_t cap2, uint64_t cap4, const JobArg &ja) {
     if (n_bound == 0) return;
     const uint32_t want = blocks_for(n_bound, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     const size_t lds = kLutEntries * sizeof(uint16_t);
-    hipLaunchKernelGGL(k_expand_alloc, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
-                       (unsigned long long *)tot, out2, out4, or_and2, or_and4, cap2, cap4);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
+                           (unsigned long long *)tot, out2, out4, or_and2, or_and4, cap2, cap4, ja);
+    };
+    if (ja.tab) go(k_expand_alloc<true>);
+    else go(k_expand_alloc<false>);
 }
 
 void launch_expand_hash(hipStream_t s, const LaunchShape &ls, const uint64_t *par, uint64_t n_bound,
                         const uint64_t *n_ptr, const uint16_t *lut, const PatternDev &p, int do_check, uint64_t *tot,
                         uint64_t *set2, uint64_t cap2, uint64_t *set4, uint64_t cap4, uint64_t *meta2,
-                        uint64_t *meta4) {
+                        uint64_t *meta4, const JobArg &ja) {
     if (n_bound == 0) return;
     const uint32_t want = blocks_for(n_bound, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
     const size_t lds = kLutEntries * sizeof(uint16_t);
-    hipLaunchKernelGGL(k_expand_hash, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
-                       (unsigned long long *)tot, set2, set2 ? cap2 - 1 : 0ull, set4, set4 ? cap4 - 1 : 0ull, meta2,
-                       meta4);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(ls.threads), lds, s, par, n_bound, n_ptr, lut, p, do_check,
+                           (unsigned long long *)tot, set2, set2 ? cap2 - 1 : 0ull, set4, set4 ? cap4 - 1 : 0ull,
+                           meta2, meta4, ja);
+    };
+    if (ja.tab) go(k_expand_hash<true>);
+    else go(k_expand_hash<false>);
 }
 
 void launch_set_insert(hipStream_t s, const uint64_t *keys, uint64_t n, uint64_t *set, uint64_t cap) {
@@ -1632,7 +1771,7 @@ template <typename T>
 void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, uint64_t n, const uint16_t *lut,
                   const PatternDev &p, int do_check, const HashDev &h1, const HashDev &h2,
                   const TableDev<T> &t1, const TableDev<T> &t2, double w2, double w4, T max_scale, T zero,
-                  T *rates, const HashDev &clear, const HashDev &own) {
+                  T *rates, const HashDev &clear, const HashDev &own, const JobArg &ja) {
     const uint64_t clear_n = clear.bytes() / sizeof(ulonglong2);
     if (n == 0) {
         if (clear_n) (void)hipMemsetAsync(clear.table(), 0, clear.bytes(), s);
@@ -1640,9 +1779,13 @@ void launch_solve(hipStream_t s, const LaunchShape &ls, const uint64_t *keys, ui
     }
     const uint32_t want = blocks_for(n > clear_n ? n : clear_n, ls.threads);
     const uint32_t grid = want < ls.blocks ? want : ls.blocks;
-    hipLaunchKernelGGL((k_solve<T>), dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n,
-                       lut, p, do_check, h1, h2, t1, t2, w2, w4, max_scale, zero, rates, (ulonglong2 *)clear.table(),
-                       clear_n, own);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(ls.threads), kLutEntries * sizeof(uint16_t), s, keys, n, lut, p,
+                           do_check, h1, h2, t1, t2, w2, w4, max_scale, zero, rates, (ulonglong2 *)clear.table(),
+                           clear_n, own, ja);
+    };
+    if (ja.tab) go(k_solve<T, true>);
+    else go(k_solve<T, false>);
 }
 
 // Slots for a layer of at most `bound` records: a power of two, at most a
@@ -1667,10 +1810,14 @@ void launch_hash_build(hipStream_t s, const uint64_t *keys, const T *rates, uint
 
 template <typename T>
 void launch_final_rates(hipStream_t s, const uint64_t *keys, uint64_t n, const PatternDev &p, T max_scale, T zero,
-                        T *rates) {
+                        T *rates, const JobArg &ja) {
     if (n == 0) return;
-    hipLaunchKernelGGL((k_final_rates<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, p, max_scale, zero,
-                       rates);
+    if (ja.tab)
+        hipLaunchKernelGGL((k_final_rates<T, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, p, max_scale,
+                           zero, rates, ja);
+    else
+        hipLaunchKernelGGL((k_final_rates<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, p,
+                           max_scale, zero, rates, ja);
 }
 
 uint64_t compact_tiles(uint64_t n) { return blocks_for(n ? n : 1, kCompTile); }
@@ -1727,10 +1874,11 @@ void launch_unpack_records(hipStream_t s, const uint32_t *recs, uint64_t n, int 
     template void launch_solve<T>(hipStream_t, const LaunchShape &, const uint64_t *, uint64_t,              \
                                   const uint16_t *, const PatternDev &, int, const HashDev &,                 \
                                   const HashDev &, const TableDev<T> &, const TableDev<T> &, double, double,  \
-                                  T, T, T *, const HashDev &, const HashDev &);                               \
+                                  T, T, T *, const HashDev &, const HashDev &, const JobArg &);               \
     template void launch_hash_build<T>(hipStream_t, const uint64_t *, const T *, uint64_t, const uint64_t *,  \
                                        const HashDev &);                                                      \
-    template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T, T *); \
+    template void launch_final_rates<T>(hipStream_t, const uint64_t *, uint64_t, const PatternDev &, T, T, T *, \
+                                        const JobArg &);                                                      \
     template void launch_compact<T>(hipStream_t, const uint64_t *, const T *, uint64_t, uint64_t *, T *,      \
                                     unsigned long long *, uint64_t *, LookbackWork &, T, const HashDev &,     \
                                     bool, const uint8_t *);                                                   \

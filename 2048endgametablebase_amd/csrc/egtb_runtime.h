@@ -174,7 +174,10 @@ struct LayerDev {
     uint64_t and_all;  // AND of every candidate key of layer i
     uint64_t maxbits;  // bits of the largest rate
     uint64_t tot;      // candidates of forward step i: c2 | c4 << 32
-    uint64_t pad[2];
+    uint64_t pad[2];   // pad[0]: layer i's key set overflowed
+    // at least the candidates of forward step i of each spawn value, summed
+    // in 64 bits (reruns add again): below 2^32, tot's halves are exact
+    uint64_t c2x, c4x;
 };
 
 // A solved layer resident in HBM: SoA keys/rates (arena) plus its bucket index.
@@ -267,5 +270,12 @@ void write_file_atomic(const std::string &path, const void *data, size_t bytes);
 void progress_set(uint32_t cur, uint32_t total);
 void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *o,
                     Result &res);
+// A batched build through the plain schedule's stages (float32 rates, books
+// kept in HBM): keys[0..n_keys) are the seeds of every job as job keys, job
+// by job; o.steps is the largest job's layer count.  Kernels raise bits of
+// *job_err (device memory, zero before) for children that cannot be stored
+// as job keys.
+void run_build_jobs(const uint64_t *keys, size_t n_keys, const egtb_options &o, const JobDev *jobs,
+                    uint32_t *job_err, Result &res);
 
 }  // namespace egtb

@@ -99,16 +99,24 @@ DeviceCtx &device_ctx(int dev) {
 // those can starve an allocation of another size or of the other kind: on
 // failure the device is synchronised, the pool's unused blocks are handed
 // back and the allocation is tried once more.
+static void stats_add_host(const char *name, double ms);
+
 hipError_t device_alloc(void **p, size_t nbytes, hipStream_t s, bool async) {
+    const auto t0 = std::chrono::steady_clock::now();
     hipError_t e = async ? hipMallocAsync(p, nbytes, s) : hipMalloc(p, nbytes);
-    if (e != hipErrorOutOfMemory && e != hipErrorMemoryAllocation) return e;
-    (void)hipGetLastError();
-    int dev = 0;
-    hipMemPool_t pool;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
-        hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess || hipMemPoolTrimTo(pool, 0) != hipSuccess)
-        return e;
-    return async ? hipMallocAsync(p, nbytes, s) : hipMalloc(p, nbytes);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+        (void)hipGetLastError();
+        int dev = 0;
+        hipMemPool_t pool;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+            hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && hipMemPoolTrimTo(pool, 0) == hipSuccess) {
+            stats_add_host("host_pool_trim", 0.0);
+            e = async ? hipMallocAsync(p, nbytes, s) : hipMalloc(p, nbytes);
+        }
+    }
+    stats_add_host(async ? "host_alloc_async" : "host_alloc",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    return e;
 }
 
 void DBuf::alloc(size_t nbytes, hipStream_t s) {
@@ -196,6 +204,15 @@ static void resolve_pending_locked() {
         g_event_pool.push_back(pe.b);
     }
     g_pending.clear();
+}
+
+// Host time of a host-side operation (device allocations), per family.
+static void stats_add_host(const char *name, double ms) {
+    if (!g_stats_on.load(std::memory_order_relaxed)) return;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
+    auto &acc = g_stats[name];
+    acc.launches++;
+    acc.ms += ms;
 }
 
 // Add algorithmic bytes known only after a build to a kernel family.
@@ -532,10 +549,12 @@ void Arena::trim_last(void *p, size_t bytes) {
 }
 
 void Arena::free_chunk(Chunk &c) {
+    const auto t0 = std::chrono::steady_clock::now();
     if (stream_)
         (void)hipFreeAsync(c.p, stream_);
     else
         (void)hipFree(c.p);
+    stats_add_host("host_free", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
 }
 
 void Arena::release_after(const void *p) {
@@ -1465,6 +1484,11 @@ struct BuildCtx {
     std::chrono::steady_clock::time_point t_begin, t_fwd;
     bool two_stream = false;  // the backward pass's schedule
     uint64_t max_n = 0;       // largest layer
+    // batched builds: the device job table (keys are job keys; the pattern,
+    // docheck step and spawn weights of the options are not used)
+    const JobDev *jobs = nullptr;
+    uint32_t *job_err = nullptr;
+    JobArg ja(int step) const { return JobArg{jobs, job_err, step}; }
 
     BuildCtx(const egtb_options &o_, const PatternDev &p_, Result &res_, bool one_minus)
         : C(device_ctx(o_.device)), s(C.stream), o(o_), p(p_), res(res_), steps(o_.steps),
@@ -1514,7 +1538,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
             Timer t(s, "expand_count", 8.0 * (double)bound);
             launch_expand(s, false, C.shape, res.raw[step - 1], bound, &M[step - 1].n_raw, C.lut, p,
                           step > o.docheck_step ? 1 : 0, (uint64_t *)counts.p, nullptr, nullptr, nullptr,
-                          &M[step].or_all, &M[step + 1].or_all);
+                          &M[step].or_all, &M[step + 1].or_all, 0, ~0ull, ~0ull, b.ja(step));
         }
         launch_scan_u64(s, (const uint64_t *)counts.p, bound, &M[step - 1].n_raw, (uint64_t *)offs.p,
                         (uint64_t *)w.partial.p, &M[step].tot);
@@ -1532,7 +1556,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
         Timer t(s, "expand", 0.0);  // its bytes are counted once per step, below
         launch_expand_alloc(s, C.shape, res.raw[step - 1], bound, &M[step - 1].n_raw, C.lut, p,
                             step > o.docheck_step ? 1 : 0, &M[step].tot, out2, out4, &M[step].or_all,
-                            &M[step + 1].or_all, cap2, cap4);
+                            &M[step + 1].or_all, cap2, cap4, b.ja(step));
     };
 
     // ---- chunked steps: runs (sorted unique pieces) of the layer being
@@ -1664,7 +1688,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
         launch_expand_hash(s, C.shape, parents, bound, &M[step - 1].n_raw, C.lut, p, step > o.docheck_step ? 1 : 0,
                            tot, s2 ? (uint64_t *)s2->buf.p : nullptr, s2 ? s2->cap : 0,
                            s4 ? (uint64_t *)s4->buf.p : nullptr, s4 ? s4->cap : 0, &M[step].or_all,
-                           &M[step + 1].or_all);
+                           &M[step + 1].or_all, b.ja(step));
     };
     // Sorts of unique layers on the second stream, in batches of kSortBatch
     // layers: one fork (the second stream waits for everything queued on
@@ -1751,9 +1775,12 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
         for (hipError_t q; (q = hipEventQuery(hev[L & 1])) != hipSuccess;)
             if (q != hipErrorNotReady) HIPCHK(q);
     };
-    auto check_layer = [&](int L, uint64_t n) {
-        if (n * 64ull >= (1ull << 32))
-            throw Error(EGTB_ERR_ARG, "layer " + std::to_string(L) + " has too many states for one step");
+    // step i's candidates are counted as c2 | c4 << 32 (one atomic gives a
+    // block both write positions), exact while each half stays below 2^32
+    auto check_step = [&](int i) {
+        if (H[i].c2x >= (1ull << 32) || H[i].c4x >= (1ull << 32))
+            throw Error(EGTB_ERR_ARG, "forward step " + std::to_string(i) +
+                                          " has 2^32 or more candidates of one spawn value");
     };
     // layer L (size known, sets drained) sorted into the raw arena
     auto sort_layer = [&](int L, uint64_t n) {
@@ -1811,7 +1838,6 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
                           o.forward_hash_steps != 1;
     if (hash_fwd) {
         res.info[0].n_raw = n_seeds;
-        check_layer(0, n_seeds);
         const uint64_t c1 = cap_for(1, 0), c2 = cap_for(2, 0);
         if (c1 + c2 <= budget_keys(c1 + c2) && std::max(c1, c2) <= max_slots) {
             queue_raw(0, n_seeds);
@@ -1861,7 +1887,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
                 res.info[i].c4 = H[i].tot >> 32;
                 stats_add_bytes("expand", 8.0 * (double)res.info[i - 1].n_raw +
                                               8.0 * (double)(res.info[i].c2 + res.info[i].c4));
-                check_layer(i, n);
+                check_step(i);
                 sort_layer(i, n);
                 if (handover) {
                     join_sorts();  // the sort loop reads the sorted layers on the build stream
@@ -1882,7 +1908,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
                         Timer t(s, "expand", 0.0);
                         launch_expand_alloc(s, C.shape, res.raw[i - 1], res.info[i - 1].n_raw, &M[i - 1].n_raw,
                                             C.lut, p, i > o.docheck_step ? 1 : 0, (uint64_t *)cnt_dev.p, nullptr,
-                                            (uint64_t *)cur.p, &M[i].or_all, &M[i + 1].or_all, 0, c4);
+                                            (uint64_t *)cur.p, &M[i].or_all, &M[i + 1].or_all, 0, c4, b.ja(i));
                     }
                     c4_prev = c4;
                     const uint64_t cap2 = cur.bytes / 8 - c4, cap4 = nxt.bytes / 8;
@@ -1925,8 +1951,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
         res.info[i - 1].n_raw = n_par;
         queue_raw(i - 1, n_par);
         if (streamer) streamer->reserve(n_par);
-        if (n_par * 64ull >= (1ull << 32))
-            throw Error(EGTB_ERR_ARG, "layer " + std::to_string(i - 1) + " has too many states for one step");
+        check_step(i);
         const uint64_t c2 = H[i].tot & 0xFFFFFFFFull, c4 = H[i].tot >> 32;
         res.info[i].c2 = c2;
         res.info[i].c4 = c4;
@@ -1987,7 +2012,8 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
         } else {
             // parent ranges whose candidates fit the budget, from the prefix
             res.chunked_steps++;
-            uint32_t K = (uint32_t)std::max<uint64_t>(1, (std::max(c2, c4) + budget - 1) / budget);
+            const JobArg ja_i = b.ja(i);
+            uint32_t K =(uint32_t)std::max<uint64_t>(1, (std::max(c2, c4) + budget - 1) / budget);
             std::vector<uint64_t> bnd, ofs;
             for (;;) {
                 bnd.assign(K + 1, 0);
@@ -2020,7 +2046,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
                     Timer t(s, "expand", 8.0 * (double)(b - a) + 8.0 * (double)(k2 + k4));
                     launch_expand(s, true, C.shape, res.raw[i - 1] + a, b - a, nullptr, C.lut, p, do_check, nullptr,
                                   (const uint64_t *)offs.p + a, (uint64_t *)chunk2.p, (uint64_t *)chunk4.p, nullptr,
-                                  nullptr, ofs[k]);
+                                  nullptr, ofs[k], ~0ull, ~0ull, ja_i);
                 }
                 to_run((uint64_t *)chunk2.p, k2, H[i].or_all, H[i].and_all, runs_cur, runs_cur_n);
                 to_run((uint64_t *)chunk4.p, k4, H[i + 1].or_all, H[i + 1].and_all, runs_next, runs_next_n);
@@ -2121,7 +2147,7 @@ static void final_layers(BuildCtx<T> &b) {
     rates.ensure(max_n * sizeof(T) + 8, s);
     for (int i = steps - 2; i <= steps - 1; ++i) {
         const uint64_t n = res.info[i].n_raw;
-        launch_final_rates<T>(s, res.raw[i], n, p, max_scale, zero, (T *)rates.p);
+        launch_final_rates<T>(s, res.raw[i], n, p, max_scale, zero, (T *)rates.p, b.ja(i));
         make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, true, zero);
     }
     if (raw_writer) raw_writer->finish();
@@ -2205,7 +2231,7 @@ static void backward_pass(BuildCtx<T> &b) {
                 Timer t(s, "solve", (double)n * (8 + sizeof(T)));
                 launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
                                 hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, zero, ri, next,
-                                own);
+                                own, b.ja(i));
             }
             carve_book<T>(res, i, n);  // before the fork: a new arena chunk is allocated on the build stream
             HIPCHK(hipEventRecord(fork_ev, s));
@@ -2241,7 +2267,7 @@ static void backward_pass(BuildCtx<T> &b) {
             Timer t(s, "solve", (double)n * (8 + sizeof(T)));
             launch_solve<T>(s, C.shape, res.raw[i], n, C.lut, p, i > o.docheck_step ? 1 : 0, hash_of(b1),
                             hash_of(b2), table_of<T>(b1), table_of<T>(b2), w2, w4, max_scale, zero, (T *)rates.p,
-                            hs);
+                            hs, HashDev{}, b.ja(i));
         }
         make_book<T>(s, res, i, res.raw[i], (const T *)rates.p, n, w, i > 0, zero, hs.present());
         drop_hash(res, i + 2);  // its last reader was this layer's solve
@@ -2458,8 +2484,13 @@ static void compress_pass(BuildCtx<T> &b) {
 // post-solve passes (optimal branches, compression) and the timings.
 template <typename T>
 void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const egtb_options &o, Result &res,
-               bool one_minus) {
+               bool one_minus, const JobDev *jobs = nullptr, uint32_t *job_err = nullptr) {
     BuildCtx<T> b(o, p, res, one_minus);
+    b.jobs = jobs;
+    b.job_err = job_err;
+    if (jobs && (o.write_files || o.host_tables || o.optimal_branch_only || o.deletion_threshold > 0.0 ||
+                 o.compress || o.write_raw_files))
+        throw Error(EGTB_ERR_ARG, "batched builds keep their books in HBM: no files, host tables or pruning");
     hipStream_t s = b.s;
     const int steps = b.steps;
     const bool files = b.files;
@@ -2576,6 +2607,13 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     res.bwd_ms = std::chrono::duration<double, std::milli>(t_end - b.t_fwd).count();
     res.total_ms = std::chrono::duration<double, std::milli>(t_end - b.t_begin).count();
     progress_set(b.prog_total, b.prog_total);
+}
+
+void run_build_jobs(const uint64_t *keys, size_t n_keys, const egtb_options &o, const JobDev *jobs,
+                    uint32_t *job_err, Result &res) {
+    PatternDev unused{};
+    unused.any_success = 1;
+    run_build<float>(keys, n_keys, unused, o, res, false, jobs, job_err);
 }
 
 void build_dispatch(const uint64_t *seeds, size_t n_seeds, const egtb_pattern *pattern, const egtb_options *o,

@@ -84,7 +84,8 @@ class BatchOptions(ctypes.Structure):
         ("radix_bits", ctypes.c_int32),
         ("sort_schedule", ctypes.c_int32),
         ("keep_raw", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("forward_hash_steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 

@@ -161,18 +161,20 @@ typedef struct egtb_batch_job {
 
 typedef struct egtb_batch_options {
     int32_t device;
-    int32_t radix_bits;     /* 0 = autotuned */
-    int32_t sort_schedule;  /* 0 = autotuned */
+    int32_t radix_bits;          /* 0 = autotuned */
+    int32_t sort_schedule;       /* 0 = autotuned */
     int32_t keep_raw;
-    int32_t reserved[4];
+    int32_t forward_hash_steps;  /* as egtb_options::forward_hash_steps */
+    int32_t reserved[3];
 } egtb_batch_options;
 
 typedef struct egtb_batch_result egtb_batch_result;
 
-/* Build every job: one set of kernel launches per forward step and per
- * backward layer covers all jobs.  Each job's layers equal its egtb_build
- * layers.  No reference counterpart (the reference builds one table per
- * run_pattern_build call). */
+/* Build every job through the plain build's schedule over job keys
+ * ((board >> 8) | job << 56): one set of kernel launches per forward step
+ * and per backward layer covers all jobs.  Each job's layers equal its
+ * egtb_build layers.  No reference counterpart (the reference builds one
+ * table per run_pattern_build call). */
 EGTB_EXPORT int egtb_batch_build(const egtb_batch_job *jobs, int n_jobs, const egtb_batch_options *options,
                                  egtb_batch_result **out);
 EGTB_EXPORT int egtb_batch_timing(const egtb_batch_result *b, double *fwd_ms, double *bwd_ms, double *total_ms);

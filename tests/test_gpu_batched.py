@@ -52,16 +52,19 @@ def test_batch_rejects_unpinned_patterns(pkg):
     assert rc != 0 and "pin nibbles" in native.last_error()
 
 
-def test_batch_key_sets_match_sorted_dedup(pkg):
-    """The batched forward pass deduplicates in key sets by default; with an
-    explicit sort schedule it sorts and uniques every step. Both give every
-    job the same books."""
+@pytest.mark.parametrize("variant", [dict(sort_schedule=1), dict(forward_hash_steps=-1),
+                                     dict(forward_hash_steps=-1, sort_schedule=2), dict(forward_hash_steps=40)])
+def test_batch_key_sets_match_sorted_dedup(pkg, variant):
+    """The batched forward pass deduplicates in key sets by default; with
+    forward_hash_steps = -1 it sorts and uniques every step's candidates, and
+    with k > 0 it hands over to sort + unique at step k.  Every variant gives
+    every job the same books."""
     from importlib import import_module
 
     batched = import_module("2048endgametablebase_amd.batched")
     jobs = [("442", 7), ("LL", 6), ("t", 6)]
-    print("case batched key sets vs sort schedule 1", flush=True)
-    with batched.build_batch(jobs) as a, batched.build_batch(jobs, sort_schedule=1) as b:
+    print(f"case batched key sets vs {variant}", flush=True)
+    with batched.build_batch(jobs) as a, batched.build_batch(jobs, **variant) as b:
         for j, (pat, t) in enumerate(jobs):
             for i in range(pkg.patterns.build_params(pat, t)["steps"]):
                 ka, ra = a.book(j, i)
