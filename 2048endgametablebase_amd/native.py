@@ -165,6 +165,12 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_size_t)]),
     "egtb_zbook_write": (ctypes.c_int, [ctypes.c_char_p, u64p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]),
     "egtb_board_moves": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int32, u64p, u64p, ctypes.c_int]),
+    "egtb_ai_search": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32)]),
+    "egtb_ai_table": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int16), ctypes.c_int]),
+    "egtb_ai_evaluate": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int32)]),
     "egtb_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(ctypes.c_void_p)]),
     "egtb_server_attach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "egtb_server_detach": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

@@ -246,6 +246,35 @@ EGTB_EXPORT int egtb_table_set_prefer_max(egtb_table *t, int32_t on);
 EGTB_EXPORT int egtb_board_moves(const uint64_t *boards, size_t n, int32_t symm_mode, uint64_t *out4,
                                  uint64_t *canon, int device);
 
+/* ---- heuristic expectimax player (the dispatcher's AI) ------------------- */
+
+/* AIPlayer(board).start_search(depth) for n boards at once on the device
+ * (reference native_core/src/AIPlayer.cpp:893-941, bound as
+ * ai_core.AIPlayer in bindings_ai.cpp:20-56 and called by the AI dispatcher,
+ * engine_core/AIPlayer.py:575-1010): apply_dynamic_mask, then
+ * search_ai_player(masked board, depth), then the search again on the
+ * unmasked board when the reference's condition holds.  depth 1..3;
+ * do_check / prune / spawn_rate4 as the player's members of those names.
+ * best (n): the best operation, 1..4 = left, right, up, down, 0 when no
+ * move is valid; scores (n x 4): top_scores, -dead_score for invalid moves.
+ * Equal to the reference's single-thread search up to depth 2 (its value
+ * cache matters there only on a signature collision); at depth 3 each board
+ * is searched at its own depth, where the reference may reuse a deeper
+ * cached value. */
+EGTB_EXPORT int egtb_ai_search(const uint64_t *boards, size_t n, int depth, int do_check, int prune,
+                               double spawn_rate4, int device, int32_t *best, int32_t *scores);
+/* AIPlayer::evaluate (AIPlayer.cpp:645-667) of n boards on the device, with
+ * table variant `variant` (see egtb_ai_table), do_check and the masked count
+ * that selects check_corner's shapes (AIPlayer.cpp:470-643). */
+EGTB_EXPORT int egtb_ai_evaluate(const uint64_t *boards, size_t n, int variant, int do_check, int masked_count,
+                                 int device, int32_t *out);
+/* The player's evaluation table (host only, no device): variant 0 is
+ * init_evaluate_tables' table (AIPlayer.cpp:103-127); variants 1..15 carry
+ * the adjustments update_specific_scores makes (AIPlayer.cpp:254-336) for
+ * flags (bit 0 flag1 .. bit 3 flag4) when the 2x2 corner shape is present.
+ * out: 65536 x {d1, d2}. */
+EGTB_EXPORT int egtb_ai_table(int16_t *out, int variant);
+
 /* ---- serving: continuous batching over resident tables -------------------- */
 
 /* A request loop that keeps tables in HBM and answers position queries.

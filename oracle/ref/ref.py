@@ -1,5 +1,6 @@
 """ctypes loader for oracle/_ref/libegtb_ref.so: the reference's own classic
-builder and classic reader compiled from their sources by oracle/ref/Makefile.
+builder, classic reader and expectimax player compiled from their sources by
+oracle/ref/Makefile.
 
 TEST INFRASTRUCTURE ONLY (golden-vector generation, CPU baseline timing).
 """
@@ -71,6 +72,18 @@ def lib():
         f.restype = ctypes.c_int
         f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p,
                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]
+        L.egtb_ref_ai_table.restype = None
+        L.egtb_ref_ai_table.argtypes = [ctypes.c_void_p]
+        L.egtb_ref_ai_table_adjusted.restype = None
+        L.egtb_ref_ai_table_adjusted.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_void_p]
+        L.egtb_ref_ai_evaluate.restype = ctypes.c_int32
+        L.egtb_ref_ai_evaluate.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+        L.egtb_ref_ai_search.restype = None
+        L.egtb_ref_ai_search.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -191,3 +204,34 @@ def trie_search(path_prefix, board, dtype):
     if rc < 0:
         raise RuntimeError("trie_decompress_search_native raised")
     return v.value if rc else None
+
+
+def ai_table():
+    """The reference's evaluation table (_diffs_merged after
+    init_evaluate_tables): int16 array [65536, 2] of (d1, d2)."""
+    out = np.zeros((65536, 2), dtype=np.int16)
+    lib().egtb_ref_ai_table(out.ctypes.data)
+    return out
+
+
+def ai_table_adjusted(board, f1, f2, f3, f4):
+    """The table as update_specific_scores(board, f1, f2, f3, f4) leaves it."""
+    out = np.zeros((65536, 2), dtype=np.int16)
+    lib().egtb_ref_ai_table_adjusted(int(board), int(f1), int(f2), int(f3), int(f4), out.ctypes.data)
+    return out
+
+
+def ai_evaluate(board, do_check=False, masked_count=0):
+    """AIPlayer::evaluate(board) with the unadjusted table."""
+    return int(lib().egtb_ref_ai_evaluate(int(board), int(bool(do_check)), int(masked_count)))
+
+
+def ai_search(board, depth, do_check=False, prune=False, spawn_rate4=0.1):
+    """AIPlayer(board).start_search(depth) on one thread: (best operation
+    1..4 or 0, [4 top scores], dead_score, masked_count, fixed_mask)."""
+    best, dead, mc, fm = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64()
+    scores = np.zeros(4, dtype=np.int32)
+    lib().egtb_ref_ai_search(int(board), int(depth), int(bool(do_check)), int(bool(prune)), float(spawn_rate4),
+                             ctypes.byref(best), scores.ctypes.data, ctypes.byref(dead), ctypes.byref(mc),
+                             ctypes.byref(fm))
+    return int(best.value), [int(x) for x in scores], int(dead.value), int(mc.value), int(fm.value)
