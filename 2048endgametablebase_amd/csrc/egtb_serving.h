@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "egtb.h"
+#include "egtb_ai.h"
 #include "egtb_kernels.h"
 #include "egtb_runtime.h"
 
@@ -28,6 +29,7 @@ struct TableSet {
     DBuf tabs;  // TableDev<T>[steps] on the device
     size_t bytes = 0;
     const Result *borrowed = nullptr;  // tables owned by a build result
+    AiTables ai{};                     // the off-table fallback's evaluation tables on this device
 };
 
 void configure_serving_kernels();

@@ -6,8 +6,11 @@
 // boards: layer = (tile sum - seed tile sum) / 2; the first symmetry op
 // (0..7) under which the board is in the pattern and has a valid move gives
 // the four directional rates (NaN = invalid move), remapped to absolute
-// directions.  Boards that no op answers take a depth-2 expectimax fallback
-// with a fixed empty-cell heuristic (status EGTB_LOOKUP_FALLBACK).
+// directions.  Boards that no op answers get the heuristic expectimax
+// player's scores instead (status EGTB_LOOKUP_FALLBACK): the reference hands
+// such positions to its AIPlayer (engine_core/AIPlayer.py:273-291 ->
+// AIPlayer::start_search); here its search at depth 1 -- two moves deep:
+// move, spawn, move, evaluate -- runs in the lookup's own thread.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -20,6 +23,7 @@
 #include <vector>
 
 #include "egtb.h"
+#include "egtb_ai.h"
 #include "egtb_device.h"
 #include "egtb_kernels.h"
 #include "egtb_runtime.h"
@@ -59,43 +63,13 @@ __device__ __forceinline__ uint32_t tile_sum(uint64_t b) {
     return s;
 }
 
-// Heuristic of the fallback: fraction of empty cells.
-__device__ __forceinline__ double fallback_h(uint64_t b) { return (double)__popc(empty_cells(b)) / 16.0; }
-
-// Depth-2 expectimax value of playing move d on b: mean over the spawns on
-// the moved board (2 with 1-p4, 4 with p4, uniform cell) of the best next
-// move's heuristic (0 when the spawned board has no move).
-__device__ double fallback_value(uint64_t moved, const uint16_t *lut, double p4) {
-    uint32_t em = empty_cells(moved);
-    const int e = __popc(em);
-    if (e == 0) return fallback_h(moved);
-    double acc = 0.0;
-    const SpawnMoves smv(moved, lut);
-    while (em) {
-        const int k = __builtin_ctz(em);
-        em &= em - 1;
-#pragma unroll
-        for (int s = 1; s <= 2; ++s) {
-            const uint64_t sp = moved | ((uint64_t)s << (4 * k));
-            uint64_t mv[4];
-            smv.of(k, (uint64_t)s, lut, mv);
-            double best = 0.0;
-#pragma unroll
-            for (int d = 0; d < 4; ++d)
-                if (mv[d] != sp) best = fmax(best, fallback_h(mv[d]));
-            acc += (s == 1 ? 1.0 - p4 : p4) * best;
-        }
-    }
-    return acc / (double)e;
-}
-
 template <typename T, typename OutT>
 __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ boards, uint64_t n,
                                                  const TableDev<T> *__restrict__ tabs, int steps,
                                                  int64_t nums_adjust, PatternDev p, double scale, double zero,
                                                  double p4,
                                                  int prefer_max, int round9, const uint16_t *__restrict__ glut,
-                                                 OutT *__restrict__ out4,
+                                                 AiTables ai, int64_t w4, OutT *__restrict__ out4,
                                                  int32_t *__restrict__ status, int32_t *__restrict__ op_out) {
     extern __shared__ uint16_t lut[];
     stage_lut(lut, glut);
@@ -144,12 +118,16 @@ __global__ __launch_bounds__(1024) void k_lookup(const uint64_t *__restrict__ bo
             }
         }
         if (used < 0) {
+            // the player's four top scores; NaN where the move does not
+            // change the queried board
             st = EGTB_LOOKUP_FALLBACK;
+            int32_t sc[4];
+            ai_start_search<1>(b, ai, 0, 0, w4, sc);
             uint64_t mv[4];
             move4(b, lut, mv);
 #pragma unroll
             for (int d = 0; d < 4; ++d)
-                if (mv[d] != b) v[d] = (OutT)fallback_value(mv[d], lut, p4);
+                if (mv[d] != b) v[d] = (OutT)sc[d];
         }
 #pragma unroll
         for (int d = 0; d < 4; ++d) out4[q * 4 + d] = v[d];
@@ -200,9 +178,12 @@ static void launch_lookup_t(hipStream_t s, const TableSet &ts, const DeviceCtx &
     uint64_t want = (n + threads - 1) / threads;
     const uint64_t cap = (uint64_t)C.cus;
     const uint32_t grid = (uint32_t)(want < cap ? want : cap);
+    // the player's spawn weight of a 4, truncated to 1/65536 (AIPlayer.cpp:636-641)
+    const int64_t w4 = (int64_t)(ts.p4 * 65536.0);
     hipLaunchKernelGGL((k_lookup<T, OutT>), dim3(grid ? grid : 1), dim3(threads), kLutEntries * sizeof(uint16_t), s,
                        boards, n, (const TableDev<T> *)ts.tabs.p, ts.steps, ts.nums_adjust, ts.pattern, ts.scale,
-                       ts.zero, ts.p4, ts.prefer_max, ts.rate_size == 4 ? 1 : 0, C.lut, out4, status, op_out);
+                       ts.zero, ts.p4, ts.prefer_max, ts.rate_size == 4 ? 1 : 0, C.lut, ts.ai, w4, out4, status,
+                       op_out);
 }
 
 template <typename OutT>
@@ -267,6 +248,7 @@ std::unique_ptr<TableSet> table_from_files(const char *prefix, int steps, const 
     hipStream_t s = C.stream;
     auto ts = std::make_unique<TableSet>();
     ts->dev = device;
+    ts->ai = ai_tables(device);
     ts->steps = steps;
     ts->pattern = make_pattern(pat, target);
     ts->nums_adjust = -tile_sum;
@@ -354,6 +336,7 @@ std::unique_ptr<TableSet> table_from_result(const Result &r, const egtb_pattern 
     configure_serving_kernels();
     auto ts = std::make_unique<TableSet>();
     ts->dev = r.dev;
+    ts->ai = ai_tables(r.dev);
     ts->steps = r.steps;
     ts->pattern = make_pattern(pat, target);
     ts->nums_adjust = -tile_sum;

@@ -166,7 +166,7 @@ def serving_sweep(pkg, pattern, target, seconds):
     table.close()
     res.close()
     return {"workload": f"{pattern}@{2**target} table in HBM, 4096 games, Poisson arrivals, 20% off-table "
-                        "(depth-2 expectimax fallback)",
+                        "(the expectimax player's depth-1 search, two moves deep, as the fallback)",
             "qps_at_p99_lt_5ms": best["qps_achieved"] if best else None,
             "p99_ms_at_that_qps": best["p99_ms"] if best else None,
             "sweep_stopped": stop or "end of the rate list",
