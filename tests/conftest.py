@@ -38,8 +38,10 @@ def load_golden(tag):
 
 
 def golden_tags():
-    """Builder goldens (per-layer book digests); reader fixtures are reader_*."""
-    return sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json") and not f.startswith("reader_"))
+    """Builder goldens (per-layer book digests); reader fixtures are reader_*,
+    the expectimax player's ai_*."""
+    return sorted(f[:-5] for f in os.listdir(GOLDEN)
+                  if f.endswith(".json") and not f.startswith("reader_") and not f.startswith("ai_"))
 
 
 def reader_tags():
