@@ -33,9 +33,16 @@ struct DeviceCtx {
 };
 
 DeviceCtx &device_ctx(int dev);
-// hipMallocAsync / hipMalloc that, when the device is out of memory, hands
-// the pool's cached blocks back and tries once more
-hipError_t device_alloc(void **p, size_t nbytes, hipStream_t s, bool async);
+// Device blocks from the per-device cache (idle blocks of the request's size
+// class are reused after the taking stream waits for their release; new
+// blocks come from hipMalloc, and when the device is out of memory the idle
+// blocks are freed and the request tried once more).  The flag is unused.
+hipError_t device_alloc(void **p, size_t nbytes, hipStream_t s, bool async = true);
+// Back to the cache, idle once the work queued on `s` so far is done (s ==
+// nullptr: once the device is idle).
+void device_release(void *p, hipStream_t s);
+// Bytes of the current device's idle cached blocks: free for this library.
+size_t device_idle_bytes(int dev);
 
 // Move-only device buffer, allocated and freed in stream order.
 struct DBuf {

@@ -94,29 +94,141 @@ DeviceCtx &device_ctx(int dev) {
 // Stream-ordered device buffers
 // ---------------------------------------------------------------------------
 
-// Device memory, from the stream-ordered pool (async) or not.  The pool
-// keeps freed blocks for reuse (its release threshold is unlimited), and
-// those can starve an allocation of another size or of the other kind: on
-// failure the device is synchronised, the pool's unused blocks are handed
-// back and the allocation is tried once more.
+// Device memory: a per-device cache of blocks.  A freed block is kept,
+// idle, for the next request of its size class instead of going back to
+// the driver: after a build frees tens of GB, fresh allocations stalled for
+// seconds (single hipMalloc / hipMallocAsync calls of 1-3 s, with the pool
+// holding no memory and 284 GB free), which points at the driver clearing
+// the freed memory before handing it out again.  Classes are powers of two
+// up to 1 MiB and eight steps per power of two above (at most 12.5% over the
+// request).  A freed block carries an event recorded on the stream that freed
+// it, and the stream that takes it next waits for that event, so reuse keeps
+// stream order.  A block freed without a stream waits for the whole device
+// first, as hipFree would.  When the device runs out of memory the idle
+// blocks are freed (after a device synchronisation), the stream-ordered pool
+// is trimmed, and the request is tried once more.
 static void stats_add_host(const char *name, double ms);
 
-hipError_t device_alloc(void **p, size_t nbytes, hipStream_t s, bool async) {
+namespace {
+
+struct IdleBlock {
+    void *p;
+    hipEvent_t ev;
+};
+
+struct DevCache {
+    std::map<size_t, std::vector<IdleBlock>> idle;  // by class size
+    size_t idle_bytes = 0;
+};
+
+// never destroyed: buffers held by other static objects are released at
+// process exit, in any order
+struct CacheState {
+    std::mutex mu;
+    std::map<int, DevCache> devs;
+    std::map<void *, std::pair<int, size_t>> live;  // block -> (device, class size)
+    std::vector<hipEvent_t> free_events;
+};
+CacheState &cache_state() {
+    static CacheState *c = new CacheState();
+    return *c;
+}
+
+size_t size_class(size_t n) {
+    if (n <= 256) return 256;
+    if (n <= ((size_t)1 << 20)) {
+        size_t c = 256;
+        while (c < n) c <<= 1;
+        return c;
+    }
+    size_t p2 = (size_t)1 << (63 - __builtin_clzll((unsigned long long)n));
+    const size_t step = p2 / 8;
+    return (n + step - 1) / step * step;
+}
+
+// frees every idle block of device `dev` (caller holds the cache mutex)
+void flush_idle_locked(int dev) {
+    CacheState &C = cache_state();
+    (void)hipDeviceSynchronize();
+    DevCache &dc = C.devs[dev];
+    for (auto &kv : dc.idle)
+        for (IdleBlock &b : kv.second) {
+            (void)hipFree(b.p);
+            C.free_events.push_back(b.ev);
+        }
+    dc.idle.clear();
+    dc.idle_bytes = 0;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+}
+
+}  // namespace
+
+size_t device_idle_bytes(int dev) {
+    CacheState &C = cache_state();
+    std::lock_guard<std::mutex> lk(C.mu);
+    auto it = C.devs.find(dev);
+    return it == C.devs.end() ? 0 : it->second.idle_bytes;
+}
+
+hipError_t device_alloc(void **p, size_t nbytes, hipStream_t s, bool) {
     const auto t0 = std::chrono::steady_clock::now();
-    hipError_t e = async ? hipMallocAsync(p, nbytes, s) : hipMalloc(p, nbytes);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const size_t cls = size_class(nbytes);
+    CacheState &C = cache_state();
+    std::unique_lock<std::mutex> lk(C.mu);
+    DevCache &dc = C.devs[dev];
+    auto it = dc.idle.find(cls);
+    if (it != dc.idle.end() && !it->second.empty()) {
+        const IdleBlock b = it->second.back();
+        it->second.pop_back();
+        dc.idle_bytes -= cls;
+        C.live[b.p] = {dev, cls};
+        e = hipStreamWaitEvent(s, b.ev, 0);
+        C.free_events.push_back(b.ev);
+        *p = b.p;
+        stats_add_host("host_alloc_reused", 0.0);
+        return e;
+    }
+    e = hipMalloc(p, cls);
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
         (void)hipGetLastError();
-        int dev = 0;
-        hipMemPool_t pool;
-        if (hipGetDevice(&dev) == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
-            hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && hipMemPoolTrimTo(pool, 0) == hipSuccess) {
-            stats_add_host("host_pool_trim", 0.0);
-            e = async ? hipMallocAsync(p, nbytes, s) : hipMalloc(p, nbytes);
-        }
+        flush_idle_locked(dev);
+        stats_add_host("host_cache_flush", 0.0);
+        e = hipMalloc(p, cls);
     }
-    stats_add_host(async ? "host_alloc_async" : "host_alloc",
+    if (e == hipSuccess) C.live[*p] = {dev, cls};
+    lk.unlock();
+    stats_add_host("host_alloc",
                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     return e;
+}
+
+void device_release(void *p, hipStream_t s) {
+    if (!p) return;
+    CacheState &C = cache_state();
+    std::lock_guard<std::mutex> lk(C.mu);
+    auto it = C.live.find(p);
+    if (it == C.live.end()) return;
+    const int dev = it->second.first;
+    const size_t cls = it->second.second;
+    C.live.erase(it);
+    if (!s) (void)hipDeviceSynchronize();
+    hipEvent_t ev = nullptr;
+    if (!C.free_events.empty()) {
+        ev = C.free_events.back();
+        C.free_events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipDeviceSynchronize();  // no event: the block is idle once the device is
+        (void)hipFree(p);
+        return;
+    }
+    (void)hipEventRecord(ev, s);
+    DevCache &dc = C.devs[dev];
+    dc.idle[cls].push_back(IdleBlock{p, ev});
+    dc.idle_bytes += cls;
 }
 
 void DBuf::alloc(size_t nbytes, hipStream_t s) {
@@ -134,7 +246,7 @@ void DBuf::alloc(size_t nbytes, hipStream_t s) {
 }
 
 void DBuf::release() {
-    if (p) (void)hipFreeAsync(p, stream);
+    if (p) device_release(p, stream);
     p = nullptr;
     bytes = 0;
 }
@@ -548,14 +660,7 @@ void Arena::trim_last(void *p, size_t bytes) {
     if (end < c.used) c.used = end;
 }
 
-void Arena::free_chunk(Chunk &c) {
-    const auto t0 = std::chrono::steady_clock::now();
-    if (stream_)
-        (void)hipFreeAsync(c.p, stream_);
-    else
-        (void)hipFree(c.p);
-    stats_add_host("host_free", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-}
+void Arena::free_chunk(Chunk &c) { device_release(c.p, stream_); }
 
 void Arena::release_after(const void *p) {
     for (size_t k = 0; k < chunks_.size(); ++k) {
@@ -631,7 +736,7 @@ void BookRing::release(int layer) {
 }
 
 void BookRing::free_all() {
-    for (auto &c : chunks_) (void)hipFreeAsync(c.p, stream_);
+    for (auto &c : chunks_) device_release(c.p, stream_);
     chunks_.clear();
     spare_.clear();
     cur_ = -1;
@@ -1575,6 +1680,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
         if (budget_cache == 0 || need > budget_cache / 2) {
             size_t fr = 0, tot = 0;
             HIPCHK(hipMemGetInfo(&fr, &tot));
+            fr += device_idle_bytes(o.device);  // the cache's idle blocks are free to this build
             // a chunk's two streams, the sort scratch and the unique output
             budget_cache = std::max<uint64_t>(1ull << 20, (uint64_t)(fr / 8 / 6));
         }
@@ -2279,7 +2385,9 @@ static void backward_pass(BuildCtx<T> &b) {
             streamer->reclaim();
             if (!o.keep_raw) res.raw_arena.release_after(res.raw[i]);
             size_t fr = 0, tot = 0;
-            if (hipMemGetInfo(&fr, &tot) == hipSuccess) res.peak_device_bytes = std::max<uint64_t>(res.peak_device_bytes, tot - fr);
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess)
+                res.peak_device_bytes =
+                    std::max<uint64_t>(res.peak_device_bytes, tot - fr - device_idle_bytes(o.device));
         }
         if (files) {
             read_meta(s, res, H, i, i);
@@ -2331,12 +2439,11 @@ static void backward_pass(BuildCtx<T> &b) {
         streamer->retire(1, s);
         streamer->retire(0, s);
         streamer->finish();
-        // hand the pooled chunks of this build back to the device
-        hipMemPool_t pool;
+        // the build's chunks go back to the device cache (idle, for later
+        // builds)
         res.book_ring.free_all();
         if (!o.keep_raw) res.raw_arena.release();
         HIPCHK(hipStreamSynchronize(s));
-        if (hipDeviceGetDefaultMemPool(&pool, res.dev) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
     }
     // the build stream's look-back error flag (the second stream's is read
     // at the end of the forward pass): a raised flag means a partial prefix
@@ -2504,7 +2611,12 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     res.raw.assign(steps, nullptr);
     res.books.assign(steps, Book{});
     res.info.assign(steps, LayerInfo{});
-    if (o.host_tables) res.raw_arena.set_stream(s);
+    // the arenas' chunks come from the stream-ordered pool, which keeps the
+    // memory earlier builds freed: a plain hipMalloc of a 1 GiB chunk after a
+    // build that freed tens of GB into the pool stalled for seconds (3.2 s
+    // and 1.6 s in single calls) while the driver reclaimed that memory
+    res.raw_arena.set_stream(s);
+    res.book_arena.set_stream(s);
     res.host_tables = o.host_tables != 0;
     if (res.host_tables) {
         if (files) throw Error(EGTB_ERR_ARG, "host_tables and write_files cannot be combined");
@@ -2578,6 +2690,7 @@ void run_build(const uint64_t *seeds, size_t n_seeds, const PatternDev &p, const
     // fits with one stream never fails for want of the second's buffers
     size_t hbm_free = 0, hbm_total = 0;
     HIPCHK(hipMemGetInfo(&hbm_free, &hbm_total));
+    hbm_free += device_idle_bytes(o.device);
     b.two_stream = steps >= 3 && !files && !b.streamer && !res.book_index && o.deletion_threshold <= 0.0 &&
                    (double)rates_all_bytes <= kTwoStreamRatesMax && rates_all_bytes + 256 <= (uint64_t)hbm_free / 2;
     if (b.two_stream) res.hash_ring = 4;
