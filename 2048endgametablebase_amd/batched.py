@@ -28,10 +28,15 @@ DEFAULT_JOBS: List[Tuple[str, int]] = [
     ("4432", 5), ("4441", 5), ("4442", 4),
 ]
 
-# Many small builds: the regime where per-layer dispatch dominates a plain
-# build and batching shares it across all jobs.
+# 24 jobs at targets 16..64 of every 10- and 12-free-cell pattern family:
+# merged layers of ~1e7 states, past the key-set limit.
 SMALL_JOBS: List[Tuple[str, int]] = [(p, t) for p in ("L3", "442", "t", "LL", "444", "4431")
                                      for t in (4, 5, 6, 6)]
+
+# 24 tiny 10-free-cell jobs at targets 8..32 (layers of 1e3..1e5 states):
+# the regime where a plain build is its fixed per-layer cost (L3@8: 52
+# layers in ~3.7 ms) and batching shares that cost across all jobs.
+TINY_JOBS: List[Tuple[str, int]] = [(p, t) for p in ("L3", "442", "t") for t in (3, 3, 4, 4, 5, 5, 3, 4)]
 
 
 class BatchResult:

@@ -353,7 +353,8 @@ def main():
         bm.build_batch(bm.SMALL_JOBS).close()  # warm-up
         batched = {"workload_24_jobs": [f"{p}@{2**t}" for p, t in bm.DEFAULT_JOBS],
                    "uneven_24_jobs": bm.compare_with_plain(bm.DEFAULT_JOBS),
-                   "small_24_jobs": bm.compare_with_plain(bm.SMALL_JOBS, reps=3)}
+                   "small_24_jobs": bm.compare_with_plain(bm.SMALL_JOBS, reps=3),
+                   "tiny_24_jobs": bm.compare_with_plain(bm.TINY_JOBS, reps=3)}
 
     serving = None
     if not args.no_serving:
