@@ -14,6 +14,7 @@ DEFAULTS = {
     "direct_io_queue_depth": 16,
     "direct_io_chunk_mib": 8,
     "deletion_threshold": 0.0,
+    "SmallTileSumLimit": 96,
     "4_spawn_rate": 0.1,
     "success_rate_dtype": "uint32",
     "filepath_map": {},

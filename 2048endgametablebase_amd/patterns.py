@@ -198,3 +198,20 @@ def build_params(pattern: str, target: int) -> Dict[str, object]:
         "tile_sum": meta.tile_sum,
         "is_free": is_free,
     }
+
+
+def ad_params(pattern: str, target: int, small_tile_sum_limit: int = 96) -> Dict[str, object]:
+    """build_params plus what the advanced (masked) builder needs
+    (engine_core/BookBuilder.py:90-108 with Config.py:250-256): the number of
+    32768 tiles of the first seed that no mask pins (``n_free_32k``), the
+    shifts of the pinned ones (``fixed_32k_shifts``, cells that every mask
+    holds at 0xF) and the small-tile sum limit (SmallTileSumLimit, default
+    96)."""
+    p = build_params(pattern, target)
+    meta = pattern_catalog[pattern]
+    seed = int(meta.seed_boards[0])
+    count = sum(1 for k in range(16) if (seed >> (4 * k)) & 0xF == 0xF)
+    fixed = list(meta.fixed_positions)
+    p.update(n_free_32k=count - len(fixed), fixed_32k_shifts=fixed,
+             small_tile_sum_limit=int(small_tile_sum_limit))
+    return p

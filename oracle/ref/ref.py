@@ -1,6 +1,6 @@
 """ctypes loader for oracle/_ref/libegtb_ref.so: the reference's own classic
-builder, classic reader and expectimax player compiled from their sources by
-oracle/ref/Makefile.
+builder, classic reader, advanced (AD) builder and expectimax player compiled
+from their sources by oracle/ref/Makefile.
 
 TEST INFRASTRUCTURE ONLY (golden-vector generation, CPU baseline timing).
 """
@@ -79,6 +79,11 @@ def lib():
                                                  ctypes.c_int, ctypes.c_void_p]
         L.egtb_ref_ai_evaluate.restype = ctypes.c_int32
         L.egtb_ref_ai_evaluate.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+        f = L.egtb_ref_build_ad
+        f.restype = ctypes.c_int
+        P, Z, I = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+        f.argtypes = [P, Z, P, Z, I, I, P, Z, I, I, I, I, ctypes.c_char_p, I, ctypes.c_double, ctypes.c_char_p,
+                      I, I, ctypes.c_char_p, Z]
         L.egtb_ref_ai_search.restype = None
         L.egtb_ref_ai_search.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p,
@@ -104,6 +109,28 @@ def run_build(seeds, masks, shifts, symm, target, steps, docheck, pathname, is_f
                                      int(is_free), p4, dtype.encode(), threads,
                                      float(deletion_threshold), int(bool(optimal_branch_only)),
                                      int(bool(compress)), err, 1024)
+        if rc == 0:
+            return
+        msg = err.value.decode(errors="replace")
+        if attempt or not msg.startswith("The length multiplier"):
+            raise RuntimeError(msg)
+
+
+def run_build_ad(seeds, masks, symm, n_free_32k, fixed_shifts, target, steps, docheck, pathname,
+                 is_free=False, small_tile_sum_limit=96, p4=0.1, dtype="float32", threads=8,
+                 stop_after_forward=True) -> None:
+    """Run the reference's advanced builder run_pattern_build_ad_cpp; with
+    stop_after_forward its raw layer files <pathname>0 .. <pathname>steps-2
+    are left in place and the solver does not run."""
+    s = np.ascontiguousarray(seeds, dtype=np.uint64)
+    m = np.ascontiguousarray(masks if len(masks) else [0], dtype=np.uint64)
+    fx = np.ascontiguousarray(fixed_shifts if len(fixed_shifts) else [0], dtype=np.uint8)
+    for attempt in range(2):
+        err = ctypes.create_string_buffer(1024)
+        rc = lib().egtb_ref_build_ad(s.ctypes.data, s.size, m.ctypes.data, len(masks), symm, n_free_32k,
+                                     fx.ctypes.data, len(fixed_shifts), small_tile_sum_limit, target, steps,
+                                     docheck, pathname.encode(), int(is_free), p4, dtype.encode(), threads,
+                                     int(bool(stop_after_forward)), err, 1024)
         if rc == 0:
             return
         msg = err.value.decode(errors="replace")

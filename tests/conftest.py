@@ -39,9 +39,20 @@ def load_golden(tag):
 
 def golden_tags():
     """Builder goldens (per-layer book digests); reader fixtures are reader_*,
-    the expectimax player's ai_*."""
+    the expectimax player's ai_*, the advanced forward pass's ad_*."""
     return sorted(f[:-5] for f in os.listdir(GOLDEN)
-                  if f.endswith(".json") and not f.startswith("reader_") and not f.startswith("ai_"))
+                  if f.endswith(".json") and not f.startswith("reader_") and not f.startswith("ai_")
+                  and not f.startswith("ad_"))
+
+
+def ad_tags():
+    """Advanced (masked) forward-pass goldens: per raw layer file digests."""
+    return sorted(f[3:-5] for f in os.listdir(GOLDEN) if f.startswith("ad_") and f.endswith(".json"))
+
+
+def load_ad_golden(tag):
+    with open(os.path.join(GOLDEN, f"ad_{tag}.json")) as fh:
+        return json.load(fh)
 
 
 def reader_tags():

@@ -1,0 +1,87 @@
+"""Generate golden vectors of the advanced (AD, large-tile masking) forward
+pass from the reference's own builder.
+
+Runs oracle/_ref/libegtb_ref.so (the reference C++ compiled from its sources
+by oracle/ref/Makefile, in this container only): run_pattern_build_ad_cpp
+(native_core/src/BookGeneratorAD.cpp:1421-1438), stopped after its forward
+pass (generate_process_ad, :1256-1396), and writes
+tests/golden/ad_<pattern>_<tile>.json holding, per raw layer file
+<pathname><i> (i = 0 .. steps-2):
+
+  * ``n``      -- number of uint64 boards in the file;
+  * ``sha256`` -- SHA-256 of the file's bytes (the layer, sorted and unique
+                  for i >= 1; the masked seeds as given for i = 0);
+  * ``sample`` -- up to 8 evenly spaced boards (hex).
+
+The spec is engine_core/BookBuilder.py:90-108's: SmallTileSumLimit 96, the
+32768 tiles of the first seed split into free and pinned ones.
+
+Usage:  python tests/golden/make_ad_golden.py [name ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import importlib
+import json
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+sys.path.insert(0, ROOT)
+
+from oracle.ref import ref  # noqa: E402
+
+patterns = importlib.import_module("2048endgametablebase_amd.patterns")
+
+# (pattern, log2 target): identity (442, L3, t with free 32768 tiles),
+# diagonal (LL) and horizontal (444) canonical forms
+RUNS = [("442", 6), ("L3", 6), ("t", 6), ("442", 7), ("L3", 7), ("t", 7), ("LL", 6), ("444", 6),
+        ("442", 8), ("444", 5)]
+
+
+def digest_run(name, t, workdir):
+    p = patterns.ad_params(name, t)
+    prefix = os.path.join(workdir, f"{name}_{2 ** t}_")
+    ref.run_build_ad(p["seeds"], p["masks"], p["symm_mode"], p["n_free_32k"], p["fixed_32k_shifts"], t,
+                     p["steps"], p["docheck_step"], prefix, is_free=p["is_free"],
+                     small_tile_sum_limit=p["small_tile_sum_limit"], threads=8)
+    layers = []
+    for i in range(p["steps"] - 1):
+        raw = open(prefix + str(i), "rb").read()
+        a = np.frombuffer(raw, dtype="<u8")
+        idx = np.unique(np.linspace(0, max(a.size - 1, 0), num=min(8, a.size)).astype(np.int64))
+        layers.append({"i": i, "n": int(a.size), "sha256": hashlib.sha256(raw).hexdigest(),
+                       "sample": [f"{int(a[j]):016x}" for j in idx] if a.size else []})
+    return {
+        "pattern": name, "target": t, "steps": p["steps"], "seeds": [f"{s:016x}" for s in p["seeds"]],
+        "masks": [f"{m:x}" for m in p["masks"]], "symm": p["symm_mode"], "n_free_32k": p["n_free_32k"],
+        "fixed_32k_shifts": p["fixed_32k_shifts"], "small_tile_sum_limit": p["small_tile_sum_limit"],
+        "source": "reference run_pattern_build_ad_cpp forward pass, g++ -Ofast -march=x86-64-v3 -flto "
+        "(oracle/ref/Makefile)", "layers": layers,
+    }
+
+
+def main(argv):
+    if not ref.available() and not ref.build():
+        raise SystemExit("reference sources not present; cannot make golden vectors")
+    want = set(argv)
+    for name, t in RUNS:
+        tag = f"ad_{name}_{2 ** t}"
+        if want and tag not in want and name not in want:
+            continue
+        work = tempfile.mkdtemp(prefix="egtb_adgolden_", dir="/dev/shm")
+        try:
+            out = digest_run(name, t, work)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        with open(os.path.join(os.path.dirname(__file__), f"{tag}.json"), "w") as fh:
+            json.dump(out, fh, separators=(",", ":"))
+        print(tag, "layers", len(out["layers"]), "states", sum(l["n"] for l in out["layers"]), flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

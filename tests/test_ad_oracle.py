@@ -1,0 +1,165 @@
+"""The CPU oracle of the advanced (AD, large-tile masking) forward pass
+(oracle/ad_oracle.c) pinned before it checks the GPU:
+
+  * the masked row table against an independent merge written over tile
+    values (2^v arithmetic), on all 65536 rows;
+  * the new-masked-tile flag against its definition (some merge of the move
+    reached 64 or more) on random boards;
+  * the large-tile combination table against the properties every entry
+    must have, and entries worked out by hand;
+  * whole forward passes against the raw layer digests written by the
+    reference's own AD builder (tests/golden/ad_*.json): every layer of
+    442@64 and t@64 (two free 32768 tiles), the first layers of LL@64
+    (diagonal symmetry) and 444@64 (horizontal symmetry), including their
+    validation steps.
+"""
+import hashlib
+import importlib
+import os
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import load_ad_golden
+from oracle import ad_oracle
+
+patterns = importlib.import_module("2048endgametablebase_amd.patterns")
+
+
+def _tiles(row):
+    # cell 0 of the line is the high nibble; values as tiles (0xF = 32768)
+    return [0 if v == 0 else 1 << v for v in ((row >> 12) & 0xF, (row >> 8) & 0xF, (row >> 4) & 0xF, row & 0xF)]
+
+
+def _merge_values(tiles):
+    """Slide toward cell 0; equal tiles merge once; 32768 never merges; a sum
+    of 64 or more is stored as 32768 (masked)."""
+    line = [t for t in tiles if t]
+    out, i = [], 0
+    while i < len(line):
+        if i + 1 < len(line) and line[i] == line[i + 1] and line[i] != 32768:
+            s = 2 * line[i]
+            out.append(32768 if s >= 64 else s)
+            i += 2
+        else:
+            out.append(line[i])
+            i += 1
+    out += [0] * (4 - len(out))
+    return out
+
+
+def _row(tiles):
+    r = 0
+    for k, t in enumerate(tiles):
+        r |= (0 if t == 0 else t.bit_length() - 1) << (4 * (3 - k))
+    return r
+
+
+def test_masked_row_table_all_rows():
+    for row in range(65536):
+        assert ad_oracle.row_left(row) == _row(_merge_values(_tiles(row))), hex(row)
+
+
+def _rows(b):
+    return [(b >> (48 - 16 * r)) & 0xFFFF for r in range(4)]
+
+
+def _transpose(b):
+    cells = [[(b >> (4 * (15 - (4 * r + c)))) & 0xF for c in range(4)] for r in range(4)]
+    out = 0
+    for r in range(4):
+        for c in range(4):
+            out |= cells[c][r] << (4 * (15 - (4 * r + c)))
+    return out
+
+
+def _big_merge(rows_, reverse):
+    for row in rows_:
+        line = [t for t in _tiles(row) if t]
+        if reverse:
+            line = line[::-1]
+        i = 0
+        while i < len(line):
+            if i + 1 < len(line) and line[i] == line[i + 1] and line[i] != 32768:
+                if 2 * line[i] >= 64:
+                    return True
+                i += 2
+            else:
+                i += 1
+    return False
+
+
+def test_new_masked_tile_flag():
+    rng = np.random.default_rng(3)
+    for _ in range(3000):
+        cells = rng.choice([0, 0, 0, 1, 2, 3, 4, 5, 5, 6, 6, 7, 15], size=16)
+        b = 0
+        for k, v in enumerate(cells):
+            b |= int(v) << (4 * k)
+        moved, mnt = ad_oracle.move4(b)
+        want = [_big_merge(_rows(b), False), _big_merge(_rows(b), True),
+                _big_merge(_rows(_transpose(b)), False), _big_merge(_rows(_transpose(b)), True)]
+        assert mnt == want, hex(b)
+        # left equals the row-wise slide; up equals left of the transpose
+        left = 0
+        for r, row in enumerate(_rows(b)):
+            left |= ad_oracle.row_left(row) << (48 - 16 * r)
+        assert moved[0] == left
+        up = 0
+        for r, row in enumerate(_rows(_transpose(b))):
+            up |= ad_oracle.row_left(row) << (48 - 16 * r)
+        assert moved[2] == _transpose(up)
+
+
+def test_combination_table():
+    t = ad_oracle.comb_table()
+    n_set = 0
+    for idx in range(2560):
+        n = int(t[idx, 0])
+        a, b = idx & 0xFF, idx >> 8
+        if n == 0:
+            continue
+        n_set += 1
+        v = [int(x) for x in t[idx, 1:1 + n]]
+        assert n == b and a >= 1
+        assert v == sorted(v) and all(6 <= x <= 13 for x in v)
+        assert sum(1 << x for x in v) == a << 6
+        counts = sorted((v.count(x) for x in set(v)), reverse=True)
+        assert counts[0] == 1 or counts[:2] in ([2], [2, 1]) or (counts[0] == 3 and v.count(6) == 3), v
+    assert n_set > 500
+
+    def entry(total, count):
+        idx = (total >> 6) + (count << 8)
+        return [int(x) for x in t[idx, 1:1 + t[idx, 0]]]
+
+    assert entry(64, 1) == [6]
+    assert entry(64, 2) == []          # one 64 cannot be two tiles
+    assert entry(384, 3) == [6, 6, 8]  # 256 + 64 + 64, largest tile first
+    assert entry(1024, 3) == [8, 8, 9]
+    assert entry(448, 4) == [6, 6, 6, 8]
+    assert entry(192, 3) == [6, 6, 6]
+    assert entry(64 + 128 + 1024, 3) == [6, 7, 10]
+
+
+def _check_forward(pattern, target, n_layers=None):
+    g = load_ad_golden(f"{pattern}_{2 ** target}")
+    p = patterns.ad_params(pattern, target)
+    work = tempfile.mkdtemp(prefix="egtb_adorc_")
+    try:
+        prefix = os.path.join(work, "x_")
+        steps = p["steps"] if n_layers is None else n_layers + 1
+        ad_oracle.forward(p, prefix, steps=steps)
+        for layer in g["layers"][:steps - 1]:
+            raw = open(prefix + str(layer["i"]), "rb").read()
+            assert len(raw) // 8 == layer["n"], (pattern, target, layer["i"])
+            assert hashlib.sha256(raw).hexdigest() == layer["sha256"], (pattern, target, layer["i"])
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+@pytest.mark.parametrize("pattern,target,n_layers", [("442", 6, None), ("t", 6, None), ("LL", 6, 28),
+                                                     ("444", 6, 30)])
+def test_forward_matches_reference(pattern, target, n_layers):
+    _check_forward(pattern, target, n_layers)
