@@ -1,0 +1,119 @@
+"""The advanced (AD, large-tile masking) builder's forward pass on the GPU.
+
+``generate_ad(pattern, target, pathname)`` runs what the reference's
+``formation_core.run_pattern_build_ad`` does before its solver
+(native_core/src/BookGeneratorAD.cpp:1256-1396, called from
+run_pattern_build_ad_cpp :1421-1438): raw layer files
+``<pathname>0 .. <pathname><steps-2>`` of masked boards (tiles of 64 or more
+stored as 0xF), byte-equal to the reference's.  ``run_pattern_generate_ad``
+takes the reference binding's arguments (seeds, an AdvancedPatternSpec-like
+dict, steps, pathname).  The AD solver that turns these layers into books is
+not part of this engine.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import native, patterns
+from .config import SingletonConfig
+
+
+class AdResult:
+    """Layers of one advanced forward pass (held in HBM when kept)."""
+
+    def __init__(self, handle: int):
+        self._h = ctypes.c_void_p(handle)
+        L = native.lib()
+        steps = ctypes.c_int()
+        ms = ctypes.c_double()
+        cand = ctypes.c_uint64()
+        chunked = ctypes.c_uint64()
+        native.check(L.egtb_ad_result_info(self._h, ctypes.byref(steps), None, ctypes.byref(ms),
+                                           ctypes.byref(cand), ctypes.byref(chunked)))
+        self.steps = steps.value
+        sizes = np.zeros(max(self.steps - 1, 1), dtype=np.uint64)
+        native.check(L.egtb_ad_result_info(self._h, None, sizes.ctypes.data_as(native.u64p), None, None, None))
+        self.sizes = sizes
+        self.fwd_ms = ms.value
+        self.candidates = cand.value
+        self.chunked_steps = chunked.value
+
+    def layer(self, i: int) -> np.ndarray:
+        L = native.lib()
+        n = ctypes.c_size_t()
+        native.check(L.egtb_ad_result_layer(self._h, int(i), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.uint64)
+        native.check(L.egtb_ad_result_layer(self._h, int(i), out.ctypes.data_as(native.u64p), out.size,
+                                            ctypes.byref(n)))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            native.lib().egtb_ad_result_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_pattern_generate_ad(arr_init: Sequence[int], spec: Dict[str, object], steps: int,
+                            pathname: Optional[str] = None, *, keep_layers: bool = False, device: int = 0,
+                            radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0) -> AdResult:
+    """The forward pass of formation_core.run_pattern_build_ad
+    (bindings_formation.cpp:231-235) for ``spec`` with the fields of
+    AdvancedPatternSpec (FormationRuntime.h:88-96): pattern_masks,
+    symm_mode, num_free_32k, fixed_32k_shifts, small_tile_sum_limit and
+    target (log2)."""
+    seeds = np.ascontiguousarray(np.asarray(arr_init, dtype=np.uint64))
+    masks = np.ascontiguousarray(np.asarray(list(spec["pattern_masks"]) or [0], dtype=np.uint64))
+    fixed = np.ascontiguousarray(np.asarray(list(spec["fixed_32k_shifts"]) or [0], dtype=np.uint8))
+    pat = native.AdPattern()
+    pat.masks = masks.ctypes.data_as(native.u64p)
+    pat.n_masks = len(spec["pattern_masks"])
+    pat.symm_mode = int(spec["symm_mode"])
+    pat.n_free_32k = int(spec["num_free_32k"])
+    pat.fixed_32k_shifts = fixed.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    pat.n_fixed = len(spec["fixed_32k_shifts"])
+    pat.small_tile_sum_limit = int(spec.get("small_tile_sum_limit", 96))
+    pat.target = int(spec["target"])
+    opt = native.AdOptions()
+    opt.steps = int(steps)
+    opt.pathname = pathname.encode() if pathname is not None else None
+    opt.keep_layers = int(bool(keep_layers))
+    opt.device = int(device)
+    opt.radix_bits = int(radix_bits)
+    opt.sort_schedule = int(sort_schedule)
+    opt.chunk_mkeys = int(chunk_mkeys)
+    h = ctypes.c_void_p()
+    native.check(native.lib().egtb_ad_generate(seeds.ctypes.data_as(native.u64p), seeds.size, ctypes.byref(pat),
+                                               ctypes.byref(opt), ctypes.byref(h)))
+    return AdResult(h.value)
+
+
+def ad_spec(pattern: str, target: int) -> Dict[str, object]:
+    """The AdvancedPatternSpec engine_core/BookBuilder.py:90-108 builds,
+    with SmallTileSumLimit from the configuration."""
+    limit = int(SingletonConfig().config.get("SmallTileSumLimit", 96))
+    p = patterns.ad_params(pattern, target, limit)
+    return {"pattern_masks": p["masks"], "symm_mode": p["symm_mode"], "num_free_32k": p["n_free_32k"],
+            "fixed_32k_shifts": p["fixed_32k_shifts"], "small_tile_sum_limit": limit, "target": int(target),
+            "steps": p["steps"], "seeds": p["seeds"]}
+
+
+def generate_ad(pattern: str, target: int, pathname: Optional[str] = None, **kw) -> AdResult:
+    """Forward pass of the advanced build of ``pattern`` at 2**target
+    (raw layer files under ``pathname`` when given)."""
+    s = ad_spec(pattern, target)
+    return run_pattern_generate_ad(s["seeds"], s, s["steps"], pathname, **kw)

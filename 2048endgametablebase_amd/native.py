@@ -118,6 +118,32 @@ class LoadgenReport(ctypes.Structure):
                [(n, ctypes.c_uint64) for n in ("completed", "batches", "max_batch", "fallbacks")]
 
 
+class AdPattern(ctypes.Structure):
+    _fields_ = [
+        ("masks", u64p),
+        ("n_masks", ctypes.c_uint32),
+        ("symm_mode", ctypes.c_int32),
+        ("n_free_32k", ctypes.c_int32),
+        ("fixed_32k_shifts", ctypes.POINTER(ctypes.c_uint8)),
+        ("n_fixed", ctypes.c_uint32),
+        ("small_tile_sum_limit", ctypes.c_uint32),
+        ("target", ctypes.c_int32),
+    ]
+
+
+class AdOptions(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int32),
+        ("pathname", ctypes.c_char_p),
+        ("keep_layers", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("radix_bits", ctypes.c_int32),
+        ("sort_schedule", ctypes.c_int32),
+        ("chunk_mkeys", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "egtb_build": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(Pattern), ctypes.POINTER(Options)]),
@@ -136,6 +162,13 @@ SIGNATURES = {
     "egtb_result_memory": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p, ctypes.POINTER(ctypes.c_double), u64p, u64p]),
     "egtb_result_forward": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p]),
     "egtb_result_free": (None, [ctypes.c_void_p]),
+    "egtb_ad_generate": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(AdPattern), ctypes.POINTER(AdOptions),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "egtb_ad_result_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), u64p,
+                                           ctypes.POINTER(ctypes.c_double), u64p, u64p]),
+    "egtb_ad_result_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u64p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_size_t)]),
+    "egtb_ad_result_free": (None, [ctypes.c_void_p]),
     "egtb_batch_build": (ctypes.c_int, [ctypes.POINTER(BatchJob), ctypes.c_int, ctypes.POINTER(BatchOptions),
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "egtb_batch_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),

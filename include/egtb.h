@@ -143,6 +143,57 @@ EGTB_EXPORT int egtb_result_memory(const egtb_result *r, uint64_t *peak_device_b
 EGTB_EXPORT int egtb_result_forward(const egtb_result *r, uint64_t *hash_steps, uint64_t *set_regrows);
 EGTB_EXPORT void egtb_result_free(egtb_result *r);
 
+/* ---- advanced (AD, large-tile masking) forward pass ----------------------- */
+
+/* Pattern of an advanced build.  Replaces AdvancedPatternSpec
+ * (reference native_core/include/FormationRuntime.h:88-96) as
+ * engine_core/BookBuilder.py:90-108 fills it: the pattern's masks and
+ * canonical mode, the 32768 tiles of the first seed that no mask pins
+ * (n_free_32k), the nibble shifts of the pinned ones, SmallTileSumLimit and
+ * the target (log2). */
+typedef struct egtb_ad_pattern {
+    const uint64_t *masks;
+    uint32_t n_masks;                /* <= 16 */
+    int32_t symm_mode;               /* 0 identity, 1 full, 2 diagonal, 3 horizontal */
+    int32_t n_free_32k;
+    const uint8_t *fixed_32k_shifts; /* nibble shifts 0, 4, ..., 60 */
+    uint32_t n_fixed;                /* <= 16 */
+    uint32_t small_tile_sum_limit;   /* reference default 96 */
+    int32_t target;
+} egtb_ad_pattern;
+
+typedef struct egtb_ad_options {
+    int32_t steps;           /* layers of the build (steps-1 raw layers are made) */
+    const char *pathname;    /* NULL, or the prefix of raw layer files <pathname><i> */
+    int32_t keep_layers;     /* 1: keep every raw layer in HBM for egtb_ad_result_layer */
+    int32_t device;          /* HIP device ordinal */
+    int32_t radix_bits;      /* 0 = autotuned; else 4, 6, 8 or 11 */
+    int32_t sort_schedule;   /* 0 = autotuned, 1 = per-digit 3 kernels, 2 = single-pass look-back */
+    int32_t chunk_mkeys;     /* boards per parent range in Mi keys; 0 = from free HBM */
+    int32_t reserved[3];
+} egtb_ad_options;
+
+typedef struct egtb_ad_result egtb_ad_result;
+
+/* The advanced builder's forward pass on the GPU: raw layers 0 .. steps-2
+ * of masked boards (layer 0 = the masked seeds as given, layer i sorted and
+ * unique), validated every 32 steps, written as the reference writes them
+ * when pathname is set.  Replaces generate_process_ad
+ * (reference native_core/src/BookGeneratorAD.cpp:1256-1396, called from
+ * run_pattern_build_ad_cpp :1421-1438, bound as
+ * formation_core.run_pattern_build_ad, bindings_formation.cpp:231-235).  A fresh
+ * build: existing raw files are overwritten, not resumed.  out may be NULL. */
+EGTB_EXPORT int egtb_ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern *pattern,
+                                 const egtb_ad_options *options, egtb_ad_result **out);
+/* steps, the steps-1 layer sizes (sizes may be NULL), wall time, boards
+ * emitted before dedup, and the steps that ran over parent ranges. */
+EGTB_EXPORT int egtb_ad_result_info(const egtb_ad_result *r, int *steps, uint64_t *sizes, double *fwd_ms,
+                                    uint64_t *candidates, uint64_t *chunked_steps);
+/* Copy raw layer `layer` to host (needs options.keep_layers; keys may be
+ * NULL to query n). */
+EGTB_EXPORT int egtb_ad_result_layer(const egtb_ad_result *r, int layer, uint64_t *keys, size_t cap, size_t *n);
+EGTB_EXPORT void egtb_ad_result_free(egtb_ad_result *r);
+
 /* ---- batched builds: many (pattern, target) jobs through one pipeline ------ */
 
 /* One job of a batched build: the arguments of egtb_build for one table.

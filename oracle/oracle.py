@@ -20,7 +20,7 @@ f32p = ctypes.POINTER(ctypes.c_float)
 
 
 def build() -> str:
-    subprocess.run(["make", "-s", "-C", _HERE, "libegtb_oracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", _HERE, "libegtb_oracle.so", "libegtb_ad_oracle.so"], check=True)
     return _LIB_PATH
 
 

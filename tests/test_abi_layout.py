@@ -17,6 +17,8 @@ STRUCTS = {
     "Options": "egtb_options",
     "BatchJob": "egtb_batch_job",
     "BatchOptions": "egtb_batch_options",
+    "AdPattern": "egtb_ad_pattern",
+    "AdOptions": "egtb_ad_options",
     "ServerOptions": "egtb_server_options",
     "LoadgenOptions": "egtb_loadgen_options",
     "LoadgenReport": "egtb_loadgen_report",

@@ -106,3 +106,22 @@ def test_every_sort_state_error_flag_is_read():
     assert len(states) >= 2, states
     for name in states:
         assert f"{name}.read_error(" in body, name
+
+
+def test_advanced_forward_arguments_fail_before_device_use(pkg):
+    """The advanced forward pass checks its pattern and options on the host."""
+    import importlib
+
+    adv = importlib.import_module("2048endgametablebase_amd.advanced")
+    s = adv.ad_spec("442", 7)
+    for bad, msg in ((dict(symm_mode=5), "symm_mode"), (dict(target=0), "target"),
+                     (dict(num_free_32k=-1), "n_free_32k"), (dict(fixed_32k_shifts=[3]), "nibble shifts"),
+                     (dict(pattern_masks=list(range(1, 18))), "16 masks")):
+        with pytest.raises(pkg.native.EgtbError, match=msg):
+            adv.run_pattern_generate_ad(s["seeds"], {**s, **bad}, s["steps"])
+    with pytest.raises(pkg.native.EgtbError, match="steps"):
+        adv.run_pattern_generate_ad(s["seeds"], s, 1)
+    with pytest.raises(pkg.native.EgtbError, match="radix_bits"):
+        adv.run_pattern_generate_ad(s["seeds"], s, s["steps"], radix_bits=3)
+    p = pkg.patterns.ad_params("t", 7)
+    assert p["n_free_32k"] == 2 and p["fixed_32k_shifts"] == [0, 4, 16, 20]
