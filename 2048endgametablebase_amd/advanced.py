@@ -117,3 +117,31 @@ def generate_ad(pattern: str, target: int, pathname: Optional[str] = None, **kw)
     (raw layer files under ``pathname`` when given)."""
     s = ad_spec(pattern, target)
     return run_pattern_generate_ad(s["seeds"], s, s["steps"], pathname, **kw)
+
+
+def _main(argv=None) -> None:
+    """``python -m 2048endgametablebase_amd.advanced PATTERN LOG2_TARGET [--reps N]``:
+    time the forward pass (layers in HBM, no files) and print one JSON line."""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("pattern")
+    ap.add_argument("target", type=int)
+    ap.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args(argv)
+    times = []
+    for _ in range(a.reps):
+        t0 = time.perf_counter()
+        r = generate_ad(a.pattern, a.target)
+        times.append(time.perf_counter() - t0)
+        n, c = int(r.sizes.sum()), int(r.candidates)
+        r.close()
+    print(json.dumps({"workload": f"{a.pattern}@{2 ** a.target} advanced forward pass", "states": n,
+                      "boards_before_dedup": c, "ms": [round(t * 1000.0, 2) for t in times],
+                      "states_per_s": round(n / min(times), 1)}))
+
+
+if __name__ == "__main__":
+    _main()

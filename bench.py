@@ -133,6 +133,49 @@ def cpu_baseline(pattern, target, budget_threads, n_states):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def cpu_baseline_advanced(pattern, target, threads):
+    """The reference's advanced builder (compiled from its sources into
+    oracle/_ref) on the same forward pass as the GPU leg, stopped after its
+    raw layers are written to tmpfs; None when that library is absent."""
+    pkg = importlib.import_module("2048endgametablebase_amd")
+    from oracle.ref import ref
+
+    if not ref.available():
+        return None
+    p = pkg.patterns.ad_params(pattern, target)
+    work = tempfile.mkdtemp(prefix="egtb_cpu_ad_", dir=TMPFS)
+    try:
+        prefix = os.path.join(work, f"{pattern}_{2**target}_")
+        t0 = time.perf_counter()
+        ref.run_build_ad(p["seeds"], p["masks"], p["symm_mode"], p["n_free_32k"], p["fixed_32k_shifts"], target,
+                         p["steps"], p["docheck_step"], prefix, is_free=p["is_free"],
+                         small_tile_sum_limit=p["small_tile_sum_limit"], threads=threads)
+        dt = time.perf_counter() - t0
+        n = sum(os.path.getsize(prefix + str(i)) // 8 for i in range(p["steps"] - 1))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return dict(value=round(n / dt, 1), unit="states/s", cores=threads, kind="reference",
+                sample=f"{pattern}@{2**target} advanced forward pass by the reference's run_pattern_build_ad_cpp "
+                       f"(g++ -Ofast, {threads} OpenMP threads, raw layer files on tmpfs): {n} states in {dt:.1f} s")
+
+
+def advanced_leg(pkg, pattern, target, reps=3):
+    """Best wall time of the GPU advanced (masked) forward pass, raw layers
+    kept only while needed, no files."""
+    adv = importlib.import_module("2048endgametablebase_amd.advanced")
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = adv.generate_ad(pattern, target)
+        dt = time.perf_counter() - t0
+        n, c, layers = int(r.sizes.sum()), int(r.candidates), int(r.sizes.size)
+        r.close()
+        best = dt if best is None else min(best, dt)
+    return {"workload": f"{pattern}@{2**target} advanced (masked) forward pass, layers in HBM", "layers": layers,
+            "states": n, "boards_before_dedup": c, "ms": round(best * 1000.0, 2),
+            "states_per_s": round(n / best, 1)}
+
+
 def serving_sweep(pkg, pattern, target, seconds):
     """Continuous-batching server on the device-resident table: Poisson
     arrivals at rates doubling from 10k QPS, 80% on-table / 20% fallback
@@ -368,7 +411,9 @@ def main():
                  "fp64": fp64_max_error(pkg, args.pattern, args.target),
                  "C0_L3_256": build_leg(pkg, "L3", 8, reps=3),
                  "C4_444_2048_in_hbm": build_leg(pkg, "444", 11),
-                 "C4_444_2048_host_tables": build_leg(pkg, "444", 11, host_tables=True)}
+                 "C4_444_2048_host_tables": build_leg(pkg, "444", 11, host_tables=True),
+                 "advanced_442_256": advanced_leg(pkg, "442", 8),
+                 "advanced_LL_64": advanced_leg(pkg, "LL", 6)}
 
     cpu = None
     hostinfo = host_cpu()
@@ -385,6 +430,16 @@ def main():
                            f"take {cpu['reference_thread_rule']} on this {hostinfo['nproc']}-thread host")
         except Exception as exc:  # reported, never fatal for the GPU number
             cpu = {"error": str(exc)}
+        if extra is not None:
+            try:
+                ad = cpu_baseline_advanced("442", 8, max(4, min(32, share)))
+                if ad is not None:
+                    gpu = extra["advanced_442_256"]["states_per_s"]
+                    ad["gpu_states_per_s"] = gpu
+                    ad["gpu_over_cpu"] = round(gpu / ad["value"], 1)
+                    cpu["advanced_forward"] = ad
+            except Exception as exc:
+                cpu["advanced_forward"] = {"error": str(exc)}
 
     line = {
         "metric": METRIC,
