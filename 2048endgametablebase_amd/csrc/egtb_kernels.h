@@ -1,6 +1,9 @@
 // Host-visible declarations of the kernel launch wrappers.
 #pragma once
 
+#include <utility>
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -270,6 +273,14 @@ struct LookbackWork {
 
 // keys per radix tile for a sort of n keys with the given schedule (1 or 2)
 uint32_t radix_tile(int bits, uint64_t n, int schedule);
+
+// Tile tunings installed by the autotuner (egtb_set_tile_tuning): per kind,
+// ascending rows of (max_n, value); tuned_tile gives the value of the first
+// row with n <= max_n (the last row also covers larger n), 0 when the kind
+// has no rows (the built-in rule applies).
+enum TileKind { kTileRadixItems = 0, kTileLookupThreads = 1, kTileKinds = 2 };
+void set_tile_tuning(int kind, std::vector<std::pair<uint64_t, int>> rows);
+int tuned_tile(int kind, uint64_t n);
 // Classic schedule: per digit a tile-histogram kernel, a per-digit scan of
 // the [digit][tile] counts and a scatter kernel.  counts holds
 // tiles << bits words, offsets tiles << bits words plus 2^bits digit totals

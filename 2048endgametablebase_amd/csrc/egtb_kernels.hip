@@ -13,6 +13,8 @@
 //
 // Launch wrappers at the bottom are the only entry points used by the host.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -1657,7 +1659,25 @@ void launch_scan_u32(hipStream_t s, const uint32_t *in, uint64_t n, uint32_t *ou
 // 0.189, 4 Mi 0.367 / 0.325 / 0.323, 16 Mi 1.086 / 0.92 / 0.84.
 static int classic_items(int bits, uint64_t n) {
     if (bits >= 11) return 16;
+    const int t = tuned_tile(kTileRadixItems, n);
+    if (t == 4 || t == 8 || t == 16) return t;
     return n <= (1u << 19) ? 4 : n <= (1u << 22) ? 8 : 16;
+}
+
+static std::mutex g_tile_mu;
+static std::vector<std::pair<uint64_t, int>> g_tile_rows[kTileKinds];
+
+void set_tile_tuning(int kind, std::vector<std::pair<uint64_t, int>> rows) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile_rows[kind] = std::move(rows);
+}
+
+int tuned_tile(int kind, uint64_t n) {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    const auto &rows = g_tile_rows[kind];
+    for (const auto &r : rows)
+        if (n <= r.first) return r.second;
+    return rows.empty() ? 0 : rows.back().second;
 }
 
 uint32_t radix_tile(int bits, uint64_t n, int schedule) {

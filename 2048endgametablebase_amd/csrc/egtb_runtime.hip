@@ -3061,6 +3061,32 @@ int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32
     return EGTB_OK;
 }
 
+int egtb_set_tile_tuning(int kind, const uint64_t *max_n, const int32_t *value, int rows) {
+    if (kind < 0 || kind >= kTileKinds || rows < 0 || (rows > 0 && (!max_n || !value))) {
+        g_last_error = "invalid tile tuning table";
+        return EGTB_ERR_ARG;
+    }
+    std::vector<std::pair<uint64_t, int>> t;
+    for (int i = 0; i < rows; ++i) {
+        if (i && max_n[i] <= max_n[i - 1]) {
+            g_last_error = "tuning rows must have ascending max_n";
+            return EGTB_ERR_ARG;
+        }
+        const int v = value[i];
+        const bool ok = kind == kTileRadixItems ? (v == 0 || v == 4 || v == 8 || v == 16)
+                                                : (v == 0 || (v >= 64 && v <= 1024 && v % 64 == 0));
+        if (!ok) {
+            g_last_error = kind == kTileRadixItems ? "radix keys per thread must be 4, 8 or 16 (0 = rule)"
+                                                   : "lookup threads per block must be a multiple of 64 in "
+                                                     "64..1024 (0 = rule)";
+            return EGTB_ERR_ARG;
+        }
+        t.emplace_back(max_n[i], v);
+    }
+    set_tile_tuning(kind, std::move(t));
+    return EGTB_OK;
+}
+
 int egtb_set_key_set_sizing(double slots_per_key, uint64_t max_slots) {
     if (!(slots_per_key >= 0.0 && slots_per_key <= 64.0)) {
         g_last_error = "slots per key must lie in (0, 64], or 0 for the default";

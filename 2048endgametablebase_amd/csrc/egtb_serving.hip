@@ -176,9 +176,11 @@ static void launch_lookup_t(hipStream_t s, const TableSet &ts, const DeviceCtx &
                             uint64_t n, OutT *out4, int32_t *status, int32_t *op_out) {
     // the 128 KiB LDS row table allows one block per CU: about n / CUs
     // threads each (whole wavefronts, up to 1024), so a small batch still
-    // spreads over every CU
+    // spreads over every CU, unless the autotuner installed block sizes
     const uint64_t per = (n + C.cus - 1) / (uint64_t)C.cus;
-    const uint32_t threads = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(64, (per + 63) / 64 * 64));
+    const int tuned = tuned_tile(kTileLookupThreads, n);
+    const uint32_t threads = tuned ? (uint32_t)tuned
+                                   : (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(64, (per + 63) / 64 * 64));
     uint64_t want = (n + threads - 1) / threads;
     const uint64_t cap = (uint64_t)C.cus;
     const uint32_t grid = (uint32_t)(want < cap ? want : cap);

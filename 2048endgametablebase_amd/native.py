@@ -217,6 +217,7 @@ SIGNATURES = {
     "sort_uint64": (None, [u64p, ctypes.c_size_t, ctypes.c_bool]),
     "unique_sorted_u64_inplace": (ctypes.c_size_t, [u64p, ctypes.c_size_t]),
     "egtb_set_key_set_sizing": (ctypes.c_int, [ctypes.c_double, ctypes.c_uint64]),
+    "egtb_set_tile_tuning": (ctypes.c_int, [ctypes.c_int, u64p, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]),
     "egtb_set_sort_tuning": (ctypes.c_int, [u64p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                             ctypes.c_int]),
     "egtb_bench_sort": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -267,10 +268,28 @@ def _install_cached_tuning(L) -> None:
 
     try:
         with open(TUNING_FILE) as fh:
-            rows = json.load(fh)["rows"]
+            data = json.load(fh)
+        rows = data["rows"]
     except (OSError, ValueError, KeyError):
         return
     install_sort_tuning(rows, L)
+    for kind, name in enumerate(TILE_KINDS):
+        install_tile_tuning(kind, data.get("tiles", {}).get(name, []), L)
+
+
+# kinds of egtb_set_tile_tuning
+TILE_KINDS = ("radix_items", "lookup_threads")
+
+
+def install_tile_tuning(kind: int, rows, L=None) -> None:
+    """rows: [{"max_n": int, "value": int}, ...] ascending (value 0 = rule)."""
+    L = L or lib()
+    n = len(rows)
+    max_n = (ctypes.c_uint64 * max(n, 1))(*[int(r["max_n"]) for r in rows])
+    val = (ctypes.c_int32 * max(n, 1))(*[int(r["value"]) for r in rows])
+    rc = L.egtb_set_tile_tuning(int(kind), max_n, val, n)
+    if rc != 0:
+        raise EgtbError(rc, last_error())
 
 
 def install_sort_tuning(rows, L=None) -> None:

@@ -434,6 +434,14 @@ EGTB_EXPORT int egtb_sort_unique_u64(const uint64_t *keys, size_t n, int schedul
 EGTB_EXPORT int egtb_set_sort_tuning(const uint64_t *max_n, const int32_t *bits, const int32_t *schedule,
                                      int rows);
 
+/* Install a tile tuning table: kind 0 = keys per thread of the per-digit
+ * radix schedule's tiles (4, 8 or 16), kind 1 = threads per block of the
+ * lookup kernel (64..1024, whole wavefronts).  A launch over n items uses
+ * the first row with n <= max_n[i] (rows ascending; the last row also covers
+ * larger n); a value of 0, or rows = 0, keeps the built-in rule.  No
+ * counterpart in the reference; written by the package's autotuner. */
+EGTB_EXPORT int egtb_set_tile_tuning(int kind, const uint64_t *max_n, const int32_t *value, int rows);
+
 /* Sizing of the forward pass's key sets.  slots_per_key: slots per
  * predicted key (default 2: a set is at most about half full when the
  * growth estimate holds; valid 0 < v <= 64).  max_slots: the largest set

@@ -116,6 +116,27 @@ def test_sort_tunings_agree(pkg, bits, schedule):
         res.close()
 
 
+@pytest.mark.parametrize("items", [4, 8, 16])
+def test_radix_tile_sizes_agree(pkg, items):
+    """The per-digit schedule's tile size (keys per thread) is an autotuner
+    knob: every size gives the reference's table."""
+    p = pkg.patterns.build_params("442", 7)
+    g = load_golden("442_128")
+    pkg.native.install_tile_tuning(0, [{"max_n": (1 << 64) - 1, "value": items}])
+    try:
+        res = pkg.run_pattern_build(p["seeds"], p["masks"], p["shifts"], p["symm_mode"], 7, p["steps"],
+                                    p["docheck_step"], "", write_files=False, keep=True, radix_bits=8,
+                                    sort_schedule=1, forward_hash_steps=-1)
+        try:
+            for layer in g["layers"]:
+                keys, rates = res.book(layer["i"])
+                assert book_sha(keys, rates) == layer["sha256"], f"layer {layer['i']}, {items} keys per thread"
+        finally:
+            res.close()
+    finally:
+        pkg.native._install_cached_tuning(pkg.native.lib())
+
+
 def test_start_build_writes_reference_layout(pkg, tmp_path):
     """start_build writes config.txt, stats.txt and every <i>.book with the
     reference's bytes (442 at 128, float32)."""

@@ -129,6 +129,30 @@ def test_fallback_and_table_sources_agree(pkg, built):
     res.close()
 
 
+@pytest.mark.parametrize("threads", [64, 256, 1024])
+def test_lookup_block_sizes_agree(pkg, built, threads):
+    """The lookup's threads per block are an autotuner knob: every size gives
+    the same answers (on-table rows, off-table player scores, statuses)."""
+    res = pkg.build_in_memory(PATTERN, TARGET)
+    t = pkg.Table.from_result(res, PATTERN, TARGET)
+    try:
+        rng = np.random.default_rng(11)
+        keys, _ = res.book(30)
+        boards = np.concatenate([keys[rng.integers(0, keys.size, 3000)] | np.uint64(1 << 60),
+                                 rng.integers(0, 2**63, size=1000, dtype=np.uint64) & np.uint64(0x7777777777777777)])
+        a, sa, oa = t.lookup_f64(boards)
+        pkg.native.install_tile_tuning(1, [{"max_n": (1 << 64) - 1, "value": threads}])
+        try:
+            b, sb, ob = t.lookup_f64(boards)
+        finally:
+            pkg.native._install_cached_tuning(pkg.native.lib())
+        assert np.array_equal(sa, sb) and np.array_equal(oa, ob)
+        assert np.array_equal(np.nan_to_num(a, nan=-1), np.nan_to_num(b, nan=-1))
+    finally:
+        t.close()
+        res.close()
+
+
 def test_prefer_max_matches_reference_reader(pkg, built):
     """prefer_max (the reference reader's rule for 4442f/4442ff/4tiler): all
     ops are tried and the one with the largest rounded best rate answers."""

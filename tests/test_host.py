@@ -125,3 +125,19 @@ def test_advanced_forward_arguments_fail_before_device_use(pkg):
         adv.run_pattern_generate_ad(s["seeds"], s, s["steps"], radix_bits=3)
     p = pkg.patterns.ad_params("t", 7)
     assert p["n_free_32k"] == 2 and p["fixed_32k_shifts"] == [0, 4, 16, 20]
+
+
+def test_tile_tuning_checks_arguments(pkg):
+    """Tile tunings are validated on the host (no GPU touched)."""
+    L = pkg.native.lib()
+    u64 = ctypes.c_uint64 * 2
+    i32 = ctypes.c_int32 * 2
+    assert L.egtb_set_tile_tuning(2, u64(1, 2), i32(4, 8), 2) != 0
+    assert L.egtb_set_tile_tuning(0, u64(1, 2), i32(4, 5), 2) != 0
+    assert "keys per thread" in pkg.native.last_error()
+    assert L.egtb_set_tile_tuning(1, u64(1, 2), i32(64, 100), 2) != 0
+    assert "multiple of 64" in pkg.native.last_error()
+    assert L.egtb_set_tile_tuning(0, u64(2, 1), i32(4, 8), 2) != 0
+    assert "ascending" in pkg.native.last_error()
+    assert L.egtb_set_tile_tuning(0, u64(1 << 20, 1 << 30), i32(4, 16), 2) == 0
+    pkg.native._install_cached_tuning(L)  # back to the committed tables
