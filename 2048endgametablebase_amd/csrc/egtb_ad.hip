@@ -471,19 +471,29 @@ struct AdRun {
         HIPCHK(hipGetLastError());
     }
 
-    // sorted unique keys of keys[0..n) (clobbered, with tmp) into a new buffer
-    DBuf dedup(uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all, uint64_t &m) {
+    // sorted unique keys of keys[0..n) (clobbered, with tmp) into a new
+    // buffer; the count lands in device slot `slot` (read with counts2)
+    DBuf dedup_async(uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all, int slot) {
         DBuf out;
         out.alloc(n * 8 + 8, s);
-        if (n == 0) {
-            m = 0;
-            return out;
-        }
         cnt_dev.ensure(64, s);
-        dedup_into(s, keys, tmp, n, or_all, and_all, (uint64_t *)out.p, (uint64_t *)cnt_dev.p, o.radix_bits, w,
-                   o.sort_schedule);
+        uint64_t *cnt = (uint64_t *)cnt_dev.p + slot;
+        if (n == 0) HIPCHK(hipMemsetAsync(cnt, 0, 8, s));
+        else dedup_into(s, keys, tmp, n, or_all, and_all, (uint64_t *)out.p, cnt, o.radix_bits, w, o.sort_schedule);
+        return out;
+    }
+
+    void counts2(uint64_t &a, uint64_t &b) {
+        uint64_t v[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(v, cnt_dev.p, 16, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        a = v[0];
+        b = v[1];
+    }
+
+    DBuf dedup(uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all, uint64_t &m) {
+        DBuf out = dedup_async(keys, tmp, n, or_all, and_all, 0);
         m = read_u64(cnt_dev.p);
-        if (w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back sort stalled");
         return out;
     }
 
@@ -652,10 +662,11 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
             }
             uint64_t m2 = 0, m4 = 0;
             const uint64_t n2 = whole ? t2_whole : t2;
-            runs2.push_back(run.dedup((uint64_t *)c2buf.p, (uint64_t *)tmp.p, n2, whole ? or2 : (a[0]),
-                                      whole ? and2 : (a[1]), m2));
+            runs2.push_back(run.dedup_async((uint64_t *)c2buf.p, (uint64_t *)tmp.p, n2, whole ? or2 : (a[0]),
+                                            whole ? and2 : (a[1]), 0));
+            runs4.push_back(run.dedup_async((uint64_t *)c4buf.p, (uint64_t *)tmp.p, t4, a[2], a[3], 1));
+            run.counts2(m2, m4);
             runs2_n.push_back(m2);
-            runs4.push_back(run.dedup((uint64_t *)c4buf.p, (uint64_t *)tmp.p, t4, a[2], a[3], m4));
             runs4_n.push_back(m4);
         }
         c2buf.release();
@@ -719,6 +730,8 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
         d1_and = and4;
         progress_set((uint32_t)i, (uint32_t)(o.steps - 2));
     }
+    // the look-back kernels' stall flag is sticky: one read covers every sort
+    if (run.w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back sort stalled");
     HIPCHK(hipStreamSynchronize(s));
     res.fwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

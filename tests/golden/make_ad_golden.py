@@ -37,14 +37,15 @@ from oracle.ref import ref  # noqa: E402
 
 patterns = importlib.import_module("2048endgametablebase_amd.patterns")
 
-# (pattern, log2 target): identity (442, L3, t with free 32768 tiles),
-# diagonal (LL) and horizontal (444) canonical forms
+# (pattern, log2 target[, SmallTileSumLimit]): identity (442, L3, t with free
+# 32768 tiles), diagonal (LL) and horizontal (444) canonical forms; two runs
+# with other small-tile limits (other thresholds and validation steps)
 RUNS = [("442", 6), ("L3", 6), ("t", 6), ("442", 7), ("L3", 7), ("t", 7), ("LL", 6), ("444", 6),
-        ("442", 8), ("444", 5)]
+        ("442", 8), ("444", 5), ("442", 7, 64), ("t", 7, 128)]
 
 
-def digest_run(name, t, workdir):
-    p = patterns.ad_params(name, t)
+def digest_run(name, t, workdir, limit=96):
+    p = patterns.ad_params(name, t, limit)
     prefix = os.path.join(workdir, f"{name}_{2 ** t}_")
     ref.run_build_ad(p["seeds"], p["masks"], p["symm_mode"], p["n_free_32k"], p["fixed_32k_shifts"], t,
                      p["steps"], p["docheck_step"], prefix, is_free=p["is_free"],
@@ -69,13 +70,15 @@ def main(argv):
     if not ref.available() and not ref.build():
         raise SystemExit("reference sources not present; cannot make golden vectors")
     want = set(argv)
-    for name, t in RUNS:
-        tag = f"ad_{name}_{2 ** t}"
+    for run in RUNS:
+        name, t = run[0], run[1]
+        limit = run[2] if len(run) > 2 else 96
+        tag = f"ad_{name}_{2 ** t}" + ("" if limit == 96 else f"_lim{limit}")
         if want and tag not in want and name not in want:
             continue
         work = tempfile.mkdtemp(prefix="egtb_adgolden_", dir="/dev/shm")
         try:
-            out = digest_run(name, t, work)
+            out = digest_run(name, t, work, limit)
         finally:
             shutil.rmtree(work, ignore_errors=True)
         with open(os.path.join(os.path.dirname(__file__), f"{tag}.json"), "w") as fh:

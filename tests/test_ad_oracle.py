@@ -11,7 +11,7 @@
     reference's own AD builder (tests/golden/ad_*.json): every layer of
     442@64 and t@64 (two free 32768 tiles), the first layers of LL@64
     (diagonal symmetry) and 444@64 (horizontal symmetry), including their
-    validation steps.
+    validation steps, and 442@128 with a small-tile limit of 64.
 """
 import hashlib
 import importlib
@@ -143,9 +143,9 @@ def test_combination_table():
     assert entry(64 + 128 + 1024, 3) == [6, 7, 10]
 
 
-def _check_forward(pattern, target, n_layers=None):
-    g = load_ad_golden(f"{pattern}_{2 ** target}")
-    p = patterns.ad_params(pattern, target)
+def _check_forward(pattern, target, n_layers=None, limit=96):
+    g = load_ad_golden(f"{pattern}_{2 ** target}" + ("" if limit == 96 else f"_lim{limit}"))
+    p = patterns.ad_params(pattern, target, limit)
     work = tempfile.mkdtemp(prefix="egtb_adorc_")
     try:
         prefix = os.path.join(work, "x_")
@@ -159,7 +159,8 @@ def _check_forward(pattern, target, n_layers=None):
         shutil.rmtree(work, ignore_errors=True)
 
 
-@pytest.mark.parametrize("pattern,target,n_layers", [("442", 6, None), ("t", 6, None), ("LL", 6, 28),
-                                                     ("444", 6, 30)])
-def test_forward_matches_reference(pattern, target, n_layers):
-    _check_forward(pattern, target, n_layers)
+@pytest.mark.parametrize("pattern,target,n_layers,limit", [("442", 6, None, 96), ("t", 6, None, 96),
+                                                           ("LL", 6, 28, 96), ("444", 6, 30, 96),
+                                                           ("442", 7, 50, 64)])
+def test_forward_matches_reference(pattern, target, n_layers, limit):
+    _check_forward(pattern, target, n_layers, limit)

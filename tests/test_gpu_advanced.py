@@ -1,5 +1,6 @@
 """The advanced (AD, large-tile masking) forward pass on the GPU against
-the reference's own AD builder: every raw layer of ten builds
+the reference's own AD builder: every raw layer of twelve builds (two of
+them with other SmallTileSumLimit values)
 (tests/golden/ad_*.json, written by tests/golden/make_ad_golden.py from the
 reference compiled in oracle/_ref) must have the reference's size and
 SHA-256.  Also: the raw files written under a path, steps split into parent
@@ -34,7 +35,9 @@ def _check_layers(res, g, tag):
 def test_layers_match_reference(pkg, tag):
     g = load_ad_golden(tag)
     print(f"case advanced forward {tag}: {sum(l['n'] for l in g['layers'])} states", flush=True)
-    with _adv().generate_ad(g["pattern"], g["target"], keep_layers=True) as res:
+    adv = _adv()
+    spec = dict(adv.ad_spec(g["pattern"], g["target"]), small_tile_sum_limit=g["small_tile_sum_limit"])
+    with adv.run_pattern_generate_ad(spec["seeds"], spec, spec["steps"], keep_layers=True) as res:
         _check_layers(res, g, tag)
         print(f"{tag}: {res.fwd_ms:.1f} ms, {res.candidates} boards before dedup")
 
