@@ -5,7 +5,8 @@
 (native_core/src/BookGeneratorAD.cpp:1256-1396, called from
 run_pattern_build_ad_cpp :1421-1438): raw layer files
 ``<pathname>0 .. <pathname><steps-2>`` of masked boards (tiles of 64 or more
-stored as 0xF), byte-equal to the reference's.  ``run_pattern_generate_ad``
+stored as 0xF), byte-equal to the reference's; with a pathname it resumes
+from the raw files already there, as the reference's handle_restart_ad does.  ``run_pattern_generate_ad``
 takes the reference binding's arguments (seeds, an AdvancedPatternSpec-like
 dict, steps, pathname).  The AD solver that turns these layers into books is
 not part of this engine.
@@ -31,15 +32,18 @@ class AdResult:
         ms = ctypes.c_double()
         cand = ctypes.c_uint64()
         chunked = ctypes.c_uint64()
+        skipped = ctypes.c_uint64()
         native.check(L.egtb_ad_result_info(self._h, ctypes.byref(steps), None, ctypes.byref(ms),
-                                           ctypes.byref(cand), ctypes.byref(chunked)))
+                                           ctypes.byref(cand), ctypes.byref(chunked), ctypes.byref(skipped)))
         self.steps = steps.value
         sizes = np.zeros(max(self.steps - 1, 1), dtype=np.uint64)
-        native.check(L.egtb_ad_result_info(self._h, None, sizes.ctypes.data_as(native.u64p), None, None, None))
+        native.check(L.egtb_ad_result_info(self._h, None, sizes.ctypes.data_as(native.u64p), None, None, None,
+                                           None))
         self.sizes = sizes
         self.fwd_ms = ms.value
         self.candidates = cand.value
         self.chunked_steps = chunked.value
+        self.skipped_steps = skipped.value
 
     def layer(self, i: int) -> np.ndarray:
         L = native.lib()

@@ -32,6 +32,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <sys/stat.h>
 #include <string>
 #include <vector>
 
@@ -422,6 +423,7 @@ struct AdResult {
     double fwd_ms = 0;
     uint64_t candidates = 0;     // boards emitted over every step
     uint64_t chunked_steps = 0, chunks = 0;
+    uint64_t skipped_steps = 0;  // steps found on disk (resume)
 };
 
 namespace {
@@ -527,6 +529,35 @@ struct AdRun {
     }
 };
 
+bool file_exists(const std::string &p) {
+    struct stat st;
+    return ::stat(p.c_str(), &st) == 0;
+}
+
+uint64_t file_size(const std::string &p) {
+    struct stat st;
+    return ::stat(p.c_str(), &st) == 0 ? (uint64_t)st.st_size : 0;
+}
+
+// read_binary_vector_direct (FileIOUtils.h:190-215): a missing file reads as
+// empty, a size that is not whole boards is an error
+std::vector<uint64_t> read_raw(const std::string &p) {
+    FILE *f = std::fopen(p.c_str(), "rb");
+    if (!f) return {};
+    std::fseek(f, 0, SEEK_END);
+    const long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (sz < 0 || sz % 8) {
+        std::fclose(f);
+        throw Error(EGTB_ERR_IO, "misaligned binary file size: " + p);
+    }
+    std::vector<uint64_t> v((size_t)sz / 8);
+    const size_t got = v.empty() ? 0 : std::fread(v.data(), 8, v.size(), f);
+    std::fclose(f);
+    if (got != v.size()) throw Error(EGTB_ERR_IO, "short read of " + p);
+    return v;
+}
+
 uint32_t seed_sum(uint64_t b) {
     uint32_t t = 0;
     for (int k = 0; k < 16; ++k) {
@@ -590,21 +621,33 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
 
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t ini = n_seeds ? seed_sum(seeds[0]) : 0u;
-    // layer 0: the masked seeds as given (handle_restart_ad, :409-411)
-    std::vector<uint64_t> h0(n_seeds);
-    for (size_t i = 0; i < n_seeds; ++i) h0[i] = mask_board(seeds[i]);
     DBuf d0, d1;
-    uint64_t n0 = n_seeds, n1 = 0;
+    uint64_t n0 = 0, n1 = 0;
     uint64_t d1_or = 0, d1_and = ~0ull;
-    d0.alloc(n0 * 8 + 8, s);
-    if (n0) HIPCHK(hipMemcpyAsync(d0.p, h0.data(), n0 * 8, hipMemcpyHostToDevice, s));
-    res.n[0] = n0;
-    if (files) run.write_raw(prefix + "0", d0, n0);
-    if (o.keep_layers) {
-        res.layers[0].alloc(n0 * 8 + 8, s);
-        if (n0) HIPCHK(hipMemcpyAsync(res.layers[0].p, d0.p, n0 * 8, hipMemcpyDeviceToDevice, s));
-    }
-
+    // keep a copy of layer i (device keys[0..n)) in the result
+    auto keep = [&](int i, const DBuf &keys, uint64_t n) {
+        if (!o.keep_layers) return;
+        res.layers[i].alloc(n * 8 + 8, s);
+        if (n) HIPCHK(hipMemcpyAsync(res.layers[i].p, keys.p, n * 8, hipMemcpyDeviceToDevice, s));
+    };
+    // a raw layer file into a new device buffer
+    auto load = [&](const std::string &path, DBuf &out, uint64_t &n, uint64_t *or_all, uint64_t *and_all) {
+        const std::vector<uint64_t> h = read_raw(path);
+        n = h.size();
+        out.alloc(n * 8 + 8, s);
+        if (n) HIPCHK(hipMemcpyAsync(out.p, h.data(), n * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));  // h leaves scope
+        if (or_all) {
+            *or_all = 0;
+            *and_all = ~0ull;
+            for (uint64_t v : h) {
+                *or_all |= v;
+                *and_all &= v;
+            }
+        }
+    };
+    auto exists = [&](const std::string &suffix) { return file_exists(prefix + suffix); };
+    bool started = false;
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     free_b += device_idle_bytes(dev);
@@ -616,6 +659,51 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
 
     for (int i = 1; i < o.steps - 1; ++i) {
         const uint32_t bsum = (uint32_t)(2 * i) + ini - 2u;
+        if (files) {
+            // handle_restart_ad (BookGeneratorAD.cpp:391-420): a step whose
+            // layer (and the next) is on disk already, as a raw file or as
+            // books, is skipped; the first step run after skipped ones reads
+            // layers i-1 and i back
+            const std::string a = std::to_string(i), b = std::to_string(i + 1);
+            const bool have_i = exists(a);
+            if ((exists(b) && have_i) || (exists(b + "b") && have_i) || (exists(b + ".z") && have_i) ||
+                exists(a + "b") || exists(a + ".z") || exists(a + "b.7z") || exists(a + ".7z")) {
+                res.n[i] = have_i ? file_size(prefix + a) / 8 : 0;
+                if (o.keep_layers && have_i) {
+                    DBuf t;
+                    uint64_t m = 0;
+                    load(prefix + a, t, m, nullptr, nullptr);
+                    keep(i, t, m);
+                }
+                ++res.skipped_steps;
+                continue;
+            }
+        }
+        if (i == 1) {
+            // layer 0: the masked seeds as given (:409-411)
+            std::vector<uint64_t> h0(n_seeds);
+            for (size_t k = 0; k < n_seeds; ++k) h0[k] = mask_board(seeds[k]);
+            n0 = n_seeds;
+            d0.alloc(n0 * 8 + 8, s);
+            if (n0) HIPCHK(hipMemcpyAsync(d0.p, h0.data(), n0 * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));
+            res.n[0] = n0;
+            if (files) run.write_raw(prefix + "0", d0, n0);
+            keep(0, d0, n0);
+            started = true;
+        } else if (!started) {
+            load(prefix + std::to_string(i - 1), d0, n0, nullptr, nullptr);
+            load(prefix + std::to_string(i), d1, n1, &d1_or, &d1_and);
+            if (o.keep_layers && !res.layers[0].p) {
+                DBuf t;
+                uint64_t m = 0;
+                load(prefix + "0", t, m, nullptr, nullptr);
+                res.n[0] = m;
+                keep(0, t, m);
+            }
+            if (res.n[0] == 0 && file_exists(prefix + "0")) res.n[0] = file_size(prefix + "0") / 8;
+            started = true;
+        }
         std::vector<DBuf> runs2, runs4;
         std::vector<uint64_t> runs2_n, runs4_n;
         uint64_t or2 = d1_or, and2 = d1_and, or4 = 0, and4 = ~0ull;
@@ -718,10 +806,7 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
         }
         res.n[i] = m0;
         if (files) run.write_raw(prefix + std::to_string(i), layer, m0);
-        if (o.keep_layers) {
-            res.layers[i].alloc(m0 * 8 + 8, s);
-            if (m0) HIPCHK(hipMemcpyAsync(res.layers[i].p, layer.p, m0 * 8, hipMemcpyDeviceToDevice, s));
-        }
+        keep(i, layer, m0);
         d0 = std::move(layer);
         n0 = m0;
         d1 = std::move(next);
@@ -775,7 +860,7 @@ int egtb_ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_patter
 }
 
 int egtb_ad_result_info(const egtb_ad_result *r, int *steps, uint64_t *sizes, double *fwd_ms, uint64_t *candidates,
-                        uint64_t *chunked_steps) {
+                        uint64_t *chunked_steps, uint64_t *skipped_steps) {
     if (!r) {
         egtb::g_last_error = "result is NULL";
         return EGTB_ERR_ARG;
@@ -785,6 +870,7 @@ int egtb_ad_result_info(const egtb_ad_result *r, int *steps, uint64_t *sizes, do
     if (fwd_ms) *fwd_ms = r->fwd_ms;
     if (candidates) *candidates = r->candidates;
     if (chunked_steps) *chunked_steps = r->chunked_steps;
+    if (skipped_steps) *skipped_steps = r->skipped_steps;
     return EGTB_OK;
 }
 

@@ -165,7 +165,7 @@ SIGNATURES = {
     "egtb_ad_generate": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(AdPattern), ctypes.POINTER(AdOptions),
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "egtb_ad_result_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), u64p,
-                                           ctypes.POINTER(ctypes.c_double), u64p, u64p]),
+                                           ctypes.POINTER(ctypes.c_double), u64p, u64p, u64p]),
     "egtb_ad_result_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u64p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "egtb_ad_result_free": (None, [ctypes.c_void_p]),

@@ -181,14 +181,17 @@ typedef struct egtb_ad_result egtb_ad_result;
  * when pathname is set.  Replaces generate_process_ad
  * (reference native_core/src/BookGeneratorAD.cpp:1256-1396, called from
  * run_pattern_build_ad_cpp :1421-1438, bound as
- * formation_core.run_pattern_build_ad, bindings_formation.cpp:231-235).  A fresh
- * build: existing raw files are overwritten, not resumed.  out may be NULL. */
+ * formation_core.run_pattern_build_ad, bindings_formation.cpp:231-235).  With
+ * files it resumes as handle_restart_ad (:391-420) does: steps whose raw
+ * layers (or books) are on disk are skipped and the first step run reads
+ * layers i-1 and i back.  out may be NULL. */
 EGTB_EXPORT int egtb_ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern *pattern,
                                  const egtb_ad_options *options, egtb_ad_result **out);
 /* steps, the steps-1 layer sizes (sizes may be NULL), wall time, boards
- * emitted before dedup, and the steps that ran over parent ranges. */
+ * emitted before dedup, the steps that ran over parent ranges, and the
+ * steps found on disk and skipped. */
 EGTB_EXPORT int egtb_ad_result_info(const egtb_ad_result *r, int *steps, uint64_t *sizes, double *fwd_ms,
-                                    uint64_t *candidates, uint64_t *chunked_steps);
+                                    uint64_t *candidates, uint64_t *chunked_steps, uint64_t *skipped_steps);
 /* Copy raw layer `layer` to host (needs options.keep_layers; keys may be
  * NULL to query n). */
 EGTB_EXPORT int egtb_ad_result_layer(const egtb_ad_result *r, int layer, uint64_t *keys, size_t cap, size_t *n);

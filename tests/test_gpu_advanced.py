@@ -66,3 +66,36 @@ def test_parent_ranges_and_sort_schedule(pkg):
     print("case advanced t@128 with the single-pass sort schedule", flush=True)
     with _adv().generate_ad("t", 7, keep_layers=True, sort_schedule=2) as res:
         _check_layers(res, g, "t_128 schedule 2")
+
+
+def test_resume_from_raw_files(pkg):
+    """An interrupted pass resumes as the reference's handle_restart_ad does:
+    steps whose raw layers are on disk are skipped, the first step run reads
+    layers i-1 and i back, and every file ends up with the reference's bytes;
+    a raw file that is not whole boards is an error."""
+    g = load_ad_golden("442_128")
+    adv = _adv()
+    work = tempfile.mkdtemp(prefix="egtb_ad_resume_")
+    try:
+        prefix = os.path.join(work, "442_128_")
+        print("case advanced resume 442@128: full pass", flush=True)
+        adv.generate_ad("442", 7, prefix).close()
+        n = len(g["layers"])
+        for i in range(41, n):
+            os.remove(prefix + str(i))
+        print("case advanced resume 442@128: from layer 40", flush=True)
+        with adv.generate_ad("442", 7, prefix, keep_layers=True) as res:
+            assert res.skipped_steps == 39
+            _check_layers(res, g, "442_128 resumed")
+        for layer in g["layers"]:
+            raw = open(prefix + str(layer["i"]), "rb").read()
+            assert hashlib.sha256(raw).hexdigest() == layer["sha256"], layer["i"]
+        for i in range(41, n):
+            os.remove(prefix + str(i))
+        with open(prefix + "40", "r+b") as fh:
+            fh.truncate(8 * (g["layers"][40]["n"] - 1) + 3)
+        print("case advanced resume 442@128: misaligned layer 40", flush=True)
+        with pytest.raises(pkg.native.EgtbError, match="misaligned"):
+            adv.generate_ad("442", 7, prefix)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
