@@ -52,7 +52,7 @@ class _Args:
                  spawn_rate4, dtype, write_files, keep_raw, device, radix_bits, sort_schedule=0,
                  chunk_mkeys=0, host_tables=False, write_raw_files=False, deletion_threshold=0.0,
                  solve_hash_mb=0, forward_hash_steps=0, solve_hash_layout=0, optimal_branch_only=False,
-                 compress=False):
+                 compress=False, own_streams=False):
         self.seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
         self.masks = np.ascontiguousarray(np.asarray(list(masks) or [0], dtype=np.uint64))
         self.shifts = np.ascontiguousarray(np.asarray(list(shifts) or [0], dtype=np.uint8))
@@ -84,6 +84,7 @@ class _Args:
         o.solve_hash_layout = int(solve_hash_layout)
         o.optimal_branch_only = int(bool(optimal_branch_only))
         o.compress = int(bool(compress))
+        o.own_streams = int(bool(own_streams))
 
 
 class BuildResult:
@@ -170,7 +171,8 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
                       host_tables: bool = False, write_raw_files: int = 0,
                       deletion_threshold: float = 0.0, solve_hash_mb: int = 0,
                       forward_hash_steps: int = 0, solve_hash_layout: int = 0,
-                      optimal_branch_only: bool = False, compress: bool = False) -> Optional[BuildResult]:
+                      optimal_branch_only: bool = False, compress: bool = False,
+                      own_streams: bool = False) -> Optional[BuildResult]:
     """The native build call (reference formation_core.run_pattern_build,
     native_core/src/bindings_formation.cpp:220-229) with the PatternSpec and
     RunOptions fields spelled out.  ``keep=True`` returns the solved layers
@@ -180,7 +182,7 @@ def run_pattern_build(arr_init, masks: Sequence[int], shifts: Sequence[int], sym
     a = _Args(arr_init, masks, shifts, symm_mode, target, steps, docheck_step, pathname, is_free,
               spawn_rate4, success_rate_dtype, write_files, keep_raw, device, radix_bits, sort_schedule,
               chunk_mkeys, host_tables, write_raw_files, deletion_threshold, solve_hash_mb, forward_hash_steps,
-              solve_hash_layout, optimal_branch_only, compress)
+              solve_hash_layout, optimal_branch_only, compress, own_streams)
     L = native.lib()
     seeds_p = a.seeds.ctypes.data_as(native.u64p)
     if keep or keep_raw:
@@ -311,7 +313,8 @@ def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "flo
                     spawn_rate4: float = 0.1, keep_raw: bool = False, device: int = 0,
                     radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
                     host_tables: bool = False, solve_hash_mb: int = 0,
-                    forward_hash_steps: int = 0, solve_hash_layout: int = 0) -> BuildResult:
+                    forward_hash_steps: int = 0, solve_hash_layout: int = 0,
+                    own_streams: bool = False) -> BuildResult:
     """Build a catalogue pattern (layer 0 as start_build makes it) and keep
     every solved layer in HBM."""
     p = build_params(pattern, target)
@@ -321,4 +324,4 @@ def build_in_memory(pattern: str, target: int, *, success_rate_dtype: str = "flo
                              keep_raw=keep_raw, device=device, radix_bits=radix_bits,
                              sort_schedule=sort_schedule, chunk_mkeys=chunk_mkeys, host_tables=host_tables,
                              solve_hash_mb=solve_hash_mb, forward_hash_steps=forward_hash_steps,
-                             solve_hash_layout=solve_hash_layout)
+                             solve_hash_layout=solve_hash_layout, own_streams=own_streams)

@@ -231,7 +231,20 @@ struct HostView {
     size_t bytes = 0;
 };
 
+// Streams a build owns (options.own_streams): made for the build and kept
+// while the result's buffers may still be released on them; the destructor
+// waits for both and destroys them.
+struct OwnStreams {
+    hipStream_t s = nullptr, s2 = nullptr;
+    OwnStreams() = default;
+    OwnStreams(const OwnStreams &) = delete;
+    OwnStreams &operator=(const OwnStreams &) = delete;
+    ~OwnStreams();
+    void create();  // on the current device
+};
+
 struct Result {
+    OwnStreams streams;  // first member: destroyed after every buffer that may use them
     int steps = 0;
     int rate_size = 4;
     bool rate_is_float = true;

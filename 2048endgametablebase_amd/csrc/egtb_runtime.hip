@@ -1564,10 +1564,37 @@ struct EventGuard {
 // What the stages of one build share: the device, options and pattern,
 // the rate type's scale and zero, the per-layer records (pinned host mirror
 // H, device M), the resume state and the scratch one stage hands the next.
+OwnStreams::~OwnStreams() {
+    if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    if (s2) {
+        (void)hipStreamSynchronize(s2);
+        (void)hipStreamDestroy(s2);
+    }
+}
+
+void OwnStreams::create() {
+    if (s) return;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+}
+
+// The build stream and second stream of a build: the device's shared pair,
+// or (options.own_streams) a pair of the result's own, so that builds from
+// several host threads run side by side.
+static hipStream_t build_stream(DeviceCtx &C, const egtb_options &o, Result &res, bool second) {
+    if (!o.own_streams) return second ? C.aux : C.stream;
+    HIPCHK(hipSetDevice(C.dev));
+    res.streams.create();
+    return second ? res.streams.s2 : res.streams.s;
+}
+
 template <typename T>
 struct BuildCtx {
     DeviceCtx &C;
-    hipStream_t s;
+    hipStream_t s, s2;
     const egtb_options &o;
     const PatternDev &p;
     Result &res;
@@ -1596,7 +1623,8 @@ struct BuildCtx {
     JobArg ja(int step) const { return JobArg{jobs, job_err, step}; }
 
     BuildCtx(const egtb_options &o_, const PatternDev &p_, Result &res_, bool one_minus)
-        : C(device_ctx(o_.device)), s(C.stream), o(o_), p(p_), res(res_), steps(o_.steps),
+        : C(device_ctx(o_.device)), s(build_stream(C, o_, res_, false)), s2(build_stream(C, o_, res_, true)), o(o_),
+          p(p_), res(res_), steps(o_.steps),
           files(o_.write_files != 0), prefix(o_.pathname ? o_.pathname : ""),
           max_scale(one_minus ? T(0) : RateTraits<T>::max_scale()), zero(one_minus ? T(-1) : T(0)),
           w4(o_.spawn_rate4), w2(1.0 - o_.spawn_rate4), prog_total((uint32_t)(2 * o_.steps)), have_book(o_.steps, 0) {}
@@ -1806,7 +1834,7 @@ static void forward_pass(BuildCtx<T> &b, const uint64_t *seeds, size_t n_seeds) 
     // host time of an event pair.
     constexpr size_t kSortBatch = 1;
     const bool defer = o.sort_schedule == 0;
-    hipStream_t s2 = C.aux;
+    hipStream_t s2 = b.s2;
     SortWork w_side;  // the second stream's sort state
     std::vector<int> pending;
     struct Batch {
@@ -2306,7 +2334,7 @@ static void backward_pass(BuildCtx<T> &b) {
         // and i+2, fills layer i's (cleared by the solve of layer i+1) and
         // clears layer i-1's; then the compaction that makes book i runs on
         // the second stream, from this layer's own rates
-        hipStream_t s2 = C.aux;
+        hipStream_t s2 = b.s2;
         std::vector<uint64_t> roff(steps + 1, 0);
         for (int i = 0; i < steps; ++i)
             roff[i + 1] = roff[i] + ((res.info[i].n_raw * sizeof(T) + 255) & ~(uint64_t)255);

@@ -61,7 +61,7 @@ class Options(ctypes.Structure):
         ("solve_hash_layout", ctypes.c_int32),
         ("optimal_branch_only", ctypes.c_int32),
         ("compress", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("own_streams", ctypes.c_int32),
     ]
 
 

@@ -83,7 +83,9 @@ typedef struct egtb_options {
     int32_t compress;                /* RunOptions::compress: at the end of a build with files, every book
                                         above 2 MiB except layers 0-2 is replaced by its compressed form
                                         <pathname><i>.z (CompressionBridge.cpp:41-60) */
-    int32_t reserved0;
+    int32_t own_streams;             /* 1: the build runs on two streams of its own, kept with its result, so
+                                        builds from several host threads overlap on the device; 0: the
+                                        device's shared pair (builds queue behind each other) */
 } egtb_options;
 
 /* Build a tablebase end to end on the GPU and write the reference's file

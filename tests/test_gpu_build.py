@@ -299,3 +299,32 @@ def test_optimal_branch_pass_resumes(pkg, tmp_path):
         file_build(pkg, gp, b)
         assert not file_digest_mismatches(g, b), tag
         assert not os.path.exists(b + "optlayer")
+
+
+def test_concurrent_builds_on_own_streams(pkg):
+    """Builds from two host threads, each on streams of its own, overlap on
+    the device and each gives the reference's table."""
+    import threading
+
+    runs = [("442", 7, "442_128"), ("L3", 7, "L3_128"), ("442", 7, "442_128"), ("LL", 6, "LL_64")]
+    errors = []
+
+    def work(pattern, target, tag):
+        try:
+            res = pkg.build_in_memory(pattern, target, own_streams=True)
+            try:
+                for layer in load_golden(tag)["layers"]:
+                    keys, rates = res.book(layer["i"])
+                    assert book_sha(keys, rates) == layer["sha256"], (tag, layer["i"])
+            finally:
+                res.close()
+        except Exception as exc:  # reported by the main thread
+            errors.append(exc)
+
+    print("case concurrent builds 442@128, L3@128, 442@128, LL@64 on own streams", flush=True)
+    threads = [threading.Thread(target=work, args=r) for r in runs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
