@@ -395,7 +395,7 @@ def main():
         bm = importlib.import_module("2048endgametablebase_amd.batched")
         bm.build_batch(bm.SMALL_JOBS).close()  # warm-up
         batched = {"workload_24_jobs": [f"{p}@{2**t}" for p, t in bm.DEFAULT_JOBS],
-                   "uneven_24_jobs": bm.compare_with_plain(bm.DEFAULT_JOBS),
+                   "uneven_24_jobs": bm.compare_with_plain(bm.DEFAULT_JOBS, reps=2),
                    "small_24_jobs": bm.compare_with_plain(bm.SMALL_JOBS, reps=3),
                    "tiny_24_jobs": bm.compare_with_plain(bm.TINY_JOBS, reps=3)}
 
