@@ -291,7 +291,8 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
     (engine_core/BookBuilder.py:264-317, classic path)."""
     cfg = SingletonConfig().config
     if cfg.get("advanced_algo", False):
-        raise NotImplementedError("the advanced (masked) algorithm is not part of this engine")
+        raise NotImplementedError("the advanced (masked) algorithm's solver is not part of this engine; "
+                                  "advanced.generate_ad writes its forward pass (raw layers)")
     if pattern not in pattern_catalog:
         raise KeyError(f"Unknown pattern: {pattern}")
     p = build_params(pattern, target)

@@ -118,9 +118,14 @@ def ad_spec(pattern: str, target: int) -> Dict[str, object]:
 
 def generate_ad(pattern: str, target: int, pathname: Optional[str] = None, **kw) -> AdResult:
     """Forward pass of the advanced build of ``pattern`` at 2**target
-    (raw layer files under ``pathname`` when given)."""
+    (raw layer files under ``pathname`` when given), from the layer 0
+    start_build uses (engine_core/BookBuilder.py:272-285: generate_free_inits
+    for free* patterns whose small tiles fit the free cells as 2s)."""
+    from .BookBuilder import resolve_seeds
+
     s = ad_spec(pattern, target)
-    return run_pattern_generate_ad(s["seeds"], s, s["steps"], pathname, **kw)
+    seeds = resolve_seeds(pattern, patterns.build_params(pattern, target), int(kw.get("device", 0)))
+    return run_pattern_generate_ad(seeds, s, s["steps"], pathname, **kw)
 
 
 def _main(argv=None) -> None:

@@ -71,22 +71,24 @@ class PatternMeta:
         return -self.tile_sum
 
     @property
-    def free_cells(self) -> int:
-        pinned = 0
+    def mask_intersection(self) -> int:
+        """Cells every mask pins (get_nibble_intersection, Config.py:163-166:
+        0 when the pattern has no masks)."""
+        if not self.pattern_masks:
+            return 0
         inter = 0xFFFFFFFFFFFFFFFF
         for m in self.pattern_masks:
             inter &= m
-        for k in range(16):
-            if (inter >> (4 * k)) & 0xF == 0xF:
-                pinned += 1
-        return 16 - pinned
+        return inter
+
+    @property
+    def free_cells(self) -> int:
+        inter = self.mask_intersection
+        return 16 - sum(1 for k in range(16) if (inter >> (4 * k)) & 0xF == 0xF)
 
     @property
     def fixed_positions(self) -> Tuple[int, ...]:
-        inter = 0xFFFFFFFFFFFFFFFF
-        for m in self.pattern_masks:
-            inter &= m
-        return _f_nibble_shifts(inter)
+        return _f_nibble_shifts(self.mask_intersection)
 
 
 def _entry(name, category, masks, target_pos, mode, seeds, extra):

@@ -120,7 +120,7 @@ def make_query_pool(result, pattern: str, n: int = 1 << 18, off_table_frac: floa
     seeded uniform index over every live record of the build, with a random
     spawn applied; the rest are such boards with one pinned cell set to a
     random small tile, so no symmetry of them is in the pattern and they
-    take the fallback."""
+    take the fallback (none for a pattern without pinned cells)."""
     rng = np.random.default_rng(seed)
     sizes = np.array([result.layer_info(i)[1] for i in range(result.steps)], dtype=np.int64)
     cum = np.concatenate([[0], np.cumsum(sizes)])
@@ -135,7 +135,8 @@ def make_query_pool(result, pattern: str, n: int = 1 << 18, off_table_frac: floa
     boards = spawn_random(boards, rng, p4)
     meta = pattern_catalog[pattern]
     pinned = np.array(meta.fixed_positions, dtype=np.uint64)
-    n_off = int(round(off_table_frac * n))
+    # a pattern without pinned cells (free*) has no off-pattern boards
+    n_off = int(round(off_table_frac * n)) if pinned.size else 0
     sh = pinned[rng.integers(0, pinned.size, n_off)]
     v = rng.integers(1, 10, n_off).astype(np.uint64)
     boards[:n_off] = (boards[:n_off] & ~(np.uint64(0xF) << sh)) | (v << sh)

@@ -39,9 +39,11 @@ patterns = importlib.import_module("2048endgametablebase_amd.patterns")
 
 # (pattern, log2 target[, SmallTileSumLimit]): identity (442, L3, t with free
 # 32768 tiles), diagonal (LL) and horizontal (444) canonical forms; two runs
-# with other small-tile limits (other thresholds and validation steps)
+# with other small-tile limits (other thresholds and validation steps); free8
+# (no masks, full symmetry, eight free 32768 tiles) from its catalogue seed
+# as given
 RUNS = [("442", 6), ("L3", 6), ("t", 6), ("442", 7), ("L3", 7), ("t", 7), ("LL", 6), ("444", 6),
-        ("442", 8), ("444", 5), ("442", 7, 64), ("t", 7, 128)]
+        ("442", 8), ("444", 5), ("442", 7, 64), ("t", 7, 128), ("free8", 6)]
 
 
 def digest_run(name, t, workdir, limit=96):

@@ -11,7 +11,8 @@
     reference's own AD builder (tests/golden/ad_*.json): every layer of
     442@64 and t@64 (two free 32768 tiles), the first layers of LL@64
     (diagonal symmetry) and 444@64 (horizontal symmetry), including their
-    validation steps, and 442@128 with a small-tile limit of 64.
+    validation steps, 442@128 with a small-tile limit of 64, and the first
+    layers of free8@64 (no masks, full symmetry, eight free 32768 tiles).
 """
 import hashlib
 import importlib
@@ -161,6 +162,6 @@ def _check_forward(pattern, target, n_layers=None, limit=96):
 
 @pytest.mark.parametrize("pattern,target,n_layers,limit", [("442", 6, None, 96), ("t", 6, None, 96),
                                                            ("LL", 6, 28, 96), ("444", 6, 30, 96),
-                                                           ("442", 7, 50, 64)])
+                                                           ("442", 7, 50, 64), ("free8", 6, 16, 96)])
 def test_forward_matches_reference(pattern, target, n_layers, limit):
     _check_forward(pattern, target, n_layers, limit)

@@ -141,3 +141,13 @@ def test_tile_tuning_checks_arguments(pkg):
     assert "ascending" in pkg.native.last_error()
     assert L.egtb_set_tile_tuning(0, u64(1 << 20, 1 << 30), i32(4, 16), 2) == 0
     pkg.native._install_cached_tuning(L)  # back to the committed tables
+
+
+def test_free_patterns_pin_no_cells(pkg):
+    """A pattern without masks pins no cell (get_nibble_intersection of no
+    masks is 0, Config.py:163-166): all its 32768 tiles are free ones."""
+    cat = pkg.patterns.pattern_catalog
+    assert cat["free8"].fixed_positions == () and cat["free8"].free_cells == 16
+    p = pkg.patterns.ad_params("free8", 7)
+    assert p["n_free_32k"] == 8 and p["fixed_32k_shifts"] == [] and p["is_free"]
+    assert pkg.patterns.ad_params("free10", 7)["n_free_32k"] == 6
