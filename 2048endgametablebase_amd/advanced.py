@@ -44,6 +44,10 @@ class AdResult:
         self.candidates = cand.value
         self.chunked_steps = chunked.value
         self.skipped_steps = skipped.value
+        hs, ov = ctypes.c_uint64(), ctypes.c_uint64()
+        native.check(L.egtb_ad_result_forward(self._h, ctypes.byref(hs), ctypes.byref(ov)))
+        self.hash_steps = hs.value
+        self.set_overflows = ov.value
 
     def layer(self, i: int) -> np.ndarray:
         L = native.lib()
@@ -74,7 +78,8 @@ class AdResult:
 
 def run_pattern_generate_ad(arr_init: Sequence[int], spec: Dict[str, object], steps: int,
                             pathname: Optional[str] = None, *, keep_layers: bool = False, device: int = 0,
-                            radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0) -> AdResult:
+                            radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
+                            forward_dedup: int = 0) -> AdResult:
     """The forward pass of formation_core.run_pattern_build_ad
     (bindings_formation.cpp:231-235) for ``spec`` with the fields of
     AdvancedPatternSpec (FormationRuntime.h:88-96): pattern_masks,
@@ -100,6 +105,7 @@ def run_pattern_generate_ad(arr_init: Sequence[int], spec: Dict[str, object], st
     opt.radix_bits = int(radix_bits)
     opt.sort_schedule = int(sort_schedule)
     opt.chunk_mkeys = int(chunk_mkeys)
+    opt.forward_dedup = int(forward_dedup)
     h = ctypes.c_void_p()
     native.check(native.lib().egtb_ad_generate(seeds.ctypes.data_as(native.u64p), seeds.size, ctypes.byref(pat),
                                                ctypes.byref(opt), ctypes.byref(h)))

@@ -11,19 +11,23 @@
 // (BoardMaskerAD.cpp:422-456).  The layers written here are the reference's
 // raw layer files <pathname><i> byte for byte.
 //
-// Layout and schedule, per forward step i (parents = layer i-1, sorted):
-//   1. k_ad_expand<false>: one thread per parent, the masked row table in
-//      LDS; per parent the number of spawn-2 and spawn-4 boards (packed
-//      c2 | c4 << 32), their OR/AND key span and exact totals;
-//   2. exclusive scan of the packed counts;
-//   3. k_ad_expand<true>: the same boards written at their offsets;
-//   4. sort + unique (the classic path's radix sort and look-back unique):
-//      layer i = spawn-2 boards of step i with the spawn-4 boards of step
-//      i-1; the spawn-4 boards of step i wait for step i+1;
-//   5. on validation steps, k_ad_validate + scan + k_ad_scatter filter both.
-// Steps whose boards exceed the candidate budget run over parent ranges,
-// each range deduplicated into a run, and the runs merged by one more
-// sort + unique.
+// Layout and schedule, per forward step i (parents = layer i-1, sorted;
+// layer i = spawn-2 boards of step i with the spawn-4 boards of step i-1,
+// which wait for step i+1 as a sorted unique list):
+//   key sets (default): the spawn-4 list of step i-1 and the step's spawn-2
+//      boards go into one open-addressing set, its spawn-4 boards into
+//      another (k_ad_expand<kHash>: one thread per parent, the masked row
+//      table in LDS; exact totals and OR/AND spans on the side); both sets
+//      are extracted (unique, any order) and sorted. Sets are sized from the
+//      last step's growth at two slots per key, up to 2^25 slots; a step
+//      whose set would be larger, or overflowed, takes the sort path:
+//   sort + unique: k_ad_expand<kCount> (per parent the number of spawn-2 and
+//      spawn-4 boards, packed c2 | c4 << 32), an exclusive scan,
+//      k_ad_expand<kWrite> (the boards at their offsets), then the classic
+//      path's radix sort + look-back unique. Steps whose boards exceed the
+//      candidate budget run over parent ranges, each range deduplicated into
+//      a run, and the runs merged by one more sort + unique.
+// On validation steps k_ad_validate + scan + k_ad_scatter filter both lists.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -206,21 +210,32 @@ __device__ __forceinline__ bool ad_validate(uint64_t b, uint32_t board_sum, cons
 // BookGeneratorAD.cpp:785-960): for every empty cell, spawn 2 then 4, the
 // four moves in the reference's order; a moved board that changed and
 // matches the pattern is canonicalised; without a new masked tile it is
-// kept, with one it goes through derive.  WRITE = false: counts[i] gets
-// c2 | c4 << 32, acc[0..5] the OR/AND spans (atomic) and exact totals;
-// WRITE = true: counts[i] holds the exclusive scan and the boards are
-// written to out2 / out4 at counts[i] - offs_base.
-template <bool WRITE>
+// kept, with one it goes through derive.
+//   kCount: counts[i] gets c2 | c4 << 32, acc[0..5] the OR/AND spans
+//           (atomic) and exact totals;
+//   kWrite: counts[i] holds the exclusive scan and the boards are written
+//           to out2 / out4 at counts[i] - offs_base;
+//   kHash:  as kCount (without counts), the boards inserted into the key
+//           sets set2 / set4 (masks mask2 / mask4); acc[6] becomes nonzero
+//           when an insert found its probe run full (the step is redone).
+enum AdMode { kCount = 0, kWrite = 1, kHash = 2 };
+
+template <int MODE>
 __global__ __launch_bounds__(kAdThreads) void k_ad_expand(const uint64_t *__restrict__ par, uint64_t n,
                                                           const uint16_t *__restrict__ glut, AdSpecDev sp,
                                                           const uint8_t *__restrict__ comb, uint32_t board_sum,
                                                           uint64_t *__restrict__ counts, uint64_t offs_base,
                                                           uint64_t *__restrict__ out2, uint64_t *__restrict__ out4,
+                                                          uint64_t mask2, uint64_t mask4,
                                                           unsigned long long *__restrict__ acc) {
+    constexpr bool WRITE = MODE == kWrite;
     extern __shared__ uint16_t lut[];
     __shared__ uint64_t red[6][kAdThreads / 64];
+    __shared__ uint32_t s_fail;
+    if (MODE == kHash && threadIdx.x == 0) s_fail = 0;
     stage_lut(lut, glut);
     uint64_t or2 = 0, and2 = ~0ull, or4 = 0, and4 = ~0ull, x2 = 0, x4 = 0;
+    bool fail = false;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t b = par[i];
@@ -232,18 +247,22 @@ __global__ __launch_bounds__(kAdThreads) void k_ad_expand(const uint64_t *__rest
         }
         uint32_t c2 = 0, c4 = 0;
         auto put2 = [&](uint64_t key) {
-            if (WRITE) out2[w2 + c2] = key;
-            else {
+            if (WRITE) {
+                out2[w2 + c2] = key;
+            } else {
                 or2 |= key;
                 and2 &= key;
+                if (MODE == kHash && !set_insert(out2, mask2, key)) fail = true;
             }
             ++c2;
         };
         auto put4 = [&](uint64_t key) {
-            if (WRITE) out4[w4 + c4] = key;
-            else {
+            if (WRITE) {
+                out4[w4 + c4] = key;
+            } else {
                 or4 |= key;
                 and4 &= key;
+                if (MODE == kHash && !set_insert(out4, mask4, key)) fail = true;
             }
             ++c4;
         };
@@ -272,12 +291,13 @@ __global__ __launch_bounds__(kAdThreads) void k_ad_expand(const uint64_t *__rest
             }
         }
         if (!WRITE) {
-            counts[i] = (uint64_t)c2 | ((uint64_t)c4 << 32);
+            if (MODE == kCount) counts[i] = (uint64_t)c2 | ((uint64_t)c4 << 32);
             x2 += c2;
             x4 += c4;
         }
     }
     if (WRITE) return;
+    if (MODE == kHash && fail) s_fail = 1;  // benign race: any writer sets it
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         or2 |= __shfl_xor(or2, off, 64);
@@ -312,11 +332,12 @@ __global__ __launch_bounds__(kAdThreads) void k_ad_expand(const uint64_t *__rest
         atomicAnd(&acc[3], (unsigned long long)and4);
         if (x2) atomicAdd(&acc[4], (unsigned long long)x2);
         if (x4) atomicAdd(&acc[5], (unsigned long long)x4);
+        if (MODE == kHash && s_fail) atomicOr(&acc[6], 1ull);
     }
 }
 
 __global__ void k_ad_acc_init(unsigned long long *acc) {
-    if (threadIdx.x < 6) acc[threadIdx.x] = (threadIdx.x == 1 || threadIdx.x == 3) ? ~0ull : 0ull;
+    if (threadIdx.x < 7) acc[threadIdx.x] = (threadIdx.x == 1 || threadIdx.x == 3) ? ~0ull : 0ull;
 }
 
 __global__ void k_ad_validate(const uint64_t *__restrict__ keys, uint64_t n, uint32_t board_sum, AdSpecDev sp,
@@ -424,6 +445,8 @@ struct AdResult {
     uint64_t candidates = 0;     // boards emitted over every step
     uint64_t chunked_steps = 0, chunks = 0;
     uint64_t skipped_steps = 0;  // steps found on disk (resume)
+    uint64_t hash_steps = 0;     // steps deduplicated in key sets
+    uint64_t set_overflows = 0;  // key-set steps redone by sort + unique
 };
 
 namespace {
@@ -454,9 +477,9 @@ struct AdRun {
         acc.ensure(64, s);
         hipLaunchKernelGGL(k_ad_acc_init, dim3(1), dim3(64), 0, s, (unsigned long long *)acc.p);
         const size_t lds = kLutEntries * sizeof(uint16_t);
-        hipLaunchKernelGGL(k_ad_expand<false>, dim3(grid_for(n, kAdThreads, cus)), dim3(kAdThreads), lds, s, par,
+        hipLaunchKernelGGL(k_ad_expand<kCount>, dim3(grid_for(n, kAdThreads, cus)), dim3(kAdThreads), lds, s, par,
                            n, D.lut, spec, D.comb, bsum, (uint64_t *)counts.p, 0ull, (uint64_t *)nullptr,
-                           (uint64_t *)nullptr, (unsigned long long *)acc.p);
+                           (uint64_t *)nullptr, 0ull, 0ull, (unsigned long long *)acc.p);
         HIPCHK(hipGetLastError());
     }
 
@@ -467,9 +490,23 @@ struct AdRun {
         launch_scan_u64(s, (const uint64_t *)counts.p, n, nullptr, (uint64_t *)counts.p, (uint64_t *)partial.p,
                         nullptr);
         const size_t lds = kLutEntries * sizeof(uint16_t);
-        hipLaunchKernelGGL(k_ad_expand<true>, dim3(grid_for(n, kAdThreads, cus)), dim3(kAdThreads), lds, s, par, n,
-                           D.lut, spec, D.comb, bsum, (uint64_t *)counts.p, 0ull, out2, out4,
+        hipLaunchKernelGGL(k_ad_expand<kWrite>, dim3(grid_for(n, kAdThreads, cus)), dim3(kAdThreads), lds, s, par,
+                           n, D.lut, spec, D.comb, bsum, (uint64_t *)counts.p, 0ull, out2, out4, 0ull, 0ull,
                            (unsigned long long *)nullptr);
+        HIPCHK(hipGetLastError());
+    }
+
+    // one step into key sets of cap2 / cap4 slots (powers of two, zero);
+    // totals, spans and the overflow flag into acc[0..6]
+    void hash(const uint64_t *par, uint64_t n, uint32_t bsum, uint64_t *set2, uint64_t cap2, uint64_t *set4,
+              uint64_t cap4) {
+        acc.ensure(64, s);
+        hipLaunchKernelGGL(k_ad_acc_init, dim3(1), dim3(64), 0, s, (unsigned long long *)acc.p);
+        if (n == 0) return;
+        const size_t lds = kLutEntries * sizeof(uint16_t);
+        hipLaunchKernelGGL(k_ad_expand<kHash>, dim3(grid_for(n, kAdThreads, cus)), dim3(kAdThreads), lds, s, par,
+                           n, D.lut, spec, D.comb, bsum, (uint64_t *)nullptr, 0ull, set2, set4, cap2 - 1, cap4 - 1,
+                           (unsigned long long *)acc.p);
         HIPCHK(hipGetLastError());
     }
 
@@ -491,6 +528,16 @@ struct AdRun {
         HIPCHK(hipStreamSynchronize(s));
         a = v[0];
         b = v[1];
+    }
+
+    // keys[0..n) (unique) sorted; the buffer that holds them
+    DBuf sort_keys(DBuf &u, uint64_t n, uint64_t or_all, uint64_t and_all) {
+        if (n < 2) return std::move(u);
+        DBuf t;
+        t.alloc(n * 8 + 8, s);
+        const int sched = o.sort_schedule == 1 || o.sort_schedule == 2 ? o.sort_schedule : 0;
+        uint64_t *out = radix_sort(s, (uint64_t *)u.p, (uint64_t *)t.p, n, or_all, and_all, o.radix_bits, w, sched);
+        return out == (uint64_t *)u.p ? std::move(u) : std::move(t);
     }
 
     DBuf dedup(uint64_t *keys, uint64_t *tmp, uint64_t n, uint64_t or_all, uint64_t and_all, uint64_t &m) {
@@ -591,6 +638,7 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
         throw Error(EGTB_ERR_ARG, "radix_bits must be 0, 4, 6, 8 or 11");
     if (o.sort_schedule < 0 || o.sort_schedule > 3) throw Error(EGTB_ERR_ARG, "sort_schedule must be 0-3");
     if (o.chunk_mkeys < 0) throw Error(EGTB_ERR_ARG, "chunk_mkeys must be >= 0");
+    if (o.forward_dedup < 0 || o.forward_dedup > 2) throw Error(EGTB_ERR_ARG, "forward_dedup must be 0, 1 or 2");
 
     AdSpecDev sp{};
     for (uint32_t i = 0; i < pat.n_masks; ++i) sp.masks[i] = pat.masks[i];
@@ -656,6 +704,24 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
     const uint64_t budget = o.chunk_mkeys ? (uint64_t)o.chunk_mkeys << 20
                                           : std::max<uint64_t>(1ull << 22, (uint64_t)(free_b / 8 / 8));
     const uint64_t kMaxSort = (1ull << 32) - 1;
+    // key sets: two slots per predicted key, sets up to 2^25 slots (256 MiB,
+    // the size of the last-level cache the random inserts live in)
+    const bool use_sets = o.forward_dedup != 1 && o.chunk_mkeys == 0;
+    const double slots_per_key = o.forward_dedup == 2 ? 2.0 / 64.0 : 2.0;
+    const uint64_t min_slots = o.forward_dedup == 2 ? 1024 : (1ull << 15), max_slots = 1ull << 25;
+    DBuf set2, set4;
+    double r2 = 0, r4 = 0;  // last step's layer / parents and next / parents
+    auto cap_for = [&](double keys) {
+        uint64_t c = min_slots;
+        while ((double)c < keys * slots_per_key) c <<= 1;
+        return c;
+    };
+    auto set_ready = [&](DBuf &b, uint64_t cap) {
+        if (b.bytes < cap * 8) {
+            b.alloc(cap * 8, s);
+            HIPCHK(hipMemsetAsync(b.p, 0, b.bytes, s));
+        }
+    };
 
     for (int i = 1; i < o.steps - 1; ++i) {
         const uint32_t bsum = (uint32_t)(2 * i) + ini - 2u;
@@ -704,101 +770,149 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
             if (res.n[0] == 0 && file_exists(prefix + "0")) res.n[0] = file_size(prefix + "0") / 8;
             started = true;
         }
-        std::vector<DBuf> runs2, runs4;
-        std::vector<uint64_t> runs2_n, runs4_n;
-        uint64_t or2 = d1_or, and2 = d1_and, or4 = 0, and4 = ~0ull;
-        // parent ranges: the whole layer, halved until a range's boards fit
-        std::vector<std::pair<uint64_t, uint64_t>> todo{{0, n0}};
-        bool whole = true;
-        DBuf c2buf, c4buf, tmp;
-        uint64_t t2_whole = 0;
-        while (!todo.empty()) {
-            const auto [lo, hi] = todo.back();
-            todo.pop_back();
-            const uint64_t np = hi - lo;
-            if (np == 0) continue;
-            const uint64_t *par = (const uint64_t *)d0.p + lo;
-            run.count(par, np, bsum);
-            uint64_t a[6];
-            HIPCHK(hipMemcpyAsync(a, run.acc.p, sizeof a, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            const uint64_t t2 = a[4], t4 = a[5];
-            const uint64_t extra = whole ? n1 : 0;  // a whole step also takes d1 into the spawn-2 buffer
-            if ((t2 + extra > budget || t4 > budget || t2 + extra > kMaxSort || t4 > kMaxSort ||
-                 t2 >= (1ull << 32) || t4 >= (1ull << 32)) && np > 1) {
-                whole = false;
-                todo.push_back({lo + np / 2, hi});
-                todo.push_back({lo, lo + np / 2});
-                continue;
-            }
-            if (t2 + extra > kMaxSort || t4 > kMaxSort)
-                throw Error(EGTB_ERR_MEMORY, "one parent's boards exceed a sort");
-            res.candidates += t2 + t4;
-            or2 |= a[0];
-            and2 &= a[1];
-            or4 |= a[2];
-            and4 &= a[3];
-            c2buf.ensure((t2 + extra) * 8 + 8, s);
-            c4buf.ensure(t4 * 8 + 8, s);
-            run.write(par, np, bsum, (uint64_t *)c2buf.p, (uint64_t *)c4buf.p);
-            tmp.ensure(std::max(t2 + extra, t4) * 8 + 8, s);
-            if (whole) {
-                if (n1) HIPCHK(hipMemcpyAsync((uint64_t *)c2buf.p + t2, d1.p, n1 * 8, hipMemcpyDeviceToDevice, s));
-                t2_whole = t2 + n1;
-            } else {
-                ++res.chunks;
-            }
-            uint64_t m2 = 0, m4 = 0;
-            const uint64_t n2 = whole ? t2_whole : t2;
-            runs2.push_back(run.dedup_async((uint64_t *)c2buf.p, (uint64_t *)tmp.p, n2, whole ? or2 : (a[0]),
-                                            whole ? and2 : (a[1]), 0));
-            runs4.push_back(run.dedup_async((uint64_t *)c4buf.p, (uint64_t *)tmp.p, t4, a[2], a[3], 1));
-            run.counts2(m2, m4);
-            runs2_n.push_back(m2);
-            runs4_n.push_back(m4);
-        }
-        c2buf.release();
-        c4buf.release();
-        tmp.release();
-        // merge runs: layer i = spawn-2 runs (+ d1 when the step was split)
-        auto merge = [&](std::vector<DBuf> &rs, std::vector<uint64_t> &rn, const DBuf *extra_buf,
-                         uint64_t extra_n, uint64_t or_all, uint64_t and_all, uint64_t &m) -> DBuf {
-            if (rs.size() == 1 && !extra_buf) {
-                m = rn[0];
-                return std::move(rs[0]);
-            }
-            uint64_t tot = extra_n;
-            for (uint64_t v : rn) tot += v;
-            DBuf cat, tbuf;
-            cat.alloc(tot * 8 + 8, s);
-            uint64_t off = 0;
-            for (size_t k = 0; k < rs.size(); ++k) {
-                if (rn[k]) HIPCHK(hipMemcpyAsync((uint64_t *)cat.p + off, rs[k].p, rn[k] * 8, hipMemcpyDeviceToDevice, s));
-                off += rn[k];
-                rs[k].release();
-            }
-            if (extra_n) HIPCHK(hipMemcpyAsync((uint64_t *)cat.p + off, extra_buf->p, extra_n * 8, hipMemcpyDeviceToDevice, s));
-            tbuf.alloc(tot * 8 + 8, s);
-            return run.dedup((uint64_t *)cat.p, (uint64_t *)tbuf.p, tot, or_all, and_all, m);
-        };
         uint64_t m0 = 0, m1 = 0;
-        DBuf layer;
-        if (runs2.empty()) {
-            // no parents: the layer is d1 as it stands
-            layer.alloc(n1 * 8 + 8, s);
-            if (n1) HIPCHK(hipMemcpyAsync(layer.p, d1.p, n1 * 8, hipMemcpyDeviceToDevice, s));
-            m0 = n1;
-        } else {
-            layer = merge(runs2, runs2_n, whole ? nullptr : &d1, whole ? 0 : n1, or2, and2, m0);
+        uint64_t or4 = 0, and4 = ~0ull;
+        DBuf layer, next;
+        bool done = false;
+        if (use_sets && n0) {
+            // the step into key sets: layer i's set takes d1 and the spawn-2
+            // boards, layer i+1's the spawn-4 boards; both are extracted
+            // (unique, any order) and sorted
+            const double pn = (double)n0;
+            const uint64_t cap2 = cap_for(std::max((double)n1, r2 * pn) * 1.3 + 1024.0);
+            const uint64_t cap4 = cap_for(r4 * pn * 1.3 + 1024.0);
+            if (cap2 <= max_slots && cap4 <= max_slots) {
+                set_ready(set2, cap2);
+                set_ready(set4, cap4);
+                if (n1) launch_set_insert(s, (const uint64_t *)d1.p, n1, (uint64_t *)set2.p, cap2);
+                run.hash((const uint64_t *)d0.p, n0, bsum, (uint64_t *)set2.p, cap2, (uint64_t *)set4.p, cap4);
+                uint64_t a[7];
+                HIPCHK(hipMemcpyAsync(a, run.acc.p, sizeof a, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                if (a[6] == 0) {
+                    DBuf u2, u4;
+                    u2.alloc(cap2 * 8, s);
+                    u4.alloc(cap4 * 8, s);
+                    run.cnt_dev.ensure(64, s);
+                    HIPCHK(hipMemsetAsync(run.cnt_dev.p, 0, 16, s));
+                    launch_set_extract(s, (uint64_t *)set2.p, cap2, (uint64_t *)u2.p, (uint64_t *)run.cnt_dev.p);
+                    launch_set_extract(s, (uint64_t *)set4.p, cap4, (uint64_t *)u4.p,
+                                       (uint64_t *)run.cnt_dev.p + 1);
+                    run.counts2(m0, m1);
+                    layer = run.sort_keys(u2, m0, d1_or | a[0], d1_and & a[1]);
+                    next = run.sort_keys(u4, m1, a[2], a[3]);
+                    or4 = a[2];
+                    and4 = a[3];
+                    res.candidates += a[4] + a[5];
+                    ++res.hash_steps;
+                    done = true;
+                } else {
+                    // an insert found its probe run full: empty both sets and
+                    // run the step again through sort + unique
+                    HIPCHK(hipMemsetAsync(set2.p, 0, cap2 * 8, s));
+                    HIPCHK(hipMemsetAsync(set4.p, 0, cap4 * 8, s));
+                    ++res.set_overflows;
+                }
+            }
         }
-        DBuf next;
-        if (runs4.empty()) {
-            next.alloc(8, s);
-            m1 = 0;
-        } else {
-            next = merge(runs4, runs4_n, nullptr, 0, or4, and4, m1);
+        if (!done) {
+            std::vector<DBuf> runs2, runs4;
+            std::vector<uint64_t> runs2_n, runs4_n;
+            uint64_t or2 = d1_or, and2 = d1_and;
+            // parent ranges: the whole layer, halved until a range's boards fit
+            std::vector<std::pair<uint64_t, uint64_t>> todo{{0, n0}};
+            bool whole = true;
+            DBuf c2buf, c4buf, tmp;
+            uint64_t t2_whole = 0;
+            while (!todo.empty()) {
+                const auto [lo, hi] = todo.back();
+                todo.pop_back();
+                const uint64_t np = hi - lo;
+                if (np == 0) continue;
+                const uint64_t *par = (const uint64_t *)d0.p + lo;
+                run.count(par, np, bsum);
+                uint64_t a[6];
+                HIPCHK(hipMemcpyAsync(a, run.acc.p, sizeof a, hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                const uint64_t t2 = a[4], t4 = a[5];
+                const uint64_t extra = whole ? n1 : 0;  // a whole step also takes d1 into the spawn-2 buffer
+                if ((t2 + extra > budget || t4 > budget || t2 + extra > kMaxSort || t4 > kMaxSort ||
+                     t2 >= (1ull << 32) || t4 >= (1ull << 32)) && np > 1) {
+                    whole = false;
+                    todo.push_back({lo + np / 2, hi});
+                    todo.push_back({lo, lo + np / 2});
+                    continue;
+                }
+                if (t2 + extra > kMaxSort || t4 > kMaxSort)
+                    throw Error(EGTB_ERR_MEMORY, "one parent's boards exceed a sort");
+                res.candidates += t2 + t4;
+                or2 |= a[0];
+                and2 &= a[1];
+                or4 |= a[2];
+                and4 &= a[3];
+                c2buf.ensure((t2 + extra) * 8 + 8, s);
+                c4buf.ensure(t4 * 8 + 8, s);
+                run.write(par, np, bsum, (uint64_t *)c2buf.p, (uint64_t *)c4buf.p);
+                tmp.ensure(std::max(t2 + extra, t4) * 8 + 8, s);
+                if (whole) {
+                    if (n1) HIPCHK(hipMemcpyAsync((uint64_t *)c2buf.p + t2, d1.p, n1 * 8, hipMemcpyDeviceToDevice, s));
+                    t2_whole = t2 + n1;
+                } else {
+                    ++res.chunks;
+                }
+                uint64_t m2 = 0, m4 = 0;
+                const uint64_t n2 = whole ? t2_whole : t2;
+                runs2.push_back(run.dedup_async((uint64_t *)c2buf.p, (uint64_t *)tmp.p, n2, whole ? or2 : (a[0]),
+                                                whole ? and2 : (a[1]), 0));
+                runs4.push_back(run.dedup_async((uint64_t *)c4buf.p, (uint64_t *)tmp.p, t4, a[2], a[3], 1));
+                run.counts2(m2, m4);
+                runs2_n.push_back(m2);
+                runs4_n.push_back(m4);
+            }
+            c2buf.release();
+            c4buf.release();
+            tmp.release();
+            // merge runs: layer i = spawn-2 runs (+ d1 when the step was split)
+            auto merge = [&](std::vector<DBuf> &rs, std::vector<uint64_t> &rn, const DBuf *extra_buf,
+                             uint64_t extra_n, uint64_t or_all, uint64_t and_all, uint64_t &m) -> DBuf {
+                if (rs.size() == 1 && !extra_buf) {
+                    m = rn[0];
+                    return std::move(rs[0]);
+                }
+                uint64_t tot = extra_n;
+                for (uint64_t v : rn) tot += v;
+                DBuf cat, tbuf;
+                cat.alloc(tot * 8 + 8, s);
+                uint64_t off = 0;
+                for (size_t k = 0; k < rs.size(); ++k) {
+                    if (rn[k]) HIPCHK(hipMemcpyAsync((uint64_t *)cat.p + off, rs[k].p, rn[k] * 8, hipMemcpyDeviceToDevice, s));
+                    off += rn[k];
+                    rs[k].release();
+                }
+                if (extra_n) HIPCHK(hipMemcpyAsync((uint64_t *)cat.p + off, extra_buf->p, extra_n * 8, hipMemcpyDeviceToDevice, s));
+                tbuf.alloc(tot * 8 + 8, s);
+                return run.dedup((uint64_t *)cat.p, (uint64_t *)tbuf.p, tot, or_all, and_all, m);
+            };
+            if (runs2.empty()) {
+                // no parents: the layer is d1 as it stands
+                layer.alloc(n1 * 8 + 8, s);
+                if (n1) HIPCHK(hipMemcpyAsync(layer.p, d1.p, n1 * 8, hipMemcpyDeviceToDevice, s));
+                m0 = n1;
+            } else {
+                layer = merge(runs2, runs2_n, whole ? nullptr : &d1, whole ? 0 : n1, or2, and2, m0);
+            }
+            if (runs4.empty()) {
+                next.alloc(8, s);
+                m1 = 0;
+            } else {
+                next = merge(runs4, runs4_n, nullptr, 0, or4, and4, m1);
+            }
+            if (!whole) ++res.chunked_steps;
+        }  // sort + unique
+        if (n0) {
+            r2 = (double)m0 / (double)n0;
+            r4 = (double)m1 / (double)n0;
         }
-        if (!whole) ++res.chunked_steps;
         // validation every 32 steps (BookGeneratorAD.cpp:1383-1387)
         if (((i + (int)(ini % 64u / 2u)) % 32) == (int)((sp.limit / 2u) % 32u) + 1) {
             m0 = run.validate(layer, m0, bsum + 2u);
@@ -871,6 +985,16 @@ int egtb_ad_result_info(const egtb_ad_result *r, int *steps, uint64_t *sizes, do
     if (candidates) *candidates = r->candidates;
     if (chunked_steps) *chunked_steps = r->chunked_steps;
     if (skipped_steps) *skipped_steps = r->skipped_steps;
+    return EGTB_OK;
+}
+
+int egtb_ad_result_forward(const egtb_ad_result *r, uint64_t *hash_steps, uint64_t *set_overflows) {
+    if (!r) {
+        egtb::g_last_error = "result is NULL";
+        return EGTB_ERR_ARG;
+    }
+    if (hash_steps) *hash_steps = r->hash_steps;
+    if (set_overflows) *set_overflows = r->set_overflows;
     return EGTB_OK;
 }
 

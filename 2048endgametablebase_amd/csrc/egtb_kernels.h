@@ -164,6 +164,27 @@ __device__ __forceinline__ uint64_t hash_mix(uint64_t x) {
     return x;
 }
 
+// Insert into an open-addressing key set (slots hold the key, 0 = empty;
+// linear probing from hash_mix). A slot only goes from 0 to a key within a
+// launch, so a plain load that sees a key is exact and one that sees 0 is
+// settled by the CAS. false: kSetProbeLimit slots in a row were taken (the
+// key was not inserted; the caller raises its overflow flag).
+constexpr uint32_t kSetProbeLimit = 256;
+
+__device__ __forceinline__ bool set_insert(uint64_t *__restrict__ tab, uint64_t mask, uint64_t k) {
+    uint64_t h = hash_mix(k) & mask;
+    for (uint32_t q = 0; q < kSetProbeLimit; ++q) {
+        const uint64_t cur = tab[h];
+        if (cur == k) return true;
+        if (cur == 0) {
+            const uint64_t old = atomicCAS((unsigned long long *)&tab[h], 0ull, (unsigned long long)k);
+            if (old == 0 || old == k) return true;
+        }
+        h = (h + 1) & mask;
+    }
+    return false;
+}
+
 template <typename T>
 __device__ __forceinline__ T rate_from_bits(uint64_t v) {
     if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, (uint32_t)v);

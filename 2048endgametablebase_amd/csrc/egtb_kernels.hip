@@ -369,22 +369,6 @@ __global__ __launch_bounds__(1024) void k_expand_alloc(const uint64_t *__restric
 // runs the step again.
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kSetProbeLimit = 256;
-
-__device__ __forceinline__ bool set_insert(uint64_t *__restrict__ tab, uint64_t mask, uint64_t k) {
-    uint64_t h = hash_mix(k) & mask;
-    for (uint32_t q = 0; q < kSetProbeLimit; ++q) {
-        const uint64_t cur = tab[h];
-        if (cur == k) return true;
-        if (cur == 0) {
-            const uint64_t old = atomicCAS((unsigned long long *)&tab[h], 0ull, (unsigned long long)k);
-            if (old == 0 || old == k) return true;
-        }
-        h = (h + 1) & mask;
-    }
-    return false;
-}
-
 template <bool JOBS>
 __global__ __launch_bounds__(1024) void k_expand_hash(const uint64_t *__restrict__ par, uint64_t n_bound,
                                                       const uint64_t *__restrict__ n_ptr,

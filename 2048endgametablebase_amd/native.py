@@ -140,7 +140,8 @@ class AdOptions(ctypes.Structure):
         ("radix_bits", ctypes.c_int32),
         ("sort_schedule", ctypes.c_int32),
         ("chunk_mkeys", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("forward_dedup", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -168,6 +169,7 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_double), u64p, u64p, u64p]),
     "egtb_ad_result_layer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u64p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_size_t)]),
+    "egtb_ad_result_forward": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p]),
     "egtb_ad_result_free": (None, [ctypes.c_void_p]),
     "egtb_batch_build": (ctypes.c_int, [ctypes.POINTER(BatchJob), ctypes.c_int, ctypes.POINTER(BatchOptions),
                                         ctypes.POINTER(ctypes.c_void_p)]),

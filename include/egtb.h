@@ -171,8 +171,11 @@ typedef struct egtb_ad_options {
     int32_t device;          /* HIP device ordinal */
     int32_t radix_bits;      /* 0 = autotuned; else 4, 6, 8 or 11 */
     int32_t sort_schedule;   /* 0 = autotuned, 1 = per-digit 3 kernels, 2 = single-pass look-back */
-    int32_t chunk_mkeys;     /* boards per parent range in Mi keys; 0 = from free HBM */
-    int32_t reserved[3];
+    int32_t chunk_mkeys;     /* boards per parent range in Mi keys (sort + unique steps); 0 = from free HBM */
+    int32_t forward_dedup;   /* 0: key sets (up to 2^25 slots), sort + unique where a set would be larger or
+                                overflowed; 1: sort + unique at every step; 2: key sets at 1/64 of their size
+                                (exercises the overflow path) */
+    int32_t reserved[2];
 } egtb_ad_options;
 
 typedef struct egtb_ad_result egtb_ad_result;
@@ -194,6 +197,9 @@ EGTB_EXPORT int egtb_ad_generate(const uint64_t *seeds, size_t n_seeds, const eg
  * steps found on disk and skipped. */
 EGTB_EXPORT int egtb_ad_result_info(const egtb_ad_result *r, int *steps, uint64_t *sizes, double *fwd_ms,
                                     uint64_t *candidates, uint64_t *chunked_steps, uint64_t *skipped_steps);
+/* Forward dedup of the pass: steps deduplicated in key sets, and key-set
+ * steps whose sets overflowed and ran again through sort + unique. */
+EGTB_EXPORT int egtb_ad_result_forward(const egtb_ad_result *r, uint64_t *hash_steps, uint64_t *set_overflows);
 /* Copy raw layer `layer` to host (needs options.keep_layers; keys may be
  * NULL to query n). */
 EGTB_EXPORT int egtb_ad_result_layer(const egtb_ad_result *r, int layer, uint64_t *keys, size_t cap, size_t *n);

@@ -68,6 +68,26 @@ def test_parent_ranges_and_sort_schedule(pkg):
         _check_layers(res, g, "t_128 schedule 2")
 
 
+@pytest.mark.parametrize("tag", ["t_128", "LL_64"])
+def test_forward_dedup_paths(pkg, tag):
+    """Key sets (the default), sort + unique at every step, and key sets
+    sized at 1/64 (steps whose sets overflow run again through sort +
+    unique) give the same layers."""
+    g = load_ad_golden(tag)
+    adv = _adv()
+    with adv.generate_ad(g["pattern"], g["target"], keep_layers=True) as res:
+        assert res.hash_steps > 0.9 * (res.steps - 2) and res.set_overflows == 0, (res.hash_steps,
+                                                                                  res.set_overflows)
+    for mode in (1, 2):
+        print(f"case advanced {tag} forward_dedup {mode}", flush=True)
+        with adv.generate_ad(g["pattern"], g["target"], keep_layers=True, forward_dedup=mode) as res:
+            _check_layers(res, g, f"{tag} forward_dedup {mode}")
+            if mode == 1:
+                assert res.hash_steps == 0
+            else:
+                assert res.set_overflows > 0 and res.hash_steps > 0, (res.hash_steps, res.set_overflows)
+
+
 def test_resume_from_raw_files(pkg):
     """An interrupted pass resumes as the reference's handle_restart_ad does:
     steps whose raw layers are on disk are skipped, the first step run reads
