@@ -13,12 +13,13 @@
 //
 // Layout and schedule, per forward step i (parents = layer i-1, sorted;
 // layer i = spawn-2 boards of step i with the spawn-4 boards of step i-1,
-// which wait for step i+1 as a sorted unique list):
+// which wait for step i+1 as a unique list):
 //   key sets (default): the spawn-4 list of step i-1 and the step's spawn-2
 //      boards go into one open-addressing set, its spawn-4 boards into
 //      another (k_ad_expand<kHash>: one thread per parent, the masked row
 //      table in LDS; exact totals and OR/AND spans on the side); both sets
-//      are extracted (unique, any order) and sorted. Sets are sized from the
+//      are extracted (unique, any order) and layer i is sorted (the spawn-4
+//      list only feeds the next step's set, in any order). Sets are sized from the
 //      last step's growth at two slots per key, up to 2^25 slots; a step
 //      whose set would be larger, or overflowed, takes the sort path:
 //   sort + unique: k_ad_expand<kCount> (per parent the number of spawn-2 and
@@ -800,7 +801,9 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
                                        (uint64_t *)run.cnt_dev.p + 1);
                     run.counts2(m0, m1);
                     layer = run.sort_keys(u2, m0, d1_or | a[0], d1_and & a[1]);
-                    next = run.sort_keys(u4, m1, a[2], a[3]);
+                    // the spawn-4 list only goes into the next step's set (or
+                    // its sort + unique), so it stays in set order
+                    next = std::move(u4);
                     or4 = a[2];
                     and4 = a[3];
                     res.candidates += a[4] + a[5];
