@@ -13,15 +13,18 @@
 //
 // Layout and schedule, per forward step i (parents = layer i-1, sorted;
 // layer i = spawn-2 boards of step i with the spawn-4 boards of step i-1,
-// which wait for step i+1 as a unique list):
-//   key sets (default): the spawn-4 list of step i-1 and the step's spawn-2
-//      boards go into one open-addressing set, its spawn-4 boards into
-//      another (k_ad_expand<kHash>: one thread per parent, the masked row
-//      table in LDS; exact totals and OR/AND spans on the side); both sets
-//      are extracted (unique, any order) and layer i is sorted (the spawn-4
-//      list only feeds the next step's set, in any order). Sets are sized from the
+// which wait for step i+1 in its key set or as a unique list):
+//   key sets (default): layer i's open-addressing set holds the spawn-4
+//      boards of step i-1 (the set carried from that step, or their list
+//      inserted) and takes the step's spawn-2 boards; layer i+1's set takes
+//      its spawn-4 boards and is carried to step i+1 (k_ad_expand<kHash>:
+//      one thread per parent, the masked row table in LDS; exact totals and
+//      OR/AND spans on the side). Layer i's set is extracted (unique, any
+//      order) and sorted. Sets are sized from the
 //      last step's growth at two slots per key, up to 2^25 slots; a step
-//      whose set would be larger, or overflowed, takes the sort path:
+//      whose set would be larger, or overflowed, takes the sort path
+//      (a carried set is drained into the list first; validation steps
+//      drain it too):
 //   sort + unique: k_ad_expand<kCount> (per parent the number of spawn-2 and
 //      spawn-4 boards, packed c2 | c4 << 32), an exclusive scan,
 //      k_ad_expand<kWrite> (the boards at their offsets), then the classic
@@ -711,7 +714,12 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
     const double slots_per_key = o.forward_dedup == 2 ? 2.0 / 64.0 : 2.0;
     const uint64_t min_slots = o.forward_dedup == 2 ? 1024 : (1ull << 15), max_slots = 1ull << 25;
     DBuf set2, set4;
-    double r2 = 0, r4 = 0;  // last step's layer / parents and next / parents
+    double r2 = 0;  // last step's layer / parents
+    // carry: the spawn-4 boards of the last step are in set2 (carry_cap
+    // slots), which the next step fills with its spawn-2 boards, instead of
+    // being the list d1
+    bool carry = false;
+    uint64_t carry_cap = 0;
     auto cap_for = [&](double keys) {
         uint64_t c = min_slots;
         while ((double)c < keys * slots_per_key) c <<= 1;
@@ -722,6 +730,17 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
             b.alloc(cap * 8, s);
             HIPCHK(hipMemsetAsync(b.p, 0, b.bytes, s));
         }
+    };
+    // the carried set's keys into the list d1 (the set is left empty)
+    auto drain = [&]() {
+        DBuf u;
+        u.alloc(carry_cap * 8, s);
+        run.cnt_dev.ensure(64, s);
+        HIPCHK(hipMemsetAsync(run.cnt_dev.p, 0, 8, s));
+        launch_set_extract(s, (uint64_t *)set2.p, carry_cap, (uint64_t *)u.p, (uint64_t *)run.cnt_dev.p);
+        n1 = run.read_u64(run.cnt_dev.p);
+        d1 = std::move(u);
+        carry = false;
     };
 
     for (int i = 1; i < o.steps - 1; ++i) {
@@ -776,48 +795,57 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
         DBuf layer, next;
         bool done = false;
         if (use_sets && n0) {
-            // the step into key sets: layer i's set takes d1 and the spawn-2
-            // boards, layer i+1's the spawn-4 boards; both are extracted
-            // (unique, any order) and sorted
+            // the step into key sets: layer i's set holds the spawn-4 boards
+            // of step i-1 (carried, or inserted from the list d1) and takes
+            // the spawn-2 boards; layer i+1's set takes the spawn-4 boards
+            // and is carried to the next step. Layer i's set is extracted
+            // (unique, any order) and sorted. Layer i+1's set is sized for
+            // the whole of layer i+1 from the last growth ratio.
             const double pn = (double)n0;
-            const uint64_t cap2 = cap_for(std::max((double)n1, r2 * pn) * 1.3 + 1024.0);
-            const uint64_t cap4 = cap_for(r4 * pn * 1.3 + 1024.0);
+            const uint64_t cap2 = carry ? carry_cap : cap_for(std::max((double)n1, r2 * pn) * 1.3 + 1024.0);
+            const uint64_t cap4 = cap_for(r2 * r2 * pn * 1.3 + 1024.0);
             if (cap2 <= max_slots && cap4 <= max_slots) {
-                set_ready(set2, cap2);
+                if (!carry) set_ready(set2, cap2);
                 set_ready(set4, cap4);
-                if (n1) launch_set_insert(s, (const uint64_t *)d1.p, n1, (uint64_t *)set2.p, cap2);
+                if (!carry && n1) launch_set_insert(s, (const uint64_t *)d1.p, n1, (uint64_t *)set2.p, cap2);
                 run.hash((const uint64_t *)d0.p, n0, bsum, (uint64_t *)set2.p, cap2, (uint64_t *)set4.p, cap4);
                 uint64_t a[7];
                 HIPCHK(hipMemcpyAsync(a, run.acc.p, sizeof a, hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
                 if (a[6] == 0) {
-                    DBuf u2, u4;
+                    DBuf u2;
                     u2.alloc(cap2 * 8, s);
-                    u4.alloc(cap4 * 8, s);
                     run.cnt_dev.ensure(64, s);
-                    HIPCHK(hipMemsetAsync(run.cnt_dev.p, 0, 16, s));
+                    HIPCHK(hipMemsetAsync(run.cnt_dev.p, 0, 8, s));
                     launch_set_extract(s, (uint64_t *)set2.p, cap2, (uint64_t *)u2.p, (uint64_t *)run.cnt_dev.p);
-                    launch_set_extract(s, (uint64_t *)set4.p, cap4, (uint64_t *)u4.p,
-                                       (uint64_t *)run.cnt_dev.p + 1);
-                    run.counts2(m0, m1);
+                    m0 = run.read_u64(run.cnt_dev.p);
                     layer = run.sort_keys(u2, m0, d1_or | a[0], d1_and & a[1]);
-                    // the spawn-4 list only goes into the next step's set (or
-                    // its sort + unique), so it stays in set order
-                    next = std::move(u4);
+                    std::swap(set2, set4);  // the emptied set takes the next step's spawn-4 boards
+                    d1.release();           // the list (if any) is in the set now
+                    n1 = 0;
+                    carry = true;
+                    carry_cap = cap4;
                     or4 = a[2];
                     and4 = a[3];
                     res.candidates += a[4] + a[5];
                     ++res.hash_steps;
                     done = true;
                 } else {
-                    // an insert found its probe run full: empty both sets and
-                    // run the step again through sort + unique
-                    HIPCHK(hipMemsetAsync(set2.p, 0, cap2 * 8, s));
+                    // an insert found its probe run full: layer i's set (with
+                    // the spawn-4 boards of step i-1 in it) becomes the list
+                    // d1, layer i+1's is emptied, and the step runs again
+                    // through sort + unique
+                    if (carry) {
+                        drain();
+                    } else {
+                        HIPCHK(hipMemsetAsync(set2.p, 0, cap2 * 8, s));
+                    }
                     HIPCHK(hipMemsetAsync(set4.p, 0, cap4 * 8, s));
                     ++res.set_overflows;
                 }
             }
         }
+        if (!done && carry) drain();  // sort + unique takes the list
         if (!done) {
             std::vector<DBuf> runs2, runs4;
             std::vector<uint64_t> runs2_n, runs4_n;
@@ -912,24 +940,24 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
             }
             if (!whole) ++res.chunked_steps;
         }  // sort + unique
-        if (n0) {
-            r2 = (double)m0 / (double)n0;
-            r4 = (double)m1 / (double)n0;
+        if (n0) r2 = (double)m0 / (double)n0;
+        if (!carry) {
+            d1 = std::move(next);
+            n1 = m1;
         }
+        d1_or = or4;
+        d1_and = and4;
         // validation every 32 steps (BookGeneratorAD.cpp:1383-1387)
         if (((i + (int)(ini % 64u / 2u)) % 32) == (int)((sp.limit / 2u) % 32u) + 1) {
             m0 = run.validate(layer, m0, bsum + 2u);
-            m1 = run.validate(next, m1, bsum + 4u);
+            if (carry) drain();
+            n1 = run.validate(d1, n1, bsum + 4u);
         }
         res.n[i] = m0;
         if (files) run.write_raw(prefix + std::to_string(i), layer, m0);
         keep(i, layer, m0);
         d0 = std::move(layer);
         n0 = m0;
-        d1 = std::move(next);
-        n1 = m1;
-        d1_or = or4;
-        d1_and = and4;
         progress_set((uint32_t)i, (uint32_t)(o.steps - 2));
     }
     // the look-back kernels' stall flag is sticky: one read covers every sort

@@ -119,3 +119,15 @@ def test_resume_from_raw_files(pkg):
             adv.generate_ad("442", 7, prefix)
     finally:
         shutil.rmtree(work, ignore_errors=True)
+
+
+def test_full_size_4442_64(pkg):
+    """4442@64 at full size (2.67e9 states over 79 raw layers, the largest
+    94.6e6 boards, most steps past the key-set limit): every layer's size
+    and SHA-256 equal the reference's."""
+    g = load_ad_golden("4442_64")
+    print("case advanced 4442@64 full size", flush=True)
+    with _adv().generate_ad("4442", 6, keep_layers=True) as res:
+        print(f"4442@64: {res.fwd_ms:.0f} ms, {res.hash_steps} key-set steps", flush=True)
+        assert res.hash_steps > 0 and res.hash_steps < res.steps - 2
+        _check_layers(res, g, "4442_64")
