@@ -401,7 +401,9 @@ int comb_entry(uint32_t sum, int count, uint8_t *out) {
     return 0;
 }
 
-// Per-device tables: the masked row table and the combination table.
+// Per-device tables: the masked row table and the combination table, made on
+// a device's first advanced pass and kept for the process's lifetime (as the
+// device context keeps the classic row table).
 struct AdDevice {
     uint16_t *lut = nullptr;
     uint8_t *comb = nullptr;
