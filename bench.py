@@ -413,7 +413,8 @@ def main():
                  "C4_444_2048_in_hbm": build_leg(pkg, "444", 11),
                  "C4_444_2048_host_tables": build_leg(pkg, "444", 11, host_tables=True),
                  "advanced_442_256": advanced_leg(pkg, "442", 8),
-                 "advanced_LL_64": advanced_leg(pkg, "LL", 6)}
+                 "advanced_LL_64": advanced_leg(pkg, "LL", 6),
+                 "advanced_4442_64": advanced_leg(pkg, "4442", 6, reps=2)}
 
     cpu = None
     hostinfo = host_cpu()
