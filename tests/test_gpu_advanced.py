@@ -131,3 +131,20 @@ def test_full_size_4442_64(pkg):
         print(f"4442@64: {res.fwd_ms:.0f} ms, {res.hash_steps} key-set steps", flush=True)
         assert res.hash_steps > 0 and res.hash_steps < res.steps - 2
         _check_layers(res, g, "4442_64")
+
+
+def test_edge_cases(pkg, tmp_path):
+    """No seeds: every layer is empty. steps = 2: the reference's loop runs
+    no step, so no raw file is written (handle_restart_ad writes layer 0 only
+    when step 1 runs)."""
+    adv = _adv()
+    spec = adv.ad_spec("442", 7)
+    print("case advanced edge: no seeds", flush=True)
+    with adv.run_pattern_generate_ad([], spec, 20, keep_layers=True) as res:
+        assert res.sizes.size == 19 and not res.sizes.any()
+        assert res.layer(5).size == 0
+    print("case advanced edge: steps = 2", flush=True)
+    prefix = str(tmp_path / "x_")
+    with adv.run_pattern_generate_ad(spec["seeds"], spec, 2, prefix) as res:
+        assert res.steps == 2 and not res.sizes.any()
+    assert not os.path.exists(prefix + "0")
