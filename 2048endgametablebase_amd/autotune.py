@@ -7,7 +7,9 @@ by timing on the GPU:
   schedule its tile size (4, 8 or 16 keys per thread), timing sort + unique
   of real layer keys;
 * the lookup kernel's threads per block (64 to 1024), timing batched
-  lookups of a real table's states.
+  lookups of a real table's states;
+* the sizes of the batched mixes' jobs (peak raw layer, total states), by
+  which the batch planner groups jobs (tuning/job_sizes.json).
 
     python -m 2048endgametablebase_amd.autotune
 
@@ -122,6 +124,30 @@ def tune_lookup(pattern: str = "442", target: int = 8, buckets=LOOKUP_BUCKETS, r
     return rows, timings_all
 
 
+def tune_job_sizes(jobs=None, log=print):
+    """Peak raw layer, total states, layers and best-of-two plain-build time
+    of each job (the mixes of batched.py by default): the table the batch
+    planner groups jobs by."""
+    from . import batched
+
+    jobs = sorted(set(jobs if jobs is not None else batched.DEFAULT_JOBS + batched.SMALL_JOBS + batched.TINY_JOBS))
+    build_in_memory("442", 7).close()  # warm-up
+    out = {}
+    for p, t in jobs:
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            r = build_in_memory(p, t)
+            dt = time.perf_counter() - t0
+            info = [r.layer_info(i)[0] for i in range(r.steps)]
+            r.close()
+            best = dt if best is None else min(best, dt)
+        out[f"{p}@{2 ** t}"] = {"peak": int(max(info)), "states": int(sum(info)), "layers": len(info),
+                                "plain_ms": round(best * 1000.0, 2)}
+        log(f"{p}@{2 ** t}: {out[f'{p}@{2 ** t}']}")
+    return out
+
+
 def main():
     rows, tiles, table = tune()
     lrows, ltable = tune_lookup()
@@ -134,6 +160,13 @@ def main():
     native.install_sort_tuning(rows)
     native.install_tile_tuning(0, tiles)
     native.install_tile_tuning(1, lrows)
+    from . import batched
+
+    with open(batched.JOB_SIZES_FILE, "w") as fh:
+        json.dump({"device": "gfx950", "created": time.strftime("%Y-%m-%d"),
+                   "note": "peak raw layer, total raw states, layers and plain-build ms of catalogue jobs (the "
+                           "sizes are properties of the pattern and target; the times are this GPU's)",
+                   "jobs": tune_job_sizes()}, fh, indent=1)
     print(json.dumps({"rows": rows, "tiles": {"radix_items": tiles, "lookup_threads": lrows}}))
 
 

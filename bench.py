@@ -396,6 +396,8 @@ def main():
         bm.build_batch(bm.SMALL_JOBS).close()  # warm-up
         batched = {"workload_24_jobs": [f"{p}@{2**t}" for p, t in bm.DEFAULT_JOBS],
                    "uneven_24_jobs": bm.compare_with_plain(bm.DEFAULT_JOBS, reps=2),
+                   "uneven_24_jobs_planned": bm.compare_with_plain(bm.DEFAULT_JOBS, reps=2, plan="auto"),
+                   "small_24_jobs_planned": bm.compare_with_plain(bm.SMALL_JOBS, reps=3, plan="auto"),
                    "small_24_jobs": bm.compare_with_plain(bm.SMALL_JOBS, reps=3),
                    "tiny_24_jobs": bm.compare_with_plain(bm.TINY_JOBS, reps=3)}
 

@@ -108,3 +108,37 @@ def test_default_mix_matches_goldens_and_plain(pkg):
             finally:
                 res.close()
         assert checked >= 8
+
+
+def test_planned_batch_matches_goldens_and_plain(pkg):
+    """plan_batch merges small jobs into groups whose peak layers sum to at
+    most MAX_GROUP_PEAK keys and builds larger ones alone; every job of the
+    planned mix equals its golden or its plain build."""
+    from importlib import import_module
+
+    from conftest import golden_tags
+
+    batched = import_module("2048endgametablebase_amd.batched")
+    jobs = batched.SMALL_JOBS[:12] + [("4442", 4), ("L3", 7), ("442", 7), ("L3", 3), ("442", 3)]
+    groups, solo = batched.plan_batch(jobs)
+    assert groups and solo and sorted(sum(groups, []) + solo) == list(range(len(jobs)))
+    assert jobs.index(("4442", 4)) in solo  # its 4.7e7-state layers would overflow a merged set
+    have = set(golden_tags())
+    print(f"case planned batch: {len(groups)} groups, {len(solo)} alone", flush=True)
+    with batched.build_planned(jobs) as br:
+        for j, (pat, t) in enumerate(jobs):
+            tag = f"{pat}_{2 ** t}"
+            if tag in have:
+                for layer in load_golden(tag)["layers"]:
+                    keys, rates = br.book(j, layer["i"])
+                    assert book_sha(keys, rates) == layer["sha256"], (tag, layer["i"])
+                continue
+            res = pkg.build_in_memory(pat, t)
+            try:
+                for i in (1, res.steps // 2, res.steps - 3):
+                    pk, pr = res.book(i)
+                    bk, brr = br.book(j, i)
+                    assert np.array_equal(pk, bk) and np.array_equal(pr.view(np.uint32), brr.view(np.uint32)), \
+                        (tag, i)
+            finally:
+                res.close()
