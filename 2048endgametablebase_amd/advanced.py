@@ -1,4 +1,5 @@
-"""The advanced (AD, large-tile masking) builder's forward pass on the GPU.
+"""The advanced (AD, large-tile masking) builder on the GPU: forward pass and
+solver.
 
 ``generate_ad(pattern, target, pathname)`` runs what the reference's
 ``formation_core.run_pattern_build_ad`` does before its solver
@@ -8,8 +9,14 @@ run_pattern_build_ad_cpp :1421-1438): raw layer files
 stored as 0xF), byte-equal to the reference's; with a pathname it resumes
 from the raw files already there, as the reference's handle_restart_ad does.  ``run_pattern_generate_ad``
 takes the reference binding's arguments (seeds, an AdvancedPatternSpec-like
-dict, steps, pathname).  The AD solver that turns these layers into books is
-not part of this engine.
+dict, steps, pathname).
+
+``build_ad(pattern, target, pathname)`` / ``run_pattern_build_ad`` run the
+whole advanced build as formation_core.run_pattern_build_ad does
+(BookGeneratorAD.cpp:1421-1438): the forward pass, then the solver
+(BookSolverAD.cpp:2086-2242), which writes each solved layer as bucket files
+``<pathname><i>b/<key>.i`` (sorted boards) and ``<key>.b`` (a row of rates
+per board, one per placement of its large tiles).
 """
 from __future__ import annotations
 
@@ -24,6 +31,8 @@ from .config import SingletonConfig
 
 class AdResult:
     """Layers of one advanced forward pass (held in HBM when kept)."""
+
+    rate_dtype = np.float32
 
     def __init__(self, handle: int):
         self._h = ctypes.c_void_p(handle)
@@ -48,6 +57,40 @@ class AdResult:
         native.check(L.egtb_ad_result_forward(self._h, ctypes.byref(hs), ctypes.byref(ov)))
         self.hash_steps = hs.value
         self.set_overflows = ov.value
+
+    # the solver's figures (egtb_ad_build)
+    def solve_info(self):
+        ms, vals, skipped = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        native.check(native.lib().egtb_ad_result_solve_info(self._h, ctypes.byref(ms), ctypes.byref(vals),
+                                                            ctypes.byref(skipped)))
+        return {"solve_ms": ms.value, "values": vals.value, "skipped_layers": skipped.value}
+
+    def book(self, layer: int, key: int):
+        """Bucket ``key`` of solved layer ``layer`` (kept with keep_books):
+        (boards, rates of shape (rows, cols))."""
+        L = native.lib()
+        rows, cols = ctypes.c_size_t(), ctypes.c_int()
+        native.check(L.egtb_ad_result_book(self._h, int(layer), int(key), None, None, 0, ctypes.byref(rows),
+                                           ctypes.byref(cols)))
+        boards = np.zeros(rows.value, dtype=np.uint64)
+        rates = np.zeros((rows.value, max(cols.value, 0)), dtype=self.rate_dtype)
+        if rows.value:
+            native.check(L.egtb_ad_result_book(self._h, int(layer), int(key), boards.ctypes.data_as(native.u64p),
+                                               rates.ctypes.data_as(ctypes.c_void_p), rows.value,
+                                               ctypes.byref(rows), ctypes.byref(cols)))
+        return boards, rates
+
+    def buckets(self, layer: int):
+        """{key: (rows, cols)} of the non-empty buckets of a kept solved layer."""
+        L = native.lib()
+        out = {}
+        for k in range(-16, 32):
+            rows, cols = ctypes.c_size_t(), ctypes.c_int()
+            native.check(L.egtb_ad_result_book(self._h, int(layer), k, None, None, 0, ctypes.byref(rows),
+                                               ctypes.byref(cols)))
+            if rows.value:
+                out[k] = (rows.value, cols.value)
+        return out
 
     def layer(self, i: int) -> np.ndarray:
         L = native.lib()
@@ -85,6 +128,19 @@ def run_pattern_generate_ad(arr_init: Sequence[int], spec: Dict[str, object], st
     AdvancedPatternSpec (FormationRuntime.h:88-96): pattern_masks,
     symm_mode, num_free_32k, fixed_32k_shifts, small_tile_sum_limit and
     target (log2)."""
+    seeds, keep, pat, opt = _ad_args(arr_init, spec, steps, pathname, keep_layers, device, radix_bits,
+                                     sort_schedule, chunk_mkeys, forward_dedup)
+    h = ctypes.c_void_p()
+    native.check(native.lib().egtb_ad_generate(seeds.ctypes.data_as(native.u64p), seeds.size, ctypes.byref(pat),
+                                               ctypes.byref(opt), ctypes.byref(h)))
+    return AdResult(h.value)
+
+
+RATE_DTYPES = {"float32": np.float32, "float64": np.float64, "uint32": np.uint32, "uint64": np.uint64}
+
+
+def _ad_args(arr_init, spec, steps, pathname, keep_layers, device, radix_bits, sort_schedule, chunk_mkeys,
+             forward_dedup):
     seeds = np.ascontiguousarray(np.asarray(arr_init, dtype=np.uint64))
     masks = np.ascontiguousarray(np.asarray(list(spec["pattern_masks"]) or [0], dtype=np.uint64))
     fixed = np.ascontiguousarray(np.asarray(list(spec["fixed_32k_shifts"]) or [0], dtype=np.uint8))
@@ -106,10 +162,45 @@ def run_pattern_generate_ad(arr_init: Sequence[int], spec: Dict[str, object], st
     opt.sort_schedule = int(sort_schedule)
     opt.chunk_mkeys = int(chunk_mkeys)
     opt.forward_dedup = int(forward_dedup)
+    return seeds, (masks, fixed), pat, opt
+
+
+def run_pattern_build_ad(arr_init: Sequence[int], spec: Dict[str, object], steps: int,
+                         pathname: Optional[str] = None, *, success_rate_dtype: str = "float32",
+                         spawn_rate4: float = 0.1, deletion_threshold: float = 0.0, keep_books: bool = False,
+                         device: int = 0, radix_bits: int = 0, sort_schedule: int = 0, chunk_mkeys: int = 0,
+                         forward_dedup: int = 0) -> AdResult:
+    """formation_core.run_pattern_build_ad (bindings_formation.cpp:231-235):
+    the forward pass and the solver for ``spec`` (AdvancedPatternSpec's
+    fields, as run_pattern_generate_ad takes them).  With a pathname the
+    solved layers are written as the reference writes them and a rerun
+    resumes; keep_books holds every solved layer in HBM (AdResult.book)."""
+    if success_rate_dtype not in RATE_DTYPES:
+        raise ValueError(f"success_rate_dtype must be one of {sorted(RATE_DTYPES)}")
+    seeds, keep, pat, opt = _ad_args(arr_init, spec, steps, pathname, False, device, radix_bits, sort_schedule,
+                                     chunk_mkeys, forward_dedup)
+    so = native.AdSolveOptions()
+    so.success_rate_dtype = success_rate_dtype.encode()
+    so.spawn_rate4 = float(spawn_rate4)
+    so.deletion_threshold = float(deletion_threshold)
+    so.keep_books = int(bool(keep_books))
     h = ctypes.c_void_p()
-    native.check(native.lib().egtb_ad_generate(seeds.ctypes.data_as(native.u64p), seeds.size, ctypes.byref(pat),
-                                               ctypes.byref(opt), ctypes.byref(h)))
-    return AdResult(h.value)
+    native.check(native.lib().egtb_ad_build(seeds.ctypes.data_as(native.u64p), seeds.size, ctypes.byref(pat),
+                                            ctypes.byref(opt), ctypes.byref(so), ctypes.byref(h)))
+    r = AdResult(h.value)
+    r.rate_dtype = RATE_DTYPES[success_rate_dtype]
+    return r
+
+
+def build_ad(pattern: str, target: int, pathname: Optional[str] = None, **kw) -> AdResult:
+    """The whole advanced build of ``pattern`` at 2**target (start_build's
+    layer 0, as generate_ad), solved books as bucket files under
+    ``pathname`` and/or kept in HBM (keep_books=True)."""
+    from .BookBuilder import resolve_seeds
+
+    s = ad_spec(pattern, target)
+    seeds = resolve_seeds(pattern, patterns.build_params(pattern, target), int(kw.get("device", 0)))
+    return run_pattern_build_ad(seeds, s, s["steps"], pathname, **kw)
 
 
 def ad_spec(pattern: str, target: int) -> Dict[str, object]:

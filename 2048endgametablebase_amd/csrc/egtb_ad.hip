@@ -44,44 +44,11 @@
 #include <string>
 #include <vector>
 
-#include "egtb.h"
-#include "egtb_device.h"
+#include "egtb_ad.h"
 #include "egtb_kernels.h"
-#include "egtb_runtime.h"
 
 namespace egtb {
 namespace {
-
-constexpr int kCombStride = 11;    // length, then up to 10 tiles (log2, ascending)
-constexpr int kCombEntries = 2560; // index (sum >> 6) + (count << 8), count < 10
-constexpr int kAdThreads = 1024;
-
-// The masked pattern (AdvancedPatternSpec + AdvancedMaskParam,
-// FormationRuntime.h:88-104) by value in every launch.
-struct AdSpecDev {
-    uint64_t masks[kMaxMasks];
-    uint64_t fixed_mask;  // 0xF at every pinned 32768 cell
-    uint32_t n_masks;
-    uint32_t symm;
-    int32_t n_free, n_fixed;
-    uint32_t limit;  // small_tile_sum_limit
-    int32_t target;  // log2 of the target tile
-};
-
-__device__ __forceinline__ bool ad_is_pattern(uint64_t b, const AdSpecDev &s) {
-    if (s.n_masks == 0) return true;
-    bool ok = false;
-    for (uint32_t i = 0; i < s.n_masks; ++i) ok |= (b & s.masks[i]) == s.masks[i];
-    return ok;
-}
-
-// Number of masked (0xF) cells.  A masked tile never merges and a merge
-// reaching 64 makes one, so a move made a new masked tile exactly when the
-// moved board holds more of them -- the flag BoardMoverAD's row bitmap
-// gives (BoardMoverAD.cpp:100-117).
-__device__ __forceinline__ int f_count(uint64_t b) {
-    return __popcll(b & (b >> 1) & (b >> 2) & (b >> 3) & 0x1111111111111111ull);
-}
 
 // Masked row slide toward the high nibble (merge_line,
 // BoardMoverAD.cpp:47-86): equal neighbours merge once from the move side,
@@ -401,14 +368,11 @@ int comb_entry(uint32_t sum, int count, uint8_t *out) {
     return 0;
 }
 
+}  // namespace
+
 // Per-device tables: the masked row table and the combination table, made on
 // a device's first advanced pass and kept for the process's lifetime (as the
 // device context keeps the classic row table).
-struct AdDevice {
-    uint16_t *lut = nullptr;
-    uint8_t *comb = nullptr;
-};
-
 AdDevice &ad_device(int dev) {
     static std::mutex mu;
     static std::vector<std::unique_ptr<AdDevice>> devs;
@@ -439,21 +403,6 @@ uint32_t grid_for(uint64_t n, uint32_t threads, uint32_t cap) {
     const uint64_t b = (n + threads - 1) / threads;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
 }
-
-}  // namespace
-
-struct AdResult {
-    int steps = 0;
-    int dev = 0;
-    std::vector<uint64_t> n;     // raw layer sizes, layers 0 .. steps-2
-    std::vector<DBuf> layers;    // kept layers (device)
-    double fwd_ms = 0;
-    uint64_t candidates = 0;     // boards emitted over every step
-    uint64_t chunked_steps = 0, chunks = 0;
-    uint64_t skipped_steps = 0;  // steps found on disk (resume)
-    uint64_t hash_steps = 0;     // steps deduplicated in key sets
-    uint64_t set_overflows = 0;  // key-set steps redone by sort + unique
-};
 
 namespace {
 
@@ -582,6 +531,8 @@ struct AdRun {
     }
 };
 
+}  // namespace
+
 bool file_exists(const std::string &p) {
     struct stat st;
     return ::stat(p.c_str(), &st) == 0;
@@ -592,8 +543,6 @@ uint64_t file_size(const std::string &p) {
     return ::stat(p.c_str(), &st) == 0 ? (uint64_t)st.st_size : 0;
 }
 
-// read_binary_vector_direct (FileIOUtils.h:190-215): a missing file reads as
-// empty, a size that is not whole boards is an error
 std::vector<uint64_t> read_raw(const std::string &p) {
     FILE *f = std::fopen(p.c_str(), "rb");
     if (!f) return {};
@@ -627,10 +576,7 @@ uint64_t mask_board(uint64_t b) {
     return r;
 }
 
-// generate_process_ad (BookGeneratorAD.cpp:1256-1396) for a fresh build.
-void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &pat, const egtb_ad_options &o,
-                 AdResult &res) {
-    if (!seeds && n_seeds) throw Error(EGTB_ERR_ARG, "seeds is NULL");
+AdSpecDev make_ad_spec(const egtb_ad_pattern &pat) {
     if (pat.n_masks > (uint32_t)kMaxMasks) throw Error(EGTB_ERR_ARG, "at most 16 masks");
     if (pat.n_masks && !pat.masks) throw Error(EGTB_ERR_ARG, "masks is NULL");
     if (pat.symm_mode < 0 || pat.symm_mode > 3)
@@ -639,13 +585,6 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
         throw Error(EGTB_ERR_ARG, "fixed_32k_shifts: at most 16 shifts");
     if (pat.n_free_32k < 0 || pat.n_free_32k > 16) throw Error(EGTB_ERR_ARG, "n_free_32k must be 0-16");
     if (pat.target < 1 || pat.target > 15) throw Error(EGTB_ERR_ARG, "target must be log2 of a tile (1-15)");
-    if (o.steps < 2) throw Error(EGTB_ERR_ARG, "steps must be at least 2");
-    if (o.radix_bits != 0 && o.radix_bits != 4 && o.radix_bits != 6 && o.radix_bits != 8 && o.radix_bits != 11)
-        throw Error(EGTB_ERR_ARG, "radix_bits must be 0, 4, 6, 8 or 11");
-    if (o.sort_schedule < 0 || o.sort_schedule > 3) throw Error(EGTB_ERR_ARG, "sort_schedule must be 0-3");
-    if (o.chunk_mkeys < 0) throw Error(EGTB_ERR_ARG, "chunk_mkeys must be >= 0");
-    if (o.forward_dedup < 0 || o.forward_dedup > 2) throw Error(EGTB_ERR_ARG, "forward_dedup must be 0, 1 or 2");
-
     AdSpecDev sp{};
     for (uint32_t i = 0; i < pat.n_masks; ++i) sp.masks[i] = pat.masks[i];
     sp.n_masks = pat.n_masks;
@@ -659,6 +598,20 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
     sp.n_free = pat.n_free_32k;
     sp.limit = pat.small_tile_sum_limit;
     sp.target = pat.target;
+    return sp;
+}
+
+// generate_process_ad (BookGeneratorAD.cpp:1256-1396) for a fresh build.
+void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &pat, const egtb_ad_options &o,
+                 AdResult &res) {
+    if (!seeds && n_seeds) throw Error(EGTB_ERR_ARG, "seeds is NULL");
+    const AdSpecDev sp = make_ad_spec(pat);
+    if (o.steps < 2) throw Error(EGTB_ERR_ARG, "steps must be at least 2");
+    if (o.radix_bits != 0 && o.radix_bits != 4 && o.radix_bits != 6 && o.radix_bits != 8 && o.radix_bits != 11)
+        throw Error(EGTB_ERR_ARG, "radix_bits must be 0, 4, 6, 8 or 11");
+    if (o.sort_schedule < 0 || o.sort_schedule > 3) throw Error(EGTB_ERR_ARG, "sort_schedule must be 0-3");
+    if (o.chunk_mkeys < 0) throw Error(EGTB_ERR_ARG, "chunk_mkeys must be >= 0");
+    if (o.forward_dedup < 0 || o.forward_dedup > 2) throw Error(EGTB_ERR_ARG, "forward_dedup must be 0, 1 or 2");
 
     const int dev = o.device;
     DeviceCtx &C = device_ctx(dev);
@@ -962,31 +915,13 @@ void ad_generate(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &p
         n0 = m0;
         progress_set((uint32_t)i, (uint32_t)(o.steps - 2));
     }
+    res.started = started;
     // the look-back kernels' stall flag is sticky: one read covers every sort
     if (run.w.read_error(s)) throw Error(EGTB_ERR_INTERNAL, "a look-back sort stalled");
     HIPCHK(hipStreamSynchronize(s));
     res.fwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-template <class F>
-int guarded_ad(F &&f) {
-    try {
-        g_last_error.clear();
-        f();
-        return EGTB_OK;
-    } catch (const Error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_last_error = "host out of memory";
-        return EGTB_ERR_MEMORY;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return EGTB_ERR_INTERNAL;
-    }
-}
-
-}  // namespace
 }  // namespace egtb
 
 struct egtb_ad_result : egtb::AdResult {};

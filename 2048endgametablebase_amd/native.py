@@ -145,6 +145,16 @@ class AdOptions(ctypes.Structure):
     ]
 
 
+class AdSolveOptions(ctypes.Structure):
+    _fields_ = [
+        ("success_rate_dtype", ctypes.c_char_p),
+        ("spawn_rate4", ctypes.c_double),
+        ("deletion_threshold", ctypes.c_double),
+        ("keep_books", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "egtb_build": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(Pattern), ctypes.POINTER(Options)]),
@@ -171,6 +181,12 @@ SIGNATURES = {
                                             ctypes.POINTER(ctypes.c_size_t)]),
     "egtb_ad_result_forward": (ctypes.c_int, [ctypes.c_void_p, u64p, u64p]),
     "egtb_ad_result_free": (None, [ctypes.c_void_p]),
+    "egtb_ad_build": (ctypes.c_int, [u64p, ctypes.c_size_t, ctypes.POINTER(AdPattern), ctypes.POINTER(AdOptions),
+                                     ctypes.POINTER(AdSolveOptions), ctypes.POINTER(ctypes.c_void_p)]),
+    "egtb_ad_result_solve_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), u64p, u64p]),
+    "egtb_ad_result_book": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, u64p, ctypes.c_void_p,
+                                           ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                           ctypes.POINTER(ctypes.c_int)]),
     "egtb_batch_build": (ctypes.c_int, [ctypes.POINTER(BatchJob), ctypes.c_int, ctypes.POINTER(BatchOptions),
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "egtb_batch_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),

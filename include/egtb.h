@@ -205,6 +205,45 @@ EGTB_EXPORT int egtb_ad_result_forward(const egtb_ad_result *r, uint64_t *hash_s
 EGTB_EXPORT int egtb_ad_result_layer(const egtb_ad_result *r, int layer, uint64_t *keys, size_t cap, size_t *n);
 EGTB_EXPORT void egtb_ad_result_free(egtb_ad_result *r);
 
+/* Solver options of an advanced build (the RunOptions fields
+ * run_pattern_solve_ad_cpp reads, reference native_core/src/BookSolverAD.cpp:
+ * 2086-2284). */
+typedef struct egtb_ad_solve_options {
+    const char *success_rate_dtype; /* "float32" (NULL), "float64", "uint32" or "uint64" */
+    double spawn_rate4;             /* probability of a 4 (reference default 0.1) */
+    double deletion_threshold;      /* > 0: layer i+2's rows with no rate above it are dropped (and rewritten) */
+    int32_t keep_books;             /* 1: keep every solved layer's buckets in HBM (egtb_ad_result_book) */
+    int32_t reserved[3];
+} egtb_ad_solve_options;
+
+/* The advanced build: the forward pass (as egtb_ad_generate, every raw layer
+ * kept in HBM) and then the solver, layers steps-3 .. 0.  Layer i's boards
+ * are split into buckets by their large tiles (expand_ad, BookSolverAD.cpp:
+ * 1243-1318); a board's row holds one rate per placement of its large tiles
+ * (derive_size_for_bucket, :291-299, columns in the order of the permutation
+ * table, BoardMaskerAD.cpp:291-358), computed from layers i+1 and i+2
+ * (recalculate_ad, :1321-1505); rows with no rate above zero are dropped
+ * (remove_died_ad, :495-536).  With options.pathname it writes the reference's
+ * files: <pathname><i>b/<key>.i (sorted boards) and <key>.b (rates) per
+ * bucket (dict_tofile, :320-352), stats.txt lines, and removes each raw
+ * layer once solved; a layer whose <i>b folder exists is skipped and the
+ * layers it needs are read back (recalculate_process_ad_impl, :2086-2242).
+ * Replaces run_pattern_build_ad_cpp (BookGeneratorAD.cpp:1421-1438, bound as
+ * formation_core.run_pattern_build_ad, bindings_formation.cpp:231-235).
+ * out may be NULL. */
+EGTB_EXPORT int egtb_ad_build(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern *pattern,
+                              const egtb_ad_options *options, const egtb_ad_solve_options *solve,
+                              egtb_ad_result **out);
+/* Solver figures: wall time of the solve, rates computed (rows x columns,
+ * before dropped rows) and layers found solved on disk. */
+EGTB_EXPORT int egtb_ad_result_solve_info(const egtb_ad_result *r, double *solve_ms, uint64_t *values,
+                                          uint64_t *skipped_layers);
+/* Bucket `key` (-16 .. 31) of solved layer `layer` (needs
+ * solve.keep_books): its rows (boards, sorted) and cols; boards (rows) and
+ * rates (rows x cols of the rate type) are copied when not NULL. */
+EGTB_EXPORT int egtb_ad_result_book(const egtb_ad_result *r, int layer, int key, uint64_t *boards, void *rates,
+                                    size_t cap_rows, size_t *rows, int *cols);
+
 /* ---- batched builds: many (pattern, target) jobs through one pipeline ------ */
 
 /* One job of a batched build: the arguments of egtb_build for one table.

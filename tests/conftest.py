@@ -47,7 +47,18 @@ def golden_tags():
 
 def ad_tags():
     """Advanced (masked) forward-pass goldens: per raw layer file digests."""
-    return sorted(f[3:-5] for f in os.listdir(GOLDEN) if f.startswith("ad_") and f.endswith(".json"))
+    return sorted(f[3:-5] for f in os.listdir(GOLDEN)
+                  if f.startswith("ad_") and not f.startswith("ad_book_") and f.endswith(".json"))
+
+
+def ad_book_tags():
+    """Advanced solver goldens: per solved layer and bucket, file digests."""
+    return sorted(f[8:-5] for f in os.listdir(GOLDEN) if f.startswith("ad_book_") and f.endswith(".json"))
+
+
+def load_ad_book_golden(tag):
+    with open(os.path.join(GOLDEN, f"ad_book_{tag}.json")) as fh:
+        return json.load(fh)
 
 
 def load_ad_golden(tag):

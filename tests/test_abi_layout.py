@@ -19,6 +19,7 @@ STRUCTS = {
     "BatchOptions": "egtb_batch_options",
     "AdPattern": "egtb_ad_pattern",
     "AdOptions": "egtb_ad_options",
+    "AdSolveOptions": "egtb_ad_solve_options",
     "ServerOptions": "egtb_server_options",
     "LoadgenOptions": "egtb_loadgen_options",
     "LoadgenReport": "egtb_loadgen_report",

@@ -1,0 +1,86 @@
+"""The advanced (AD) solver on the GPU against the reference's own advanced
+builder: every solved layer's bucket files (<i>b/<key>.i, the sorted boards,
+and <key>.b, a row of rates per board) of each build in
+tests/golden/ad_book_*.json (written by tests/golden/make_ad_book_golden.py
+from the reference compiled in oracle/_ref) must have the reference's rows,
+columns and SHA-256.  The builds cover one-rate buckets (442@64, 442@128
+with spelled-out pairs), rows of two and three rates (442@256, t@128 with
+free 32768 tiles), the diagonal and horizontal forms (LL, 444) and the
+uint32 and float64 rate types.  Also: the books kept in HBM equal the
+files."""
+import hashlib
+import importlib
+import os
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import ad_book_tags, load_ad_book_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _adv():
+    return importlib.import_module("2048endgametablebase_amd.advanced")
+
+
+def _sha(path):
+    with open(path, "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()
+
+
+def _check_files(prefix, g, tag):
+    """Every solved layer of golden g against the files under prefix."""
+    for layer in g["layers"]:
+        folder = f"{prefix}{layer['i']}b"
+        assert os.path.isdir(folder), (tag, folder)
+        have = sorted(int(f[:-2]) for f in os.listdir(folder) if f.endswith(".i"))
+        want = [b["key"] for b in layer["buckets"]]
+        assert have == want, (tag, layer["i"], have, want)
+        for b in layer["buckets"]:
+            base = os.path.join(folder, str(b["key"]))
+            rows = os.path.getsize(base + ".i") // 8
+            assert rows == b["rows"], (tag, layer["i"], b["key"], rows, b["rows"])
+            assert _sha(base + ".i") == b["i_sha"], (tag, layer["i"], b["key"], "boards")
+            assert _sha(base + ".b") == b["b_sha"], (tag, layer["i"], b["key"], "rates")
+
+
+def _build(g, prefix, **kw):
+    adv = _adv()
+    spec = adv.ad_spec(g["pattern"], g["target"])
+    return adv.run_pattern_build_ad(spec["seeds"], spec, spec["steps"], prefix,
+                                    success_rate_dtype=g["dtype"], spawn_rate4=g["spawn_rate4"], **kw)
+
+
+@pytest.mark.parametrize("tag", ad_book_tags())
+def test_books_match_reference(pkg, tag):
+    g = load_ad_book_golden(tag)
+    rows = sum(b["rows"] for l in g["layers"] for b in l["buckets"])
+    print(f"case advanced solve {tag}: {len(g['layers'])} layers, {rows} rows", flush=True)
+    work = tempfile.mkdtemp(prefix="egtb_adsolve_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        prefix = os.path.join(work, f"{g['pattern']}_{2 ** g['target']}_")
+        with _build(g, prefix) as res:
+            info = res.solve_info()
+            assert info["skipped_layers"] == 0
+        _check_files(prefix, g, tag)
+        # the raw layers were removed as the reference removes them
+        assert not any(os.path.exists(f"{prefix}{i}") for i in range(g["steps"]))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def test_kept_books_equal_files(pkg):
+    g = load_ad_book_golden("442_256")
+    print("case advanced solve kept books 442_256", flush=True)
+    with _build(g, None, keep_books=True) as res:
+        for layer in g["layers"][::7]:
+            got = res.buckets(layer["i"])
+            assert sorted(got) == [b["key"] for b in layer["buckets"]], layer["i"]
+            for b in layer["buckets"]:
+                boards, rates = res.book(layer["i"], b["key"])
+                assert rates.shape == (b["rows"], b["cols"])
+                assert hashlib.sha256(boards.astype("<u8").tobytes()).hexdigest() == b["i_sha"]
+                assert hashlib.sha256(rates.astype("<f4").tobytes()).hexdigest() == b["b_sha"]
