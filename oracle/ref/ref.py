@@ -83,7 +83,7 @@ def lib():
         f.restype = ctypes.c_int
         P, Z, I = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
         f.argtypes = [P, Z, P, Z, I, I, P, Z, I, I, I, I, ctypes.c_char_p, I, ctypes.c_double, ctypes.c_char_p,
-                      I, I, ctypes.c_char_p, Z]
+                      I, I, ctypes.c_double, ctypes.c_char_p, Z]
         L.egtb_ref_ai_search.restype = None
         L.egtb_ref_ai_search.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p,
@@ -118,7 +118,7 @@ def run_build(seeds, masks, shifts, symm, target, steps, docheck, pathname, is_f
 
 def run_build_ad(seeds, masks, symm, n_free_32k, fixed_shifts, target, steps, docheck, pathname,
                  is_free=False, small_tile_sum_limit=96, p4=0.1, dtype="float32", threads=8,
-                 stop_after_forward=True) -> None:
+                 stop_after_forward=True, deletion_threshold=0.0) -> None:
     """Run the reference's advanced builder run_pattern_build_ad_cpp; with
     stop_after_forward its raw layer files <pathname>0 .. <pathname>steps-2
     are left in place and the solver does not run."""
@@ -130,7 +130,7 @@ def run_build_ad(seeds, masks, symm, n_free_32k, fixed_shifts, target, steps, do
         rc = lib().egtb_ref_build_ad(s.ctypes.data, s.size, m.ctypes.data, len(masks), symm, n_free_32k,
                                      fx.ctypes.data, len(fixed_shifts), small_tile_sum_limit, target, steps,
                                      docheck, pathname.encode(), int(is_free), p4, dtype.encode(), threads,
-                                     int(bool(stop_after_forward)), err, 1024)
+                                     int(bool(stop_after_forward)), float(deletion_threshold), err, 1024)
         if rc == 0:
             return
         msg = err.value.decode(errors="replace")

@@ -28,7 +28,8 @@ extern "C" int egtb_ref_build_ad(const uint64_t *seeds, size_t n_seeds, const ui
                                  int symm_mode, int num_free_32k, const uint8_t *fixed_shifts, size_t n_fixed,
                                  int small_tile_sum_limit, int target, int steps, int docheck_step,
                                  const char *pathname, int is_free, double spawn_rate4, const char *dtype,
-                                 int num_threads, int stop_after_forward, char *err, size_t err_len) {
+                                 int num_threads, int stop_after_forward, double deletion_threshold,
+                                 char *err, size_t err_len) {
     const std::string blocker = std::string(pathname) + std::to_string(steps - 1) + "b";
     try {
         AdvancedPatternSpec spec;
@@ -49,7 +50,7 @@ extern "C" int egtb_ref_build_ad(const uint64_t *seeds, size_t n_seeds, const ui
         opt.is_variant = false;
         opt.spawn_rate4 = spawn_rate4;
         opt.success_rate_dtype = dtype;
-        opt.deletion_threshold = 0.0;
+        opt.deletion_threshold = deletion_threshold;
         opt.compress = false;
         opt.compress_temp_files = false;
         opt.optimal_branch_only = false;

@@ -41,26 +41,29 @@ patterns = importlib.import_module("2048endgametablebase_amd.patterns")
 
 NP_TYPES = {"float32": "<f4", "float64": "<f8", "uint32": "<u4", "uint64": "<u8"}
 
-# (pattern, log2 target, dtype): one-rate buckets only (442@64, 442@128 with
-# its spelled-out pairs), rows of two rates (442@256), free 32768 tiles (t,
-# rows of three), the diagonal and horizontal forms (LL, 444), nine free
-# 32768 tiles (free8), and the integer and double rate types
+# (pattern, log2 target, dtype[, deletion threshold]): one-rate buckets only
+# (442@64, 442@128 with its spelled-out pairs), rows of two rates (442@256),
+# free 32768 tiles (t, rows of three), the diagonal and horizontal forms (LL,
+# 444), nine free 32768 tiles (free8), the integer and double rate types, and
+# rows pruned below a deletion threshold (layer i+2 rewritten after layer i)
 RUNS = [("442", 6, "float32"), ("442", 7, "float32"), ("442", 8, "float32"), ("t", 6, "float32"),
         ("t", 7, "float32"), ("L3", 7, "float32"), ("LL", 6, "float32"), ("444", 6, "float32"),
-        ("442", 7, "uint32"), ("442", 7, "float64"), ("free8", 6, "float32")]
+        ("442", 7, "uint32"), ("442", 7, "float64"), ("free8", 6, "float32"), ("442", 8, "float32", 0.3),
+        ("t", 7, "uint32", 0.05)]
 
 
-def tag_of(name, t, dtype):
-    return f"ad_book_{name}_{2 ** t}" + ("" if dtype == "float32" else f"_{dtype}")
+def tag_of(name, t, dtype, thr=0.0):
+    return (f"ad_book_{name}_{2 ** t}" + ("" if dtype == "float32" else f"_{dtype}") +
+            (f"_del{int(round(thr * 100)):02d}" if thr else ""))
 
 
-def digest_run(name, t, dtype, workdir):
+def digest_run(name, t, dtype, workdir, thr=0.0):
     p = patterns.ad_params(name, t, 96)
     prefix = os.path.join(workdir, f"{name}_{2 ** t}_")
     t0 = time.time()
     ref.run_build_ad(p["seeds"], p["masks"], p["symm_mode"], p["n_free_32k"], p["fixed_32k_shifts"], t,
                      p["steps"], p["docheck_step"], prefix, is_free=p["is_free"], dtype=dtype, threads=8,
-                     stop_after_forward=False)
+                     stop_after_forward=False, deletion_threshold=thr)
     secs = time.time() - t0
     layers = []
     for i in range(p["steps"] - 2):
@@ -81,6 +84,7 @@ def digest_run(name, t, dtype, workdir):
         layers.append({"i": i, "buckets": buckets})
     return {
         "pattern": name, "target": t, "steps": p["steps"], "dtype": dtype, "spawn_rate4": 0.1,
+        "deletion_threshold": thr,
         "seeds": [f"{s:016x}" for s in p["seeds"]], "masks": [f"{m:x}" for m in p["masks"]],
         "symm": p["symm_mode"], "n_free_32k": p["n_free_32k"], "fixed_32k_shifts": p["fixed_32k_shifts"],
         "small_tile_sum_limit": 96,
@@ -93,13 +97,15 @@ def main(argv):
     if not ref.available() and not ref.build():
         raise SystemExit("reference sources not present; cannot make golden vectors")
     want = set(argv)
-    for name, t, dtype in RUNS:
-        tag = tag_of(name, t, dtype)
+    for run in RUNS:
+        name, t, dtype = run[:3]
+        thr = run[3] if len(run) > 3 else 0.0
+        tag = tag_of(name, t, dtype, thr)
         if want and tag not in want:
             continue
         work = tempfile.mkdtemp(prefix="egtb_adbook_", dir="/dev/shm")
         try:
-            out = digest_run(name, t, dtype, work)
+            out = digest_run(name, t, dtype, work, thr)
         finally:
             shutil.rmtree(work, ignore_errors=True)
         with open(os.path.join(os.path.dirname(__file__), f"{tag}.json"), "w") as fh:

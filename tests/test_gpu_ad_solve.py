@@ -5,9 +5,11 @@ tests/golden/ad_book_*.json (written by tests/golden/make_ad_book_golden.py
 from the reference compiled in oracle/_ref) must have the reference's rows,
 columns and SHA-256.  The builds cover one-rate buckets (442@64, 442@128
 with spelled-out pairs), rows of two and three rates (442@256, t@128 with
-free 32768 tiles), the diagonal and horizontal forms (LL, 444) and the
-uint32 and float64 rate types.  Also: the books kept in HBM equal the
-files."""
+free 32768 tiles), the diagonal and horizontal forms (LL, 444), the
+uint32 and float64 rate types and two deletion thresholds (layer i+2's rows
+pruned and rewritten after layer i).  Also: the books kept in HBM equal the
+files, and a rerun over partly solved folders resumes and finishes them as
+the reference's recalculate_process_ad_impl does."""
 import hashlib
 import importlib
 import os
@@ -51,7 +53,8 @@ def _build(g, prefix, **kw):
     adv = _adv()
     spec = adv.ad_spec(g["pattern"], g["target"])
     return adv.run_pattern_build_ad(spec["seeds"], spec, spec["steps"], prefix,
-                                    success_rate_dtype=g["dtype"], spawn_rate4=g["spawn_rate4"], **kw)
+                                    success_rate_dtype=g["dtype"], spawn_rate4=g["spawn_rate4"],
+                                    deletion_threshold=g.get("deletion_threshold", 0.0), **kw)
 
 
 @pytest.mark.parametrize("tag", ad_book_tags())
@@ -84,3 +87,24 @@ def test_kept_books_equal_files(pkg):
                 assert rates.shape == (b["rows"], b["cols"])
                 assert hashlib.sha256(boards.astype("<u8").tobytes()).hexdigest() == b["i_sha"]
                 assert hashlib.sha256(rates.astype("<f4").tobytes()).hexdigest() == b["b_sha"]
+
+
+def test_solve_resumes_from_folders(pkg):
+    """Solved layers 0..39 removed after a full build: the rerun regenerates
+    their raw layers, skips the layers still solved on disk, reads layers 40
+    and 41 back and solves 39..0 to the reference's bytes."""
+    g = load_ad_book_golden("442_128")
+    print("case advanced solve resume 442_128 from layer 40", flush=True)
+    work = tempfile.mkdtemp(prefix="egtb_adsolve_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        prefix = os.path.join(work, "442_128_")
+        with _build(g, prefix):
+            pass
+        for i in range(40):
+            shutil.rmtree(f"{prefix}{i}b")
+        with _build(g, prefix) as res:
+            info = res.solve_info()
+        assert info["skipped_layers"] == g["steps"] - 2 - 40
+        _check_files(prefix, g, "442_128 resumed")
+    finally:
+        shutil.rmtree(work, ignore_errors=True)

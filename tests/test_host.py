@@ -127,6 +127,38 @@ def test_advanced_forward_arguments_fail_before_device_use(pkg):
     assert p["n_free_32k"] == 2 and p["fixed_32k_shifts"] == [0, 4, 16, 20]
 
 
+def test_advanced_build_arguments_fail_before_device_use(pkg):
+    """The advanced build (forward pass + solver) checks its solver options
+    on the host before the forward pass starts."""
+    import importlib
+
+    adv = importlib.import_module("2048endgametablebase_amd.advanced")
+    s = adv.ad_spec("442", 7)
+    with pytest.raises(ValueError, match="success_rate_dtype"):
+        adv.run_pattern_build_ad(s["seeds"], s, s["steps"], success_rate_dtype="float16")
+    with pytest.raises(pkg.native.EgtbError, match="spawn_rate4"):
+        adv.run_pattern_build_ad(s["seeds"], s, s["steps"], spawn_rate4=1.5)
+    with pytest.raises(pkg.native.EgtbError, match="deletion_threshold"):
+        adv.run_pattern_build_ad(s["seeds"], s, s["steps"], deletion_threshold=-0.1)
+    with pytest.raises(pkg.native.EgtbError, match="3 steps"):
+        adv.run_pattern_build_ad(s["seeds"], s, 2)
+    with pytest.raises(pkg.native.EgtbError, match="symm_mode"):
+        adv.run_pattern_build_ad(s["seeds"], {**s, "symm_mode": 7}, s["steps"])
+    # the C entry point names an unknown rate type itself
+    L = pkg.native.lib()
+    seeds, keep, pat, opt = adv._ad_args(s["seeds"], s, s["steps"], None, False, 0, 0, 0, 0, 0)
+    so = pkg.native.AdSolveOptions()
+    so.success_rate_dtype = b"1-float32"
+    so.spawn_rate4 = 0.1
+    h = ctypes.c_void_p()
+    assert L.egtb_ad_build(seeds.ctypes.data_as(pkg.native.u64p), seeds.size, ctypes.byref(pat), ctypes.byref(opt),
+                           ctypes.byref(so), ctypes.byref(h)) != 0
+    assert "success_rate_dtype" in pkg.native.last_error()
+    assert L.egtb_ad_build(seeds.ctypes.data_as(pkg.native.u64p), seeds.size, ctypes.byref(pat), ctypes.byref(opt),
+                           None, ctypes.byref(h)) != 0
+    assert L.egtb_ad_result_book(None, 0, 0, None, None, 0, None, None) != 0
+
+
 def test_tile_tuning_checks_arguments(pkg):
     """Tile tunings are validated on the host (no GPU touched)."""
     L = pkg.native.lib()
