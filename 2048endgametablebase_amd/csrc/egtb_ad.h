@@ -65,9 +65,18 @@ struct AdBucket {
     DBuf ind, rates;
     uint64_t rows = 0;
     uint32_t cols = 0;
+    // range index over the bits below the boards' shared prefix: the boards
+    // with window value v are ind[idx[v] .. idx[v+1]) (idx empty: none)
+    DBuf idx;
+    uint64_t pre_mask = 0, pre_val = 0;
+    uint32_t shift = 0, bits = 0;
 };
 struct AdBookLayer {
     AdBucket b[kAdKeys];
+    // the solver's device table of the 48 buckets (and its host copy, which
+    // outlives the copy to the device)
+    DBuf desc;
+    std::vector<uint8_t> desc_host;
 };
 
 struct AdResult {

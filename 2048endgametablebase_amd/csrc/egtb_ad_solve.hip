@@ -75,6 +75,9 @@ struct AdBucketDev {
     const T *rates;
     uint64_t rows;
     uint32_t cols, pad;
+    const uint32_t *idx;  // range index (nullptr: plain binary search)
+    uint64_t pre_mask, pre_val;
+    uint32_t shift, bits;
 };
 
 template <typename T>
@@ -276,12 +279,21 @@ __device__ __forceinline__ void rank_of(const uint64_t *D, int nd, uint32_t *R) 
 }
 
 // search_arr_ad (:466-478): the row of `board` in bucket `key`, or nullptr.
+// A bucket's boards share the bits above its index window; the window's
+// value selects the range to search (as the reference's AdaptiveIndex
+// narrows the search of buckets over 100k boards, :413-430).
 template <typename T>
 __device__ const T *ad_find(const AdBucketDev<T> *L, int key, uint64_t board, uint32_t &cols) {
     if (key < kAdKeyMin || key > kAdKeyMax) return nullptr;
     const AdBucketDev<T> &B = L[key - kAdKeyMin];
     if (B.rows == 0 || B.cols == 0) return nullptr;
     uint64_t lo = 0, hi = B.rows;
+    if (B.idx) {
+        if ((board & B.pre_mask) != B.pre_val) return nullptr;
+        const uint32_t v = (uint32_t)(board >> B.shift) & ((1u << B.bits) - 1u);
+        lo = B.idx[v];
+        hi = B.idx[v + 1];
+    }
     while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         if (B.ind[mid] < board) lo = mid + 1;
@@ -574,9 +586,16 @@ template <typename T, int DM>
 __global__ __launch_bounds__(256) void k_ad_solve(const uint64_t *__restrict__ ind, uint64_t rows, uint32_t cols,
                                                   int key, T *__restrict__ out, const AdBucketDev<T> *L1,
                                                   const AdBucketDev<T> *L2, uint32_t bsum, AdSolveCtx<T> c,
-                                                  double w2, double w4, unsigned *errp) {
+                                                  double w2, double w4, unsigned *errp,
+                                                  unsigned long long *maxbits) {
     const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+    unsigned long long mx = 0;  // bits of the row's largest rate (rates are >= 0)
+    auto note = [&](T v) {
+        unsigned long long x = 0;
+        __builtin_memcpy(&x, &v, sizeof(T));
+        mx = x > mx ? x : mx;
+    };
+    if (r < rows) {
     const uint64_t b = ind[r];
     unsigned err = 0;
     const uint32_t em = empty_cells(b);
@@ -590,7 +609,9 @@ __global__ __launch_bounds__(256) void k_ad_solve(const uint64_t *__restrict__ i
             const T s4 = sosr_one(b, 2, pos, bsum + 4u, key, L2, c, err);
             sp = __dadd_rn(sp, __fma_rn((double)s2, w2, __dmul_rn((double)s4, w4)));
         }
-        out[r] = empty == 0 ? c.zero : (T)__ddiv_rn(sp, (double)empty);
+        const T v = empty == 0 ? c.zero : (T)__ddiv_rn(sp, (double)empty);
+        out[r] = v;
+        note(v);
     } else {
         const int d = (int)cols;
         double sp[DM];
@@ -609,9 +630,40 @@ __global__ __launch_bounds__(256) void k_ad_solve(const uint64_t *__restrict__ i
         }
         T *o = out + r * cols;
         const double inv = empty == 0 ? 0.0 : __ddiv_rn(1.0, (double)empty);
-        for (int j = 0; j < d; ++j) o[j] = empty == 0 ? c.zero : (T)__dmul_rn(sp[j], inv);
+        for (int j = 0; j < d; ++j) {
+            const T v = empty == 0 ? c.zero : (T)__dmul_rn(sp[j], inv);
+            o[j] = v;
+            note(v);
+        }
     }
     if (err) atomicOr(errp, err);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0 && mx) atomicMax(maxbits, mx);
+}
+
+// the range index of a sorted bucket: idx[v] = first board whose window
+// value is >= v (boards all share pre_val above the window), idx[2^bits] = n
+__global__ void k_ad_range_index(const uint64_t *__restrict__ keys, uint64_t n, uint64_t pre_val, uint32_t shift,
+                                 uint32_t bits, uint32_t *__restrict__ idx) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nb = 1ull << bits;
+    if (v > nb) return;
+    if (v == nb) {
+        idx[v] = (uint32_t)n;
+        return;
+    }
+    const uint64_t target = pre_val | (v << shift);
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    idx[v] = (uint32_t)lo;
 }
 
 // bucket key of every board of a raw layer (int8; -128 = dropped) and the
@@ -665,25 +717,6 @@ __global__ void k_ad_scatter_rows(const uint64_t *__restrict__ ind, const T *__r
     out_ind[o] = ind[r];
     if (rates)
         for (uint32_t j = 0; j < cols; ++j) out_rates[o * cols + j] = rates[r * cols + j];
-}
-
-// the largest rate's bits (rates are >= 0, so the bit patterns order as the
-// values do)
-template <typename T>
-__global__ void k_ad_max_bits(const T *__restrict__ v, uint64_t n, unsigned long long *__restrict__ out) {
-    unsigned long long m = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        unsigned long long x = 0;
-        const T t = v[i];
-        __builtin_memcpy(&x, &t, sizeof(T));
-        m = x > m ? x : m;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(m, off, 64);
-        m = o > m ? o : m;
-    }
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
 }
 
 template <typename T>
@@ -842,49 +875,59 @@ struct AdSolver {
         return v;
     }
 
-    // the device table of a layer's 48 bucket descriptors
-    void describe(const AdBookLayer &L, DBuf &out) {
-        AdBucketDev<T> h[kAdKeys];
+    // the device table of a finished layer's 48 bucket descriptors, kept
+    // with the layer (its host copy too, so the copy needs no wait)
+    void describe(AdBookLayer &L) {
+        L.desc_host.resize(sizeof(AdBucketDev<T>) * kAdKeys);
+        auto *h = (AdBucketDev<T> *)L.desc_host.data();
         for (int k = 0; k < kAdKeys; ++k) {
             const AdBucket &B = L.b[k];
-            h[k] = AdBucketDev<T>{(const uint64_t *)B.ind.p, (const T *)B.rates.p, B.rows, B.cols, 0};
+            h[k] = AdBucketDev<T>{(const uint64_t *)B.ind.p, (const T *)B.rates.p, B.rows, B.cols, 0,
+                                  (const uint32_t *)B.idx.p, B.pre_mask, B.pre_val, B.shift, B.bits};
         }
-        out.ensure(sizeof h, s);
-        HIPCHK(hipMemcpyAsync(out.p, h, sizeof h, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));  // h leaves scope
+        L.desc.ensure(L.desc_host.size(), s);
+        HIPCHK(hipMemcpyAsync(L.desc.p, L.desc_host.data(), L.desc_host.size(), hipMemcpyHostToDevice, s));
     }
 
-    // rows flagged in flags[0..rows) of bucket B, compacted in place
-    void compact(AdBucket &B) {
-        const uint64_t rows = B.rows;
-        if (rows == 0) return;
-        offs.ensure(rows * 4 + 4, s);
-        part.ensure(scan_partials_needed(rows) * 4 + 64, s);
-        launch_scan_u32(s, (const uint32_t *)flags.p, rows, (uint32_t *)offs.p, (uint32_t *)part.p);
-        DBuf ni, nr;
-        ni.alloc(rows * 8 + 8, s);
-        if (B.cols) nr.alloc(rows * B.cols * sizeof(T) + 8, s);
-        acc.ensure(64, s);
-        hipLaunchKernelGGL(k_ad_scatter_rows<T>, dim3(grid_for(rows, 256, 1u << 30)), dim3(256), 0, s,
-                           (const uint64_t *)B.ind.p, (const T *)B.rates.p, rows, B.cols, (const uint32_t *)flags.p,
-                           (const uint32_t *)offs.p, (uint64_t *)ni.p, (T *)nr.p, (unsigned long long *)acc.p);
-        HIPCHK(hipGetLastError());
-        B.rows = read_u64(acc.p);
-        B.ind = std::move(ni);
-        B.rates = std::move(nr);
-        if (B.rows == 0) B.cols = 0;
-    }
-
-    // remove_died_ad: rows with no rate above thr leave the layer
+    // remove_died_ad: rows with no rate above thr leave the layer.  Every
+    // bucket is compacted into new buffers (the flag and offset buffers are
+    // reused in stream order), its kept count into acc[k]; one read at the end.
     void prune(AdBookLayer &L, T thr) {
+        DBuf ni[kAdKeys], nr[kAdKeys];
+        bool any = false;
+        acc.ensure(kAdKeys * 8, s);
         for (int k = 0; k < kAdKeys; ++k) {
             AdBucket &B = L.b[k];
             if (B.rows == 0 || B.cols == 0) continue;
-            flags.ensure(B.rows * 4 + 4, s);
-            hipLaunchKernelGGL(k_ad_live_rows<T>, dim3(grid_for(B.rows, 256, 1u << 30)), dim3(256), 0, s,
-                               (const T *)B.rates.p, B.rows, B.cols, thr, (uint32_t *)flags.p);
+            const uint64_t rows = B.rows;
+            flags.ensure(rows * 4 + 4, s);
+            offs.ensure(rows * 4 + 4, s);
+            part.ensure(scan_partials_needed(rows) * 4 + 64, s);
+            hipLaunchKernelGGL(k_ad_live_rows<T>, dim3(grid_for(rows, 256, 1u << 30)), dim3(256), 0, s,
+                               (const T *)B.rates.p, rows, B.cols, thr, (uint32_t *)flags.p);
             HIPCHK(hipGetLastError());
-            compact(B);
+            launch_scan_u32(s, (const uint32_t *)flags.p, rows, (uint32_t *)offs.p, (uint32_t *)part.p);
+            ni[k].alloc(rows * 8 + 8, s);
+            nr[k].alloc(rows * B.cols * sizeof(T) + 8, s);
+            hipLaunchKernelGGL(k_ad_scatter_rows<T>, dim3(grid_for(rows, 256, 1u << 30)), dim3(256), 0, s,
+                               (const uint64_t *)B.ind.p, (const T *)B.rates.p, rows, B.cols,
+                               (const uint32_t *)flags.p, (const uint32_t *)offs.p, (uint64_t *)ni[k].p,
+                               (T *)nr[k].p, (unsigned long long *)acc.p + k);
+            HIPCHK(hipGetLastError());
+            any = true;
+        }
+        if (!any) return;
+        uint64_t kept[kAdKeys];
+        HIPCHK(hipMemcpyAsync(kept, acc.p, sizeof kept, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < kAdKeys; ++k) {
+            AdBucket &B = L.b[k];
+            if (!ni[k].p) continue;
+            B.rows = kept[k];
+            B.ind = std::move(ni[k]);
+            B.rates = std::move(nr[k]);
+            B.idx.release();
+            if (B.rows == 0) B.cols = 0;
         }
     }
 
@@ -932,38 +975,57 @@ struct AdSolver {
         const auto *ind = (const uint64_t *)B.ind.p;
         T *out = (T *)B.rates.p;
         unsigned *e = (unsigned *)err.p;
+        unsigned long long *mx = (unsigned long long *)err.p + 1;
         if (B.cols == 1)
             hipLaunchKernelGGL((k_ad_solve<T, 1>), g, t, 0, s, ind, B.rows, B.cols, key, out, l1, l2, bsum, ctx, w2,
-                               w4, e);
+                               w4, e, mx);
         else if (B.cols <= 8)
             hipLaunchKernelGGL((k_ad_solve<T, 8>), g, t, 0, s, ind, B.rows, B.cols, key, out, l1, l2, bsum, ctx, w2,
-                               w4, e);
+                               w4, e, mx);
         else if (B.cols <= 64)
             hipLaunchKernelGGL((k_ad_solve<T, 64>), g, t, 0, s, ind, B.rows, B.cols, key, out, l1, l2, bsum, ctx,
-                               w2, w4, e);
+                               w2, w4, e, mx);
         else
             throw Error(EGTB_ERR_ARG, "advanced solve: rows of " + std::to_string(B.cols) +
                                           " rates (bucket " + std::to_string(key) + ") exceed the 64 supported");
         HIPCHK(hipGetLastError());
     }
 
-    // length_count (:480-493): rates of the layer and the largest
-    void layer_stats(const AdBookLayer &L, uint64_t &len, T &mx) {
-        len = 0;
-        acc.ensure(64, s);
-        HIPCHK(hipMemsetAsync(acc.p, 0, 8, s));
+    // the range index of every bucket of a finished layer with more than
+    // kIndexMin boards: a window of about log2(rows) - 1 bits right below the
+    // prefix the sorted boards share (that of the first and the last)
+    void index_layer(AdBookLayer &L) {
+        constexpr uint64_t kIndexMin = 256;
+        uint64_t ends[kAdKeys][2] = {};
+        bool any = false;
         for (int k = 0; k < kAdKeys; ++k) {
             const AdBucket &B = L.b[k];
-            const uint64_t m = B.rows * B.cols;
-            len += m;
-            if (!m) continue;
-            hipLaunchKernelGGL(k_ad_max_bits<T>, dim3(grid_for(m, 256, (uint32_t)cus * 8)), dim3(256), 0, s,
-                               (const T *)B.rates.p, m, (unsigned long long *)acc.p);
+            if (B.rows < kIndexMin || B.cols == 0 || B.rows > 0xFFFFFFFFull) continue;
+            HIPCHK(hipMemcpyAsync(&ends[k][0], B.ind.p, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(&ends[k][1], (const uint64_t *)B.ind.p + (B.rows - 1), 8, hipMemcpyDeviceToHost, s));
+            any = true;
+        }
+        if (!any) return;
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < kAdKeys; ++k) {
+            AdBucket &B = L.b[k];
+            B.idx.release();
+            if (B.rows < kIndexMin || B.cols == 0 || B.rows > 0xFFFFFFFFull) continue;
+            const uint64_t diff = ends[k][0] ^ ends[k][1];
+            if (!diff) continue;
+            const int top = 63 - __builtin_clzll(diff);
+            int want = 0;
+            while ((1ull << (want + 1)) <= B.rows) ++want;  // floor(log2(rows))
+            const int bits = std::max(1, std::min({want - 1, 22, top + 1}));
+            B.bits = (uint32_t)bits;
+            B.shift = (uint32_t)(top + 1 - bits);
+            B.pre_mask = top == 63 ? 0ull : ~((2ull << top) - 1ull);
+            B.pre_val = ends[k][0] & B.pre_mask;
+            B.idx.alloc(((1ull << bits) + 1) * 4 + 8, s);
+            hipLaunchKernelGGL(k_ad_range_index, dim3(grid_for((1ull << bits) + 1, 256, 1u << 30)), dim3(256), 0, s,
+                               (const uint64_t *)B.ind.p, B.rows, B.pre_val, B.shift, B.bits, (uint32_t *)B.idx.p);
             HIPCHK(hipGetLastError());
         }
-        const uint64_t bits = read_u64(acc.p);
-        std::memcpy(&mx, &bits, sizeof(T));
-        if (mx < ctx.zero) mx = ctx.zero;
     }
 
     // dict_tofile (:320-352)
@@ -1093,7 +1155,8 @@ void ad_solve_t(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &pa
         }
         ::unlink((prefix + std::to_string(steps - 2)).c_str());
     }
-    DBuf L1, L2;  // descriptor tables of layers i+1, i+2
+    S.describe(lay[steps - 1]);
+    S.describe(lay[steps - 2]);
     bool started = false;
     for (int step = steps - 3; step >= 0; --step) {
         const std::string folder = prefix + std::to_string(step) + "b";
@@ -1106,6 +1169,10 @@ void ad_solve_t(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &pa
             if (files && (step != steps - 3 || !res.started)) {
                 S.read_layer(prefix + std::to_string(step + 1) + "b", lay[step + 1]);
                 S.read_layer(prefix + std::to_string(step + 2) + "b", lay[step + 2]);
+                S.index_layer(lay[step + 1]);
+                S.index_layer(lay[step + 2]);
+                S.describe(lay[step + 1]);
+                S.describe(lay[step + 2]);
             }
         }
         const auto ts = std::chrono::steady_clock::now();
@@ -1127,18 +1194,26 @@ void ad_solve_t(const uint64_t *seeds, size_t n_seeds, const egtb_ad_pattern &pa
         const uint32_t bsum = (uint32_t)(2 * step) + ini;
         AdBookLayer &L0 = lay[step];
         S.bucketize(raw, n, bsum, pat.n_free_32k, L0);
-        S.describe(lay[step + 1], L1);
-        S.describe(lay[step + 2], L2);
-        for (int k = 0; k < kAdKeys; ++k) S.solve_bucket(k, L0.b[k], L1, L2, bsum, w2, w4);
-        const unsigned e = (unsigned)S.read_u64(S.err.p);
+        HIPCHK(hipMemsetAsync((uint64_t *)S.err.p + 1, 0, 8, s));
+        for (int k = 0; k < kAdKeys; ++k)
+            S.solve_bucket(k, L0.b[k], lay[step + 1].desc, lay[step + 2].desc, bsum, w2, w4);
+        uint64_t ev[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(ev, S.err.p, sizeof ev, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const unsigned e = (unsigned)ev[0];
         if (e & kErrDerived) throw Error(EGTB_ERR_INTERNAL, "advanced solve: a board has more placements than its row");
         if (e & kErrColumn) throw Error(EGTB_ERR_INTERNAL, "advanced solve: a rate index outside its row");
+        // length_count (:480-493): rates of the layer and the largest
         uint64_t len = 0;
+        for (int k = 0; k < kAdKeys; ++k) len += L0.b[k].rows * L0.b[k].cols;
         T mx = zero;
-        S.layer_stats(L0, len, mx);
+        std::memcpy(&mx, &ev[1], sizeof(T));
+        if (mx < zero) mx = zero;
         res.solve_values += len;
         if (so.deletion_threshold > 0.0) S.prune(lay[step + 2], del_thr);
         S.prune(L0, zero);
+        S.index_layer(L0);  // layer `step` is searched by the next two steps
+        S.describe(L0);
         HIPCHK(hipStreamSynchronize(s));
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
         if (files) {
