@@ -159,6 +159,58 @@ def cpu_baseline_advanced(pattern, target, threads):
                        f"(g++ -Ofast, {threads} OpenMP threads, raw layer files on tmpfs): {n} states in {dt:.1f} s")
 
 
+def cpu_baseline_advanced_build(pattern, target, threads):
+    """The reference's whole advanced build (forward pass and solver,
+    run_pattern_build_ad_cpp compiled into oracle/_ref) with its files on
+    tmpfs: rates written per second; None when that library is absent."""
+    pkg = importlib.import_module("2048endgametablebase_amd")
+    from oracle.ref import ref
+
+    if not ref.available():
+        return None
+    p = pkg.patterns.ad_params(pattern, target)
+    work = tempfile.mkdtemp(prefix="egtb_cpu_adb_", dir=TMPFS)
+    try:
+        prefix = os.path.join(work, f"{pattern}_{2**target}_")
+        t0 = time.perf_counter()
+        ref.run_build_ad(p["seeds"], p["masks"], p["symm_mode"], p["n_free_32k"], p["fixed_32k_shifts"], target,
+                         p["steps"], p["docheck_step"], prefix, is_free=p["is_free"],
+                         small_tile_sum_limit=p["small_tile_sum_limit"], threads=threads, stop_after_forward=False)
+        dt = time.perf_counter() - t0
+        n = 0
+        for i in range(p["steps"] - 2):
+            folder = f"{prefix}{i}b"
+            n += sum(os.path.getsize(os.path.join(folder, f)) // 4 for f in os.listdir(folder) if f.endswith(".b"))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return dict(value=round(n / dt, 1), unit="kept rates/s", cores=threads, kind="reference",
+                seconds=round(dt, 3),
+                sample=f"{pattern}@{2**target} advanced build (forward pass and solver) by the reference's "
+                       f"run_pattern_build_ad_cpp (g++ -Ofast, {threads} OpenMP threads, files on tmpfs): "
+                       f"{n} float32 rates kept in {dt:.1f} s")
+
+
+def advanced_build_leg(pkg, pattern, target, reps=2):
+    """Best wall time of the whole GPU advanced build (forward pass and
+    solver, float32 books in HBM and freed as the solve passes them, no
+    files)."""
+    adv = importlib.import_module("2048endgametablebase_amd.advanced")
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = adv.build_ad(pattern, target)
+        dt = time.perf_counter() - t0
+        info = r.solve_info()
+        rec = {"fwd_ms": round(r.fwd_ms, 2), "solve_ms": round(info["solve_ms"], 2), "states": int(r.sizes.sum()),
+               "rates": int(info["values"])}
+        r.close()
+        if best is None or dt < best[0]:
+            best = (dt, rec)
+    dt, rec = best
+    return dict({"workload": f"{pattern}@{2**target} advanced build (forward pass and solver), books in HBM",
+                 "ms": round(dt * 1000.0, 2)}, **rec, rates_per_s=round(rec["rates"] / dt, 1))
+
+
 def advanced_leg(pkg, pattern, target, reps=3):
     """Best wall time of the GPU advanced (masked) forward pass, raw layers
     kept only while needed, no files."""
@@ -416,7 +468,10 @@ def main():
                  "C4_444_2048_host_tables": build_leg(pkg, "444", 11, host_tables=True),
                  "advanced_442_256": advanced_leg(pkg, "442", 8),
                  "advanced_LL_64": advanced_leg(pkg, "LL", 6),
-                 "advanced_4442_64": advanced_leg(pkg, "4442", 6, reps=2)}
+                 "advanced_4442_64": advanced_leg(pkg, "4442", 6, reps=2),
+                 "advanced_build_442_256": advanced_build_leg(pkg, "442", 8),
+                 "advanced_build_LL_64": advanced_build_leg(pkg, "LL", 6),
+                 "advanced_build_free8_64": advanced_build_leg(pkg, "free8", 6)}
 
     cpu = None
     hostinfo = host_cpu()
@@ -443,6 +498,15 @@ def main():
                     cpu["advanced_forward"] = ad
             except Exception as exc:
                 cpu["advanced_forward"] = {"error": str(exc)}
+            try:
+                adb = cpu_baseline_advanced_build("442", 8, max(4, min(32, share)))
+                if adb is not None:
+                    gpu_s = extra["advanced_build_442_256"]["ms"] / 1000.0
+                    adb["gpu_seconds"] = round(gpu_s, 4)
+                    adb["gpu_over_cpu"] = round(adb["seconds"] / gpu_s, 1)  # same build, wall time ratio
+                    cpu["advanced_build"] = adb
+            except Exception as exc:
+                cpu["advanced_build"] = {"error": str(exc)}
 
     line = {
         "metric": METRIC,
