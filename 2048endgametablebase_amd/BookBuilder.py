@@ -290,11 +290,19 @@ def start_build(pattern: str, target: int, pathname: str) -> bool:
     """Build ``pattern`` at tile 2**target into files prefixed ``pathname``
     (engine_core/BookBuilder.py:264-317, classic path)."""
     cfg = SingletonConfig().config
-    if cfg.get("advanced_algo", False):
-        raise NotImplementedError("the advanced (masked) algorithm's solver is not part of this engine; "
-                                  "advanced.generate_ad writes its forward pass (raw layers)")
     if pattern not in pattern_catalog:
         raise KeyError(f"Unknown pattern: {pattern}")
+    if cfg.get("advanced_algo", False):
+        # _run_advanced_build (engine_core/BookBuilder.py:211-236): the
+        # advanced forward pass and solver into bucket folders <i>b
+        from .advanced import build_ad
+
+        save_config_to_txt(pathname + "config.txt")
+        with build_ad(pattern, target, pathname, success_rate_dtype=str(cfg.get("success_rate_dtype", "uint32")),
+                      spawn_rate4=float(cfg["4_spawn_rate"]), deletion_threshold=float(cfg.get("deletion_threshold", 0.0)),
+                      device=int(cfg.get("device", 0))):
+            pass
+        return True
     p = build_params(pattern, target)
     save_config_to_txt(pathname + "config.txt")
     dtype = str(cfg.get("success_rate_dtype", "uint32"))

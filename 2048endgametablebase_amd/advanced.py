@@ -226,8 +226,9 @@ def generate_ad(pattern: str, target: int, pathname: Optional[str] = None, **kw)
 
 
 def _main(argv=None) -> None:
-    """``python -m 2048endgametablebase_amd.advanced PATTERN LOG2_TARGET [--reps N]``:
-    time the forward pass (layers in HBM, no files) and print one JSON line."""
+    """``python -m 2048endgametablebase_amd.advanced PATTERN LOG2_TARGET [--reps N] [--solve]``:
+    time the forward pass (with --solve the whole build: forward pass and
+    solver), layers and books in HBM, no files, and print one JSON line."""
     import argparse
     import json
     import time
@@ -236,17 +237,23 @@ def _main(argv=None) -> None:
     ap.add_argument("pattern")
     ap.add_argument("target", type=int)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--solve", action="store_true")
     a = ap.parse_args(argv)
     times = []
+    extra = {}
     for _ in range(a.reps):
         t0 = time.perf_counter()
-        r = generate_ad(a.pattern, a.target)
+        r = build_ad(a.pattern, a.target) if a.solve else generate_ad(a.pattern, a.target)
         times.append(time.perf_counter() - t0)
         n, c = int(r.sizes.sum()), int(r.candidates)
+        if a.solve:
+            extra = {"fwd_ms": round(r.fwd_ms, 2), "solve_ms": round(r.solve_info()["solve_ms"], 2),
+                     "rates": r.solve_info()["values"]}
         r.close()
-    print(json.dumps({"workload": f"{a.pattern}@{2 ** a.target} advanced forward pass", "states": n,
-                      "boards_before_dedup": c, "ms": [round(t * 1000.0, 2) for t in times],
-                      "states_per_s": round(n / min(times), 1)}))
+    what = "advanced build (forward pass and solver)" if a.solve else "advanced forward pass"
+    print(json.dumps(dict({"workload": f"{a.pattern}@{2 ** a.target} {what}", "states": n,
+                           "boards_before_dedup": c, "ms": [round(t * 1000.0, 2) for t in times],
+                           "states_per_s": round(n / min(times), 1)}, **extra)))
 
 
 if __name__ == "__main__":

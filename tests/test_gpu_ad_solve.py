@@ -108,3 +108,26 @@ def test_solve_resumes_from_folders(pkg):
         _check_files(prefix, g, "442_128 resumed")
     finally:
         shutil.rmtree(work, ignore_errors=True)
+
+
+def test_start_build_advanced_writes_reference_books(pkg):
+    """start_build with advanced_algo runs the whole advanced build, as the
+    reference's start_build -> _run_advanced_build does."""
+    g = load_ad_book_golden("442_64")
+    print("case advanced start_build 442_64", flush=True)
+    cfg = pkg.config.SingletonConfig().config
+    saved = {k: cfg.get(k) for k in ("advanced_algo", "success_rate_dtype", "deletion_threshold")}
+    work = tempfile.mkdtemp(prefix="egtb_adsolve_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        cfg.update(advanced_algo=True, success_rate_dtype="float32", deletion_threshold=0.0)
+        prefix = os.path.join(work, "442_64_")
+        assert pkg.BookBuilder.start_build("442", 6, prefix) is True
+        _check_files(prefix, g, "start_build 442_64")
+        assert os.path.exists(prefix + "config.txt")
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                cfg.pop(k, None)
+            else:
+                cfg[k] = v
+        shutil.rmtree(work, ignore_errors=True)
