@@ -36,6 +36,9 @@ def lib():
         L.orc_ad_derive.argtypes = [ctypes.c_uint64, ctypes.c_uint32, P, I, I, I, P, I, I, I, P, P]
         L.orc_ad_validate.restype = I
         L.orc_ad_validate.argtypes = [ctypes.c_uint64, ctypes.c_uint32, P, I, I, I, P, I, I, I]
+        L.orc_ad_build.restype = I
+        L.orc_ad_build.argtypes = [P, ctypes.c_size_t, P, I, I, I, P, I, I, I, I, ctypes.c_char_p,
+                                   ctypes.c_double]
         _lib = L
     return _lib
 
@@ -60,6 +63,18 @@ def forward(p, pathname: str, steps: int | None = None) -> np.ndarray:
     if rc != 0:
         raise OSError(f"oracle AD forward pass could not write under {pathname}")
     return sizes[:max(steps - 1, 0)]
+
+
+def build(p, pathname: str, p4: float = 0.1) -> None:
+    """The whole advanced build in float32 (forward pass, then the solver of
+    layers steps-3 .. 0) as bucket files <pathname><i>b/<key>.i|.b."""
+    seeds = np.ascontiguousarray(p["seeds"], dtype=np.uint64)
+    masks, nm, fixed, nf = _spec(p)
+    rc = lib().orc_ad_build(seeds.ctypes.data, seeds.size, masks.ctypes.data, nm, int(p["symm_mode"]),
+                            int(p["n_free_32k"]), fixed.ctypes.data, nf, int(p["small_tile_sum_limit"]),
+                            int(p["target"]), int(p["steps"]), pathname.encode(), float(p4))
+    if rc != 0:
+        raise OSError(f"oracle AD build could not read or write under {pathname}")
 
 
 def row_left(row: int) -> int:

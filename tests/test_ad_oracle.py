@@ -165,3 +165,32 @@ def _check_forward(pattern, target, n_layers=None, limit=96):
                                                            ("442", 7, 50, 64), ("free8", 6, 16, 96)])
 def test_forward_matches_reference(pattern, target, n_layers, limit):
     _check_forward(pattern, target, n_layers, limit)
+
+
+@pytest.mark.parametrize("tag", ["442_64", "t_64"])
+def test_oracle_solver_matches_reference_books(tag, tmp_path):
+    """The oracle's restatement of the advanced solver (float32) writes every
+    bucket file of every solved layer byte-equal to the reference's own
+    solver (tests/golden/ad_book_<tag>.json): one-rate buckets (442@64) and
+    rows of three rates over two free 32768 tiles (t@64)."""
+    import hashlib
+    import importlib
+
+    from conftest import load_ad_book_golden
+    from oracle import ad_oracle
+
+    patterns = importlib.import_module("2048endgametablebase_amd.patterns")
+    g = load_ad_book_golden(tag)
+    p = patterns.ad_params(g["pattern"], g["target"], g["small_tile_sum_limit"])
+    pre = str(tmp_path / "x_")
+    ad_oracle.build(p, pre, g["spawn_rate4"])
+    for layer in g["layers"]:
+        folder = f"{pre}{layer['i']}b"
+        assert sorted(int(f[:-2]) for f in os.listdir(folder) if f.endswith(".i")) == \
+            [b["key"] for b in layer["buckets"]], layer["i"]
+        for b in layer["buckets"]:
+            base = os.path.join(folder, str(b["key"]))
+            with open(base + ".i", "rb") as fh:
+                assert hashlib.sha256(fh.read()).hexdigest() == b["i_sha"], (layer["i"], b["key"])
+            with open(base + ".b", "rb") as fh:
+                assert hashlib.sha256(fh.read()).hexdigest() == b["b_sha"], (layer["i"], b["key"])
