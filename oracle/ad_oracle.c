@@ -950,13 +950,14 @@ static void layer_free(slayer *L) {
 
 static int write_bucket_files(const char *folder, const slayer *L) {
     mkdir(folder, 0777);
-    char path[4200];
+    char path[4400];
+    if (strlen(folder) > 4300) return -1;
     for (int k = 0; k < N_KEYS; ++k) {
         const sbucket *B = &L->b[k];
         if (!B->rows) continue;
-        snprintf(path, sizeof path, "%s/%d.i", folder, k + KEY_MIN);
+        if (snprintf(path, sizeof path, "%.4300s/%d.i", folder, k + KEY_MIN) >= (int)sizeof path) return -1;
         if (write_raw(path, B->ind, B->rows)) return -1;
-        snprintf(path, sizeof path, "%s/%d.b", folder, k + KEY_MIN);
+        if (snprintf(path, sizeof path, "%.4300s/%d.b", folder, k + KEY_MIN) >= (int)sizeof path) return -1;
         FILE *f = fopen(path, "wb");
         if (!f) return -1;
         fwrite(B->rates, sizeof(float), B->rows * B->cols, f);
