@@ -499,7 +499,7 @@ ORC_API int orc_ad_validate(uint64_t b, uint32_t board_sum, const uint64_t *mask
 }
 
 /* ------------------------------------------------------------------ */
-/* Solver (BookSolverAD.cpp), float32 rates                            */
+/* Solver (BookSolverAD.cpp): float32, float64 or uint32 rates         */
 /* ------------------------------------------------------------------ */
 /*
  * recalculate_process_ad_impl (:2086-2242) over the raw layers the forward
@@ -507,7 +507,10 @@ ORC_API int orc_ad_validate(uint64_t b, uint32_t board_sum, const uint64_t *mask
  * <pathname><i>b/<key>.i / <key>.b (dict_tofile, :320-352).  Pinned by the
  * per-bucket digests of the reference's own solver (tests/golden/
  * ad_book_*.json).  Serial, one board at a time, the placements' column
- * order computed for each move (the reference memoises it).
+ * order computed for each move (the reference memoises it).  Rates are held
+ * as doubles, which hold every float32 and uint32 value exactly, and take
+ * the rate type's rounding where the reference converts (one conversion per
+ * stored rate).
  */
 #include <math.h>
 #include <sys/stat.h>
@@ -517,10 +520,19 @@ ORC_API int orc_ad_validate(uint64_t b, uint32_t board_sum, const uint64_t *mask
 
 typedef struct {
     uint64_t *ind;
-    float *rates;
+    double *rates;
     size_t rows;
     uint32_t cols;
 } sbucket;
+
+/* the rate type: 0 float32, 1 float64, 2 uint32 (scale 4e9) */
+static int g_rt = 0;
+static double rt_max(void) { return g_rt == 2 ? 4000000000.0 : 1.0; }
+static double rt_store(double x) {  /* static_cast<T>(double) */
+    if (g_rt == 0) return (double)(float)x;
+    if (g_rt == 2) return (double)(uint32_t)x;
+    return x;
+}
 typedef struct {
     sbucket b[N_KEYS];
 } slayer;
@@ -769,7 +781,7 @@ static void ranks(const uint64_t *b, int n, int *r) {
     }
 }
 
-static const float *find_row(const slayer *L, int key, uint64_t board, uint32_t *cols) {
+static const double *find_row(const slayer *L, int key, uint64_t board, uint32_t *cols) {
     if (key < KEY_MIN || key >= KEY_MIN + N_KEYS) return NULL;
     const sbucket *B = &L->b[key - KEY_MIN];
     size_t lo = 0, hi = B->rows;
@@ -824,9 +836,9 @@ static int map364(int x, int y, int n) {
 
 /* solve_optimal_success_rate(_arr_into) for one spawned board: osr[0..d) */
 static void solve_spawn(uint64_t b, uint64_t v, int pos, uint32_t sum, int key, int d, const slayer *L,
-                        const ad_spec *s, float *osr) {
+                        const ad_spec *s, double *osr) {
     const uint64_t bs = b | (v << (4 * pos));
-    for (int j = 0; j < d; ++j) osr[j] = 0.0f;
+    for (int j = 0; j < d; ++j) osr[j] = 0.0;
     uint64_t mv[4];
     int mnt[4];
     ad_move4(bs, mv, mnt);
@@ -844,7 +856,7 @@ static void solve_spawn(uint64_t b, uint64_t v, int pos, uint32_t sum, int key, 
         ranks(D, nd, R);
         uint32_t cols = 0;
         if (!mnt[dir]) {
-            const float *row = find_row(L, key, canon, &cols);
+            const double *row = find_row(L, key, canon, &cols);
             if (!row) continue;
             if (d == 1) { osr[0] = osr[0] < row[0] ? row[0] : osr[0]; continue; }
             for (int i = 0; i < nd && i < d; ++i)
@@ -855,10 +867,10 @@ static void solve_spawn(uint64_t b, uint64_t v, int pos, uint32_t sum, int key, 
         if (key > 15) {  /* update_mnt_osr_364_ad / _arr_ad */
             int p64, p128, prank, pos64;
             find3x64(un, s, &p64, &p128, &prank, &pos64);
-            const float *row = find_row(L, prank, canon | (0xFull << pos64), &cols);
+            const double *row = find_row(L, prank, canon | (0xFull << pos64), &cols);
             if (!row) continue;
             if (d == 1) {
-                const float x = row[map364(p64, p128, prank)];
+                const double x = row[map364(p64, p128, prank)];
                 osr[0] = osr[0] < x ? x : osr[0];
                 continue;
             }
@@ -874,7 +886,7 @@ static void solve_spawn(uint64_t b, uint64_t v, int pos, uint32_t sum, int key, 
         const tstats t = stats_large(un, s);
         const int success = t.merged_tile == (uint32_t)s->target && t.merged_found == 1;
         if (success) {
-            for (int j = 0; j < d; ++j) osr[j] = 1.0f;
+            for (int j = 0; j < d; ++j) osr[j] = rt_max();
             continue;
         }
         int nt = 0;
@@ -893,14 +905,14 @@ static void solve_spawn(uint64_t b, uint64_t v, int pos, uint32_t sum, int key, 
                     board = (canon & ~t.pos_bitmap) | (((uint64_t)c[0] * 0x1111111111111111ull) & t.pos_bitmap);
                 }
             }
-            const float *row = find_row(L, bk, board, &cols);
+            const double *row = find_row(L, bk, board, &cols);
             if (!row) continue;
-            const float x = (bk > 0 && bk <= 15) ? row[t.pos_rank] : row[0];
+            const double x = (bk > 0 && bk <= 15) ? row[t.pos_rank] : row[0];
             osr[0] = osr[0] < x ? x : osr[0];
             continue;
         }
         if (nt > 2 && c[0] == c[1] && c[0] == c[2]) {  /* arr3 */
-            const float *row = find_row(L, t.count + 13, un, &cols);
+            const double *row = find_row(L, t.count + 13, un, &cols);
             if (!row) continue;
             for (int i = 0; i < nd && i < d; ++i)
                 if (row[R[i]] > osr[i]) osr[i] = row[R[i]];
@@ -913,7 +925,7 @@ static void solve_spawn(uint64_t b, uint64_t v, int pos, uint32_t sum, int key, 
                 const uint64_t ubj = (un & ~(0xFull << lp[j])) | ((uint64_t)c[0] << lp[j]);
                 int si2;
                 const uint64_t cb = sym_pair(ubj, s->symm, &si2);
-                const float *row = find_row(L, -t.count, cb, &cols);
+                const double *row = find_row(L, -t.count, cb, &cols);
                 if (!row) continue;
                 uint64_t mb[MAXD];
                 int mi[MAXD], nm = 0, R2[MAXD];
@@ -927,7 +939,7 @@ static void solve_spawn(uint64_t b, uint64_t v, int pos, uint32_t sum, int key, 
                     if (row[R2[q]] > osr[mi[q]]) osr[mi[q]] = row[R2[q]];
             }
         } else {  /* arr2 */
-            const float *row = find_row(L, t.count, canon, &cols);
+            const double *row = find_row(L, t.count, canon, &cols);
             if (!row) continue;
             const pslot *p = &g_perm[(t.count << 4) + (t.count - s->n_free)];
             int sub[MAXD], ns = 0;
@@ -960,17 +972,31 @@ static int write_bucket_files(const char *folder, const slayer *L) {
         if (snprintf(path, sizeof path, "%.4300s/%d.b", folder, k + KEY_MIN) >= (int)sizeof path) return -1;
         FILE *f = fopen(path, "wb");
         if (!f) return -1;
-        fwrite(B->rates, sizeof(float), B->rows * B->cols, f);
+        const size_t m = B->rows * B->cols;
+        for (size_t i = 0; i < m; ++i) {  /* the rate type's bytes, little-endian */
+            if (g_rt == 0) {
+                const float v = (float)B->rates[i];
+                fwrite(&v, sizeof v, 1, f);
+            } else if (g_rt == 2) {
+                const uint32_t v = (uint32_t)B->rates[i];
+                fwrite(&v, sizeof v, 1, f);
+            } else {
+                fwrite(&B->rates[i], sizeof(double), 1, f);
+            }
+        }
         fclose(f);
     }
     return 0;
 }
 
-/* The whole advanced build, float32: the forward pass (orc_ad_forward) and
- * the solve of layers steps-3 .. 0 into <pathname><i>b.  Returns 0, or -1. */
+/* The whole advanced build: the forward pass (orc_ad_forward) and the solve
+ * of layers steps-3 .. 0 into <pathname><i>b, rate type rate_type (0
+ * float32, 1 float64, 2 uint32).  Returns 0, or -1. */
 ORC_API int orc_ad_build(const uint64_t *seeds, size_t n_seeds, const uint64_t *masks, int n_masks, int symm,
                          int n_free, const uint8_t *fixed_shifts, int n_fixed, int limit, int target, int steps,
-                         const char *pathname, double p4) {
+                         const char *pathname, double p4, int rate_type) {
+    if (rate_type < 0 || rate_type > 2) return -1;
+    g_rt = rate_type;
     if (orc_ad_forward(seeds, n_seeds, masks, n_masks, symm, n_free, fixed_shifts, n_fixed, limit, target, steps,
                        pathname, NULL))
         return -1;
@@ -1016,7 +1042,7 @@ ORC_API int orc_ad_build(const uint64_t *seeds, size_t n_seeds, const uint64_t *
             const int a = key < 0 ? -key - 2 : key > 15 ? key - 16 : key;
             B->cols = fact(a) / fact(n_free);
             B->ind = malloc(B->rows * 8 + 8);
-            B->rates = malloc(B->rows * B->cols * sizeof(float) + 8);
+            B->rates = malloc(B->rows * B->cols * sizeof(double) + 8);
             B->rows = 0;
         }
         for (size_t i = 0; i < n; ++i)
@@ -1033,7 +1059,7 @@ ORC_API int orc_ad_build(const uint64_t *seeds, size_t n_seeds, const uint64_t *
             for (size_t r = 0; r < B->rows; ++r) {
                 const uint64_t b = B->ind[r];
                 double sp[MAXD];
-                float o2[MAXD], o4[MAXD];
+                double o2[MAXD], o4[MAXD];
                 int empty = 0;
                 for (int j = 0; j < d; ++j) sp[j] = 0.0;
                 for (int pos = 0; pos < 16; ++pos) {
@@ -1042,17 +1068,17 @@ ORC_API int orc_ad_build(const uint64_t *seeds, size_t n_seeds, const uint64_t *
                     solve_spawn(b, 1, pos, sum + 2u, key, d, &L1, &s, o2);
                     solve_spawn(b, 2, pos, sum + 4u, key, d, &L2, &s, o4);
                     if (d == 1) {  /* -Ofast: sum + fma(r2, 1 - p4, r4 * p4) */
-                        sp[0] = sp[0] + fma((double)o2[0], w2, (double)o4[0] * w4);
+                        sp[0] = sp[0] + fma(o2[0], w2, o4[0] * w4);
                     } else {       /* per spawn: fma(r, w, sum) */
-                        for (int j = 0; j < d; ++j) sp[j] = fma((double)o2[j], w2, sp[j]);
-                        for (int j = 0; j < d; ++j) sp[j] = fma((double)o4[j], w4, sp[j]);
+                        for (int j = 0; j < d; ++j) sp[j] = fma(o2[j], w2, sp[j]);
+                        for (int j = 0; j < d; ++j) sp[j] = fma(o4[j], w4, sp[j]);
                     }
                 }
-                float *dst = B->rates + r * d;
-                if (d == 1) dst[0] = empty ? (float)(sp[0] / (double)empty) : 0.0f;
+                double *dst = B->rates + r * d;
+                if (d == 1) dst[0] = empty ? rt_store(sp[0] / (double)empty) : 0.0;
                 else {
                     const double inv = empty ? 1.0 / (double)empty : 0.0;
-                    for (int j = 0; j < d; ++j) dst[j] = empty ? (float)(sp[j] * inv) : 0.0f;
+                    for (int j = 0; j < d; ++j) dst[j] = empty ? rt_store(sp[j] * inv) : 0.0;
                 }
             }
         }
@@ -1063,10 +1089,10 @@ ORC_API int orc_ad_build(const uint64_t *seeds, size_t n_seeds, const uint64_t *
             size_t m = 0;
             for (size_t r = 0; r < B->rows; ++r) {
                 int keep = 0;
-                for (uint32_t j = 0; j < B->cols; ++j) keep |= B->rates[r * B->cols + j] > 0.0f;
+                for (uint32_t j = 0; j < B->cols; ++j) keep |= B->rates[r * B->cols + j] > 0.0;
                 if (!keep) continue;
                 B->ind[m] = B->ind[r];
-                memmove(B->rates + m * B->cols, B->rates + r * B->cols, B->cols * sizeof(float));
+                memmove(B->rates + m * B->cols, B->rates + r * B->cols, B->cols * sizeof(double));
                 ++m;
             }
             B->rows = m;

@@ -38,7 +38,7 @@ def lib():
         L.orc_ad_validate.argtypes = [ctypes.c_uint64, ctypes.c_uint32, P, I, I, I, P, I, I, I]
         L.orc_ad_build.restype = I
         L.orc_ad_build.argtypes = [P, ctypes.c_size_t, P, I, I, I, P, I, I, I, I, ctypes.c_char_p,
-                                   ctypes.c_double]
+                                   ctypes.c_double, I]
         _lib = L
     return _lib
 
@@ -65,14 +65,19 @@ def forward(p, pathname: str, steps: int | None = None) -> np.ndarray:
     return sizes[:max(steps - 1, 0)]
 
 
-def build(p, pathname: str, p4: float = 0.1) -> None:
-    """The whole advanced build in float32 (forward pass, then the solver of
-    layers steps-3 .. 0) as bucket files <pathname><i>b/<key>.i|.b."""
+RATE_TYPES = {"float32": 0, "float64": 1, "uint32": 2}
+
+
+def build(p, pathname: str, p4: float = 0.1, dtype: str = "float32") -> None:
+    """The whole advanced build (forward pass, then the solver of layers
+    steps-3 .. 0) as bucket files <pathname><i>b/<key>.i|.b, rates of dtype
+    float32, float64 or uint32."""
     seeds = np.ascontiguousarray(p["seeds"], dtype=np.uint64)
     masks, nm, fixed, nf = _spec(p)
     rc = lib().orc_ad_build(seeds.ctypes.data, seeds.size, masks.ctypes.data, nm, int(p["symm_mode"]),
                             int(p["n_free_32k"]), fixed.ctypes.data, nf, int(p["small_tile_sum_limit"]),
-                            int(p["target"]), int(p["steps"]), pathname.encode(), float(p4))
+                            int(p["target"]), int(p["steps"]), pathname.encode(), float(p4),
+                            RATE_TYPES[dtype])
     if rc != 0:
         raise OSError(f"oracle AD build could not read or write under {pathname}")
 

@@ -167,12 +167,13 @@ def test_forward_matches_reference(pattern, target, n_layers, limit):
     _check_forward(pattern, target, n_layers, limit)
 
 
-@pytest.mark.parametrize("tag", ["442_64", "t_64"])
+@pytest.mark.parametrize("tag", ["442_64", "t_64", "442_128_uint32"])
 def test_oracle_solver_matches_reference_books(tag, tmp_path):
-    """The oracle's restatement of the advanced solver (float32) writes every
-    bucket file of every solved layer byte-equal to the reference's own
-    solver (tests/golden/ad_book_<tag>.json): one-rate buckets (442@64) and
-    rows of three rates over two free 32768 tiles (t@64)."""
+    """The oracle's restatement of the advanced solver writes every bucket
+    file of every solved layer byte-equal to the reference's own solver
+    (tests/golden/ad_book_<tag>.json): one-rate buckets (442@64), rows of
+    three rates over two free 32768 tiles (t@64), and uint32 rates with
+    spelled-out pairs (442@128)."""
     import hashlib
     import importlib
 
@@ -183,7 +184,7 @@ def test_oracle_solver_matches_reference_books(tag, tmp_path):
     g = load_ad_book_golden(tag)
     p = patterns.ad_params(g["pattern"], g["target"], g["small_tile_sum_limit"])
     pre = str(tmp_path / "x_")
-    ad_oracle.build(p, pre, g["spawn_rate4"])
+    ad_oracle.build(p, pre, g["spawn_rate4"], g["dtype"])
     for layer in g["layers"]:
         folder = f"{pre}{layer['i']}b"
         assert sorted(int(f[:-2]) for f in os.listdir(folder) if f.endswith(".i")) == \
